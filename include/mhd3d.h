@@ -1,0 +1,167 @@
+/*
+ * mhd3d.h — C ABI of the MI355X (gfx950) RK3 timestep for 3-D compressible
+ * (magneto)hydrodynamics on a z-stretched grid.
+ *
+ * The reference program (thu-scc/Rast-3dmhd-cpp) keeps its state in global
+ * arrays `double RU[nz][ny][nx], ...` (3dmhddata.h:8,12-19) and advances it
+ * with `void step()` (3dmhdsub.h:31), which calls `fluxes()` (3dmhdsub.h:194),
+ * `step_block_ZETA0/ZETA/GAM` (3dmhdsub.h:155-192), `BCON()` (3dmhdsub.h:720)
+ * and `communicate()` (3dmhdmpi.h:11).  This header is the drop-in boundary for
+ * that call surface: the caller hands over the same [nz][ny][nx] x-contiguous
+ * FP64 arrays (1-cell ghost layer on every side) and the same scalars that
+ * `set()` (3dmhd.cpp:353-495) leaves in the globals, and gets the same arrays
+ * back.  Every entry point returns an int status (MHD3D_OK = 0).
+ *
+ * One context = one domain on one GPU with its own HIP stream.  A context is
+ * not thread-safe; independent contexts may be used from different threads.
+ */
+#ifndef MHD3D_H_
+#define MHD3D_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field slots, in the order of 3dmhddata.h:12-16 (RU,RV,RW,RO,TT) and
+ * 3dmhddata.h:17 (BX,BY,BZ).  Flux slots use the same numbering:
+ * FU,FV,FW,FR,FT (3dmhddata.h:14) and WW1,WW2,WW3 = dB/dt (3dmhdsub.h:548-694). */
+enum {
+    MHD3D_RU = 0, MHD3D_RV = 1, MHD3D_RW = 2, MHD3D_RO = 3, MHD3D_TT = 4,
+    MHD3D_BX = 5, MHD3D_BY = 6, MHD3D_BZ = 7, MHD3D_NFIELD = 8
+};
+
+/* Status codes. */
+enum {
+    MHD3D_OK = 0,
+    MHD3D_EINVAL = 1,       /* bad pointer, size or descriptor value            */
+    MHD3D_EUNSUPPORTED = 2, /* flag combination this build does not implement   */
+    MHD3D_EHIP = 3,         /* a HIP runtime call failed (see mhd3d_last_error) */
+    MHD3D_ENOMEM = 4,       /* device allocation failed                         */
+    MHD3D_ENONFINITE = 5    /* state holds NaN/Inf after the step (the DT guard
+                               of 3dmhd.cpp:292 cannot see it: minVal ignores NaN) */
+};
+
+/* Arithmetic form of the stencil evaluation. */
+enum {
+    MHD3D_FP64 = 0,   /* plain FP64, same operation order as the reference   */
+    MHD3D_MIXED = 1   /* FP32 stencils from FP32 copies, FP64 RK registers,
+                         FP64 CFL and diagnostics                             */
+};
+
+/*
+ * Run descriptor.  Sizes follow 3dmhdparam.h:25-47 for ONE domain:
+ *   nPx, nPy   interior points in x and y (x, y periodic),
+ *   nPz        z planes owned by the domain including both walls,
+ * so each array is [nPz+2][nPy+2][nPx+2] doubles (x fastest).
+ * Flags are the compile-time switches of 3dmhdparam.h:8-17,86-118, here at
+ * run time.  Scalars are the doubles `set()` computes (3dmhd.cpp:353-495):
+ * the caller passes them, so float-literal roundings (C13 = 1.0f/3.0f, ...)
+ * stay exactly those of the host program.
+ */
+typedef struct mhd3d_desc {
+    int32_t nPx, nPy, nPz;
+    int32_t lRot, lMag, lPot, lRem, lShr;        /* 3dmhdparam.h:8-17         */
+    int32_t ixCon, iyCon, izCon, iTCon, iBCon;   /* 3dmhdparam.h:86-107       */
+    int32_t iD;                                  /* 3dmhdparam.h:118          */
+    int32_t precision;                           /* MHD3D_FP64 / MHD3D_MIXED  */
+    int32_t reserved;
+    /* GRID / RUNGKU / SFF (3dmhd.cpp:354-369, 3dmhdparam.h:111) */
+    double C13, C23, C43;
+    double GAM1, GAM2, GAM3, ZETA1, ZETA2;
+    double SFF;
+    double hx, h2x, hy, h2y, hz, h2z;
+    double ORZF;        /* (double)(1.0f/float(nPz_global-1)): DZ in BCON, 3dmhdsub.h:797,849 */
+    /* CPAR / CROT / CMAG / CPEN / CPER (3dmhd.cpp:144-145,401-459) */
+    double RE, ORE, REPR, CV, OCV, GAMMA, THETA, GRAV;
+    double OMX, OMZ;
+    double RM, ORM, OBETA;
+    double TP, TB, TC;
+    double RLAX;        /* must be 0: relaxation (3dmhdsub.h:322-442) is not built  */
+    double TIMI;        /* initial simulation time (3dmhd.cpp:114)                   */
+    /* AJACOBI (3dmhddata.h:20-22): lengths nPx+2, nPy+2, nPz+2 */
+    const double *dxxdx, *d2xxdx2, *DDX;
+    const double *dyydy, *d2yydy2, *DDY;
+    const double *dzzdz, *d2zzdz2, *DDZ;
+    /* CPEN (3dmhddata.h:32): length nPz+2 */
+    const double *RKAPA, *DKAPA;
+} mhd3d_desc;
+
+typedef struct mhd3d_ctx mhd3d_ctx;
+
+/* CFL record of one TSTEP evaluation (3dmhdsub.h:36-106). */
+typedef struct mhd3d_cfl_info {
+    double wmin[4];   /* minVal(WW1), minVal(WW2), minVal(WW3), minVal(VV) (lMag) */
+    double umach;     /* UMACH (3dmhdsub.h:42-44)                                 */
+    double dt;        /* SFF * min(...) (3dmhdsub.h:101-105)                      */
+} mhd3d_cfl_info;
+
+/* Kernel timing gathered with HIP events on the context stream. */
+typedef struct mhd3d_timing {
+    double stage_ms_total;   /* summed duration of the stage kernels           */
+    int64_t stage_launches;  /* number of stage-kernel launches timed          */
+    double step_ms_total;    /* summed duration of whole steps                 */
+    int64_t steps;
+} mhd3d_timing;
+
+/* Create a context: validates the descriptor, copies the metric arrays and
+ * allocates the device registers (U ping-pong + Z).  Replaces the static
+ * storage of 3dmhd.cpp:24-79 and the setup half of main(). */
+int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out);
+int mhd3d_destroy(mhd3d_ctx* ctx);
+
+/* Upload / download the state.  `fields[f]` points at a host array
+ * [nPz+2][nPy+2][nPx+2]; the magnetic slots are read only with lMag.
+ * Upload takes the interior (x,y) of every plane including the two z ghost
+ * planes; download writes the interior and refills the x/y ghosts periodically
+ * as communicate() does (3dmhdmpi.h:59-142).  Replaces direct access to the
+ * globals of 3dmhddata.h:12-17. */
+int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields);
+int mhd3d_get_state(mhd3d_ctx* ctx, double* const* fields);
+
+/* Advance `nsteps` RK3 steps: step() of 3dmhdsub.h:31-153, nsteps times.
+ * fixed_dt == 0: each step computes DT from the CFL condition (TSTEP) and, on
+ * return, *dt_inout holds the DT of the last step.  fixed_dt != 0: every step
+ * uses *dt_inout.  Stops early (and returns MHD3D_ENONFINITE) when the state
+ * stops being finite. */
+int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout);
+
+/* fluxes() of 3dmhdsub.h:194-696 on the current state, written to host
+ * arrays in the field layout: FU,FV,FW,FR,FT and, with lMag, WW1,WW2,WW3
+ * (interior incl. walls; ghosts zero).  Pure: the state is not modified
+ * (the reference inverts RO in place and uses RU..TT as scratch). */
+int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux);
+
+/* One RK3 stage in isolation, for parity tests of the fused update:
+ * stage 1..3 of 3dmhdsub.h:116-146 (ZETA/fluxes/GAM/BCON/communicate). */
+int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt);
+
+/* BCON() (3dmhdsub.h:720-873) applied to the current state. */
+int mhd3d_bcon(mhd3d_ctx* ctx);
+
+/* TSTEP block of step() (3dmhdsub.h:36-106) on the current state. */
+int mhd3d_cfl(mhd3d_ctx* ctx, mhd3d_cfl_info* out);
+
+/* Global diagnostics in FP64: total mass sum(RO), total energy
+ * sum(0.5*|RU,RV,RW|^2/RO + CV*RO*TT [+ OBETA*|B|^2/2]) and max|div B|
+ * over the interior including walls (no metric volume weights). */
+int mhd3d_diagnostics(mhd3d_ctx* ctx, double out[3]);
+
+/* Simulation clock: nIt, TIMT, TIMC (3dmhdsub.h:150-152). */
+int mhd3d_clock(mhd3d_ctx* ctx, int64_t* nIt, double* timt, double* timc);
+
+/* Event timing of the stage kernels (enable before the timed region). */
+int mhd3d_timing_enable(mhd3d_ctx* ctx, int on);
+int mhd3d_timing_read(mhd3d_ctx* ctx, mhd3d_timing* out);
+
+int mhd3d_synchronize(mhd3d_ctx* ctx);
+/* Text of the last error on this context (or of the last failed create). */
+const char* mhd3d_last_error(const mhd3d_ctx* ctx);
+/* Bytes of device memory the context holds. */
+int64_t mhd3d_device_bytes(const mhd3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHD3D_H_ */

@@ -1,0 +1,71 @@
+/*
+ * mhd_oracle.h — CPU restatement of the reference RK3 timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  The product (libmhd3d.so) never links or calls it.
+ *
+ * It restates, sweep by sweep and in the same operation order, the reference
+ * files:
+ *   3dmhdsub.h:31-153   step(): TSTEP/CFL, three Wray RK3 stages
+ *   3dmhdsub.h:155-192  step_block_ZETA0 / ZETA / GAM
+ *   3dmhdsub.h:194-696  fluxes() (default branches: iD=0, lRem=0, RLAX=0, iTCon!=2)
+ *   3dmhdsub.h:720-873  BCON()
+ *   3dmhdutils.h:45-239 operator templates and reductions
+ *   3dmhdmpi.h:32-143   ghost refill, single domain (x, y periodic)
+ *   3dmhdphsic.h:230-355,523-547,549-610  bsstep/mmid/rzextr/derivs, RAN2, Static
+ * The arrays are the reference's [nz][ny][nx] with a 1-cell ghost layer.
+ * Domain = one z slab holding both walls: the one-sided wall stencils of both
+ * walls apply (the reference's nPEz>=2 behaviour, 3dmhdsub.h:211-226,572-584).
+ *
+ * Pinning (which check fixes which function) is listed in oracle/README.md.
+ */
+#ifndef MHD_ORACLE_H_
+#define MHD_ORACLE_H_
+
+#include "../include/mhd3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx* orc_create(const mhd3d_desc* d);
+void orc_destroy(orc_ctx* c);
+void orc_set_threads(int n);
+
+/* Named arrays of the reference (3dmhddata.h:12-19): "RU","RV","RW","RO",
+ * "TT","BX","BY","BZ","UU","VV","WW","FU","FV","FW","FR","FT","WW1","WW2",
+ * "WW3","ZRU",...,"ZBZ".  Full [nz][ny][nx] copies. Returns 0 or -1. */
+int orc_set(orc_ctx* c, const char* name, const double* a);
+int orc_get(orc_ctx* c, const char* name, double* a);
+
+void orc_communicate(orc_ctx* c);      /* 3dmhdmpi.h:11-143, single domain      */
+void orc_fluxes(orc_ctx* c);           /* 3dmhdsub.h:194-696                     */
+void orc_bcon(orc_ctx* c);             /* 3dmhdsub.h:720-873                     */
+/* TSTEP (3dmhdsub.h:36-106): sets UU, WW1..3 (, VV) like the reference,
+ * writes wmin[0..3], umach; returns DT. */
+double orc_cfl(orc_ctx* c, double wmin[4], double* umach);
+/* One stage k=1..3 of step() (3dmhdsub.h:116-147) with the given DT. */
+void orc_stage(orc_ctx* c, int k, double dt);
+/* step() (3dmhdsub.h:31-153); fixed_dt!=0 skips TSTEP and uses dt. Returns DT. */
+double orc_step(orc_ctx* c, int fixed_dt, double dt);
+void orc_clock(orc_ctx* c, long long* nIt, double* timt, double* timc);
+
+/* RAN2 (3dmhdphsic.h:523-547).  state = {IDUM, IIY, IIR[97]} (99 ints). */
+double orc_ran2(int* state);
+/* Bulirsch-Stoer polytrope of Static() (3dmhdphsic.h:596-609) on the uniform
+ * grid (nGrid=0, PZP=0): T[k], R[k] for k=0..nPz-1.  Returns 0 or -1. */
+int orc_static_profile(int nPz, double THETA, double GRAV, double zMax,
+                       double* T, double* R);
+/* RND planes of Static() (3dmhdphsic.h:705-744) for an nPEy x nPEz rank grid,
+ * written into the single-domain array `out` [nzg][nyg][nxg] (interior only):
+ * out = 1 + AMPT*(RAN2-0.5), drawn in the reference's rank order. */
+void orc_static_rnd(int nPx, int nPy, int nRz_total, int nPEy, int nPEz,
+                    double AMPT, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,81 @@
+"""Published known answers of the reference and the initial states they need.
+
+Values are read off the reference's screenshots of its own stdout
+(assets/diff.iter9.png, README.md:198-204: default build, iteration 9 then the
+start of iteration 10; assets/diff.iter9.lRot.png right-hand C++ column,
+README.md:206-270: same with lRot, par[4]=10, par[5]=0.25).  The printed point
+is rank-0 local [10][10][10] of a 4x6 rank run (nPx=nPy=nPz=256), i.e. [10][10][10]
+of the single 256x256x252 domain built by :func:`default_state`.
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "rast-3dmhd-cpp_amd"))
+sys.path.insert(0, ROOT)
+
+from mhd3d.desc import RunConfig, Descriptor, default_par  # noqa: E402
+
+# state after 9 steps, stage-3 fluxes of step 9, CFL values of step 10
+KA_DEFAULT = {
+    "RU": -2.85266123295513E-05, "RV": 1.47226777459164E-05, "RW": -1.12572508160305E-06,
+    "RO": 1.35293233493549E+00, "TT": 1.35293016916220E+00,
+    "UU": 2.25495486577571E+00, "VV": 1.12433031828299E-05, "WW": 2.17481920242362E-06,
+    "FU": -5.81971846904289E-04, "FV": -2.65230200621369E-03, "FW": -1.92178866508701E-02,
+    "FR": -5.89573017030585E-03, "FT": -7.00596774694232E-02,
+    "WW1": 5.22301068250754E-03, "WW2": 9.36285533397170E-04, "WW3": 2.34071365909616E-03,
+    "DT9": 5.53534681584488E-04,
+    "WMIN": (3.76152577723530E-03, 6.91885347138954E-04, 1.72971323897370E-03),
+}
+KA_LROT = {
+    "RU": -2.85215796137258E-05, "RV": 1.47319886830764E-05, "RW": -1.12703533419297E-06,
+    "RO": 1.35293233493915E+00, "TT": 1.35293016916632E+00,
+    "UU": 2.25495486544032E+00, "VV": 1.12499252339042E-05, "WW": 2.17387015528925E-06,
+    "FU": -5.81434951767399E-04, "FV": -2.65029895816105E-03, "FW": -1.92180366617439E-02,
+    "FR": -5.89572807332526E-03, "FT": -7.00596760935853E-02,
+    "WW1": 5.22301068289597E-03, "WW2": 9.36285533399704E-04, "WW3": 2.34071365910249E-03,
+    "WMIN": (3.76152577714958E-03, 6.91885347124525E-04, 1.72971323893763E-03),
+}
+
+NPE_Y, NPE_Z = 4, 6          # 3dmhdparam.h:19-21
+NP = 256                     # 3dmhdparam.h:25-27
+NRZ = NP // NPE_Z            # 42 planes per z rank -> 252 planes in all
+
+
+def default_config(lRot=False):
+    par = default_par()
+    if lRot:
+        par[4] = 10.0
+        par[5] = 0.25
+    return RunConfig(nPx=NP, nPy=NP, nPz=NPE_Z * NRZ, nPz_global=NP, nPy_global=NP,
+                     lRot=lRot, izCon=1, iTCon=4, par=par)
+
+
+def default_state(cfg, oracle_mod):
+    """Static() of 3dmhdphsic.h:549-763 for the 4x6 rank run, as one domain.
+
+    Plane g of the domain sits on z rank g//42; ranks 0..4 take T[g], the last
+    rank takes T[g+4] (3dmhdphsic.h:642-666 skips T[210..213]).  The RND
+    perturbation follows the rank loop order of :709-744."""
+    par = cfg.par
+    T, R = oracle_mod.static_profile(NP, par[2], par[3], par[18])
+    nzg = NPE_Z * NRZ
+    shape = (nzg + 2, cfg.nPy + 2, cfg.nPx + 2)
+    TT = np.zeros(shape)
+    RO = np.zeros(shape)
+    for z in range(nzg + 2):
+        if z == 0:
+            k = 0
+        elif z == nzg + 1:
+            k = NP - 1
+        else:
+            g = z - 1
+            k = g if g // NRZ < NPE_Z - 1 else g + (NP - nzg)
+        TT[z, 1:-1, 1:-1] = T[k]
+        RO[z, 1:-1, 1:-1] = R[k]
+    rnd = oracle_mod.static_rnd(cfg.nPx, cfg.nPy, nzg, NPE_Y, NPE_Z, par[19])
+    TT[1:-1, 1:-1, 1:-1] *= rnd[1:-1, 1:-1, 1:-1]
+    zero = np.zeros(shape)
+    return {"RU": zero, "RV": zero.copy(), "RW": zero.copy(), "RO": RO, "TT": TT}
