@@ -1,0 +1,335 @@
+// fluxes() of the reference (3dmhdsub.h:194-696) evaluated at one point.
+//
+// The reference builds each flux with whole-array sweeps (about 73 for hydro,
+// 155 for MHD with rotation); here every flux of one point is accumulated in
+// registers, in the same statement order and with the same left-to-right
+// products as the operator templates of 3dmhdutils.h:195-239, so each value is
+// rounded exactly as in the reference (the file is compiled without
+// floating-point contraction).  Intermediate arrays of the reference (WW1..3
+// as D1x(u), D1z(w), ...; RU..TT as lMag scratch) become locals; the mixed
+// derivatives D1y(D1x u) etc. evaluate their inner derivative at the
+// neighbours the outer one needs, exactly as the reference's extended-range
+// sweeps do (3dmhdsub.h:475-476,505,520-521).
+//
+// Accessor A provides, for offsets (dx,dy,dz) from the point:
+//   ru rv rw ro tt bx by bz  state values
+//   ri = 1/RO, u = RU*ri, v = RV*ri, w = RW*ri, p = RO*TT   (3dmhdsub.h:267-276)
+// WALL = 0 (interior plane), 1 (top wall, z = zl), 2 (bottom wall, z = zr-1):
+// the one-sided stencils of 3dmhdsub.h:211-226, 572-584, 640-654, 673-687.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "mhd3d_types.hpp"
+
+namespace mhd {
+
+struct FluxOut {
+    double FU, FV, FW, FR, FT, B1, B2, B3;
+};
+
+struct PointMetric {
+    double dxx, d2xx, dyy, d2yy, dzz, d2zz, kap, dkap;
+};
+
+template <typename T>
+struct Ops {
+    const T dxx, d2xx, dyy, d2yy, dzz, d2zz, hx, h2x, hy, h2y, hz, h2z;
+    // func_d1: ((f[+1]-f[-1]) * du) * h     (3dmhdutils.h:195-199)
+    __device__ __forceinline__ T d1x(T fp, T fm) const { return ((fp - fm) * dxx) * hx; }
+    __device__ __forceinline__ T d1y(T fp, T fm) const { return ((fp - fm) * dyy) * hy; }
+    __device__ __forceinline__ T d1z(T fp, T fm) const { return ((fp - fm) * dzz) * hz; }
+    // func_d2: (df*d2u*h + ((f[+1]-2f)+f[-1])*du^2*h2)   (3dmhdutils.h:215-220)
+    __device__ __forceinline__ T d2x(T fp, T f0, T fm) const
+    {
+        return (fp - fm) * d2xx * hx + ((fp - T(2) * f0) + fm) * (dxx * dxx) * h2x;
+    }
+    __device__ __forceinline__ T d2y(T fp, T f0, T fm) const
+    {
+        return (fp - fm) * d2yy * hy + ((fp - T(2) * f0) + fm) * (dyy * dyy) * h2y;
+    }
+    __device__ __forceinline__ T d2z(T fp, T f0, T fm) const
+    {
+        return (fp - fm) * d2zz * hz + ((fp - T(2) * f0) + fm) * (dzz * dzz) * h2z;
+    }
+};
+
+template <bool MAG, bool ROT, int WALL, typename T, typename A>
+__device__ __forceinline__ void flux_point(const A& a, const Phys& c, const PointMetric& m,
+                                           FluxOut& o)
+{
+    const Ops<T> op{T(m.dxx), T(m.d2xx), T(m.dyy), T(m.d2yy), T(m.dzz), T(m.d2zz),
+                    T(c.hx),  T(c.h2x),  T(c.hy),  T(c.h2y),  T(c.hz),  T(c.h2z)};
+    const T ORE = T(c.ORE), OCV = T(c.OCV), C13 = T(c.C13), C43 = T(c.C43);
+    const T ORE_C43 = T(c.ORE * c.C43);
+    const T C13_ORE = T(c.C13 * c.ORE);
+    T FU, FV, FW, FR, FT, tmp;
+
+    // ---- continuity :205-227
+    FR = op.d1x(a.ru(1, 0, 0), a.ru(-1, 0, 0)) * T(-1);
+    FR -= op.d1y(a.rv(0, 1, 0), a.rv(0, -1, 0));
+    T dzrw;
+    if (WALL == 1)
+        dzrw = (T(4) * a.rw(0, 0, 1) - a.rw(0, 0, 2)) * (T(c.hz) * op.dzz);
+    else if (WALL == 2)
+        dzrw = (a.rw(0, 0, -2) - T(4) * a.rw(0, 0, -1)) * (T(c.hz) * op.dzz);
+    else
+        dzrw = op.d1z(a.rw(0, 0, 1), a.rw(0, 0, -1));
+    FR -= dzrw;
+
+    // ---- gravity and pressure :266-272
+    FW = a.ro(0, 0, 0) * T(c.GRAV);
+    FW -= op.d1z(a.p(0, 0, 1), a.p(0, 0, -1));
+    FV = op.d1y(a.p(0, 1, 0), a.p(0, -1, 0)) * T(-1);
+    FU = op.d1x(a.p(1, 0, 0), a.p(-1, 0, 0)) * T(-1);
+
+    // ---- advection of momentum :279-289 (func_mul_d1: ((F+G+ - F-G-)*du)*h)
+    FU -= ((a.ru(1, 0, 0) * a.u(1, 0, 0) - a.ru(-1, 0, 0) * a.u(-1, 0, 0)) * op.dxx) * op.hx;
+    FU -= ((a.ru(0, 1, 0) * a.v(0, 1, 0) - a.ru(0, -1, 0) * a.v(0, -1, 0)) * op.dyy) * op.hy;
+    FU -= ((a.ru(0, 0, 1) * a.w(0, 0, 1) - a.ru(0, 0, -1) * a.w(0, 0, -1)) * op.dzz) * op.hz;
+    FV -= ((a.rv(1, 0, 0) * a.u(1, 0, 0) - a.rv(-1, 0, 0) * a.u(-1, 0, 0)) * op.dxx) * op.hx;
+    FV -= ((a.rv(0, 1, 0) * a.v(0, 1, 0) - a.rv(0, -1, 0) * a.v(0, -1, 0)) * op.dyy) * op.hy;
+    FV -= ((a.rv(0, 0, 1) * a.w(0, 0, 1) - a.rv(0, 0, -1) * a.w(0, 0, -1)) * op.dzz) * op.hz;
+    FW -= ((a.rw(1, 0, 0) * a.u(1, 0, 0) - a.rw(-1, 0, 0) * a.u(-1, 0, 0)) * op.dxx) * op.hx;
+    FW -= ((a.rw(0, 1, 0) * a.v(0, 1, 0) - a.rw(0, -1, 0) * a.v(0, -1, 0)) * op.dyy) * op.hy;
+    FW -= ((a.rw(0, 0, 1) * a.w(0, 0, 1) - a.rw(0, 0, -1) * a.w(0, 0, -1)) * op.dzz) * op.hz;
+
+    const T RI = a.ri(0, 0, 0);
+    const T U0 = a.u(0, 0, 0), V0 = a.v(0, 0, 0), W0 = a.w(0, 0, 0);
+    const T T0 = a.tt(0, 0, 0);
+
+    // ---- temperature advection :294-296 (func_d1_mul: (((df*du)*h)*g)*s)
+    const T dxT = op.d1x(a.tt(1, 0, 0), a.tt(-1, 0, 0));
+    const T dyT = op.d1y(a.tt(0, 1, 0), a.tt(0, -1, 0));
+    const T dzT = op.d1z(a.tt(0, 0, 1), a.tt(0, 0, -1));
+    FT = (dxT * U0) * T(-1);
+    FT -= dyT * V0;
+    FT -= dzT * W0;
+
+    // ---- conduction :315-319 (func_d2_mul_array_z: ((d2*g)*a[z])*s)
+    tmp = OCV / T(c.REPR);
+    const T kap = T(m.kap);
+    FT += ((op.d2x(a.tt(1, 0, 0), T0, a.tt(-1, 0, 0)) * RI) * kap) * tmp;
+    FT += ((op.d2y(a.tt(0, 1, 0), T0, a.tt(0, -1, 0)) * RI) * kap) * tmp;
+    FT += ((op.d2z(a.tt(0, 0, 1), T0, a.tt(0, 0, -1)) * RI) * kap) * tmp;
+    FT += ((dzT * RI) * T(m.dkap)) * tmp;
+
+    // ---- viscous :462-472
+    FU += op.d2x(a.u(1, 0, 0), U0, a.u(-1, 0, 0)) * ORE_C43;
+    FU += op.d2y(a.u(0, 1, 0), U0, a.u(0, -1, 0)) * ORE;
+    FU += op.d2z(a.u(0, 0, 1), U0, a.u(0, 0, -1)) * ORE;
+    FV += op.d2x(a.v(1, 0, 0), V0, a.v(-1, 0, 0)) * ORE;
+    FV += op.d2y(a.v(0, 1, 0), V0, a.v(0, -1, 0)) * ORE_C43;
+    FV += op.d2z(a.v(0, 0, 1), V0, a.v(0, 0, -1)) * ORE;
+    FW += op.d2x(a.w(1, 0, 0), W0, a.w(-1, 0, 0)) * ORE;
+    FW += op.d2y(a.w(0, 1, 0), W0, a.w(0, -1, 0)) * ORE;
+    FW += op.d2z(a.w(0, 0, 1), W0, a.w(0, 0, -1)) * ORE_C43;
+
+    // ---- cross-viscous terms, viscous heating, pressure work :475-538
+    const T dxu = op.d1x(a.u(1, 0, 0), a.u(-1, 0, 0));
+    const T dxv = op.d1x(a.v(1, 0, 0), a.v(-1, 0, 0));
+    const T dxw = op.d1x(a.w(1, 0, 0), a.w(-1, 0, 0));
+    {
+        tmp = OCV * ORE;
+        FT += ((RI * dxu) * dxu) * (tmp * C43);
+        FT += ((RI * dxv) * dxv) * tmp;
+        FT += ((RI * dxw) * dxw) * tmp;
+        FT -= (dxu * T0) * OCV;
+    }
+    {
+        // D1x(v), D1x(u) at y+-1 (WW2, WW1 over FORRANGE_EXT_Y, :475-476)
+        const T dxv_p = op.d1x(a.v(1, 1, 0), a.v(-1, 1, 0));
+        const T dxv_m = op.d1x(a.v(1, -1, 0), a.v(-1, -1, 0));
+        const T dxu_p = op.d1x(a.u(1, 1, 0), a.u(-1, 1, 0));
+        const T dxu_m = op.d1x(a.u(1, -1, 0), a.u(-1, -1, 0));
+        FU += op.d1y(dxv_p, dxv_m) * C13_ORE;
+        FV += op.d1y(dxu_p, dxu_m) * C13_ORE;
+    }
+    const T dyu = op.d1y(a.u(0, 1, 0), a.u(0, -1, 0));
+    const T dyv = op.d1y(a.v(0, 1, 0), a.v(0, -1, 0));
+    {
+        tmp = OCV * ORE;
+        FT += ((RI * dyu) * dyu) * tmp;
+        FT += ((RI * dyu) * dxv) * (tmp * T(2));
+        tmp = C43 * OCV * ORE;
+        FT += ((RI * dyv) * dyv) * tmp;
+        FT -= (dyv * T0) * OCV;
+    }
+    // D1z(w) at the point and at x+-1, y+-1 (WW3 over FORRANGE_EXT_XY, :505)
+    const T dzw = op.d1z(a.w(0, 0, 1), a.w(0, 0, -1));
+    {
+        tmp = C43 * OCV * ORE;
+        FT += ((RI * dzw) * dzw) * tmp;
+        FT += ((RI * dxu) * dzw) * (-tmp);
+        FT += ((RI * dxu) * dyv) * (-tmp);
+        FT += ((RI * dyv) * dzw) * (-tmp);
+        FT -= (dzw * T0) * OCV;
+    }
+    {
+        const T dzw_xp = op.d1z(a.w(1, 0, 1), a.w(1, 0, -1));
+        const T dzw_xm = op.d1z(a.w(-1, 0, 1), a.w(-1, 0, -1));
+        const T dzw_yp = op.d1z(a.w(0, 1, 1), a.w(0, 1, -1));
+        const T dzw_ym = op.d1z(a.w(0, -1, 1), a.w(0, -1, -1));
+        FU += op.d1x(dzw_xp, dzw_xm) * C13_ORE;
+        FV += op.d1y(dzw_yp, dzw_ym) * C13_ORE;
+    }
+    // D1z(u) at x+-1 (WW1 over FORRANGE_EXT_X) and D1z(v) at y+-1 (WW2, :520-521)
+    const T dzu = op.d1z(a.u(0, 0, 1), a.u(0, 0, -1));
+    const T dzv = op.d1z(a.v(0, 0, 1), a.v(0, 0, -1));
+    const T dyw = op.d1y(a.w(0, 1, 0), a.w(0, -1, 0));
+    {
+        tmp = OCV * ORE;
+        FT += ((RI * dzu) * dzu) * tmp;
+        FT += ((RI * dzv) * dzv) * tmp;
+        FT += ((RI * dyw) * dyw) * tmp;
+        FT += ((RI * dzv) * dyw) * (T(2) * tmp);
+    }
+    {
+        const T dzu_xp = op.d1z(a.u(1, 0, 1), a.u(1, 0, -1));
+        const T dzu_xm = op.d1z(a.u(-1, 0, 1), a.u(-1, 0, -1));
+        const T dzv_yp = op.d1z(a.v(0, 1, 1), a.v(0, 1, -1));
+        const T dzv_ym = op.d1z(a.v(0, -1, 1), a.v(0, -1, -1));
+        FW += op.d1x(dzu_xp, dzu_xm) * C13_ORE;
+        FW += op.d1y(dzv_yp, dzv_ym) * C13_ORE;
+    }
+    {
+        tmp = OCV * ORE;
+        FT += ((RI * dzu) * dxw) * (T(2) * tmp);
+    }
+
+    // ---- Coriolis :540-545
+    if (ROT) {
+        FU += a.rv(0, 0, 0) * T(c.OMZ);
+        FV -= a.ru(0, 0, 0) * T(c.OMZ);
+        FV += a.rw(0, 0, 0) * T(c.OMX);
+        FW -= a.rv(0, 0, 0) * T(c.OMX);
+    }
+
+    T B1 = T(0), B2 = T(0), B3 = T(0);
+    if (MAG) {
+        // ---- induction, Lorentz force, Joule heating :547-695
+        const T OBETA = T(c.OBETA), ORM = T(c.ORM);
+        const T KJ = T(c.OBETA * c.ORM * c.OCV);
+        const T KJ2 = T(-2.0 * c.OBETA * c.ORM * c.OCV);
+        const T BX = a.bx(0, 0, 0), BY = a.by(0, 0, 0), BZ = a.bz(0, 0, 0);
+        T s1, s2, s3, s4;
+        // :548-556 (RU,RV,RW = D1x of u,v,w)
+        B2 = BX * dxv;
+        B2 -= BY * dxu;
+        B3 = BX * dxw;
+        B3 -= BZ * dxu;
+        // :558-566
+        const T dyw_ = dyw;
+        B1 = BY * dyu;
+        B1 -= BX * dyv;
+        B3 += BY * dyw_;
+        B3 -= BZ * dyv;
+        // :568-590, one-sided D1z(w) at the walls
+        T dzw_m;
+        if (WALL == 1)
+            dzw_m = (T(4) * a.w(0, 0, 1) - a.w(0, 0, 2)) * T(c.hz) * op.dzz;
+        else if (WALL == 2)
+            dzw_m = (T(-4) * a.w(0, 0, -1) + a.w(0, 0, -2)) * T(c.hz) * op.dzz;
+        else
+            dzw_m = dzw;
+        B1 += BZ * dzu;
+        B1 -= BX * dzw_m;
+        B2 += BZ * dzv;
+        B2 -= BY * dzw_m;
+        // :592-607
+        s1 = op.d1x(a.bx(1, 0, 0), a.bx(-1, 0, 0));
+        s2 = op.d1x(a.by(1, 0, 0), a.by(-1, 0, 0));
+        s3 = op.d1x(a.bz(1, 0, 0), a.bz(-1, 0, 0));
+        s4 = op.d1y(a.bx(0, 1, 0), a.bx(0, -1, 0));
+        FU += (BY * s4) * OBETA;
+        FU -= (BY * s2) * OBETA;
+        FU -= (BZ * s3) * OBETA;
+        FV += (BX * s2) * OBETA;
+        FV -= (BX * s4) * OBETA;
+        FW += (BX * s3) * OBETA;
+        FT += ((RI * s4) * s4) * KJ;
+        FT += ((RI * s2) * s2) * KJ;
+        FT += ((RI * s3) * s3) * KJ;
+        FT += ((RI * s2) * s4) * KJ2;
+        // :609-612
+        B1 -= U0 * s1;
+        B1 -= V0 * s4;
+        B2 -= U0 * s2;
+        B3 -= U0 * s3;
+        const T dxBZ = s3;
+        // :614-624
+        s1 = op.d2x(a.bx(1, 0, 0), BX, a.bx(-1, 0, 0));
+        s2 = op.d2x(a.by(1, 0, 0), BY, a.by(-1, 0, 0));
+        s4 = op.d2y(a.bx(0, 1, 0), BX, a.bx(0, -1, 0));
+        B1 += s1 * ORM;
+        B1 += s4 * ORM;
+        B2 += s2 * ORM;
+        s4 = op.d2x(a.bz(1, 0, 0), BZ, a.bz(-1, 0, 0));
+        B3 += s4 * ORM;
+        // :626-636
+        s1 = op.d1z(a.bx(0, 0, 1), a.bx(0, 0, -1));
+        s2 = op.d1y(a.by(0, 1, 0), a.by(0, -1, 0));
+        FU += (BZ * s1) * OBETA;
+        FW -= (BX * s1) * OBETA;
+        FT += ((RI * s1) * s1) * KJ;
+        FT += ((RI * s1) * dxBZ) * KJ2;
+        B1 -= W0 * s1;
+        B2 -= V0 * s2;
+        // :638-658, one-sided D2z(BX) at the walls
+        if (WALL == 1) {
+            const T b0 = BX, b1 = a.bx(0, 0, 1), b2 = a.bx(0, 0, 2), b3 = a.bx(0, 0, 3);
+            s1 = (T(-3) * b0 + T(4) * b1 - b2) * T(c.hz) * op.d2zz +
+                 (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
+        } else if (WALL == 2) {
+            const T b0 = BX, b1 = a.bx(0, 0, -1), b2 = a.bx(0, 0, -2), b3 = a.bx(0, 0, -3);
+            s1 = (T(3) * b0 - T(4) * b1 + b2) * T(c.hz) * op.d2zz +
+                 (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
+        } else {
+            s1 = op.d2z(a.bx(0, 0, 1), BX, a.bx(0, 0, -1));
+        }
+        s2 = op.d2y(a.by(0, 1, 0), BY, a.by(0, -1, 0));
+        B1 += s1 * ORM;
+        B2 += s2 * ORM;
+        // :659-671
+        s2 = op.d1z(a.by(0, 0, 1), a.by(0, 0, -1));
+        s3 = op.d1y(a.bz(0, 1, 0), a.bz(0, -1, 0));
+        s1 = s2;
+        s1 -= s3;
+        FV += (BZ * s1) * OBETA;
+        FW -= (BY * s1) * OBETA;
+        FT += ((RI * s2) * s2) * KJ;
+        FT += ((RI * s3) * s3) * KJ;
+        FT += ((RI * s2) * s3) * KJ2;
+        B2 -= W0 * s2;
+        B3 -= V0 * s3;
+        // :672-690, one-sided D2z(BY) at the walls
+        if (WALL == 1) {
+            const T b0 = BY, b1 = a.by(0, 0, 1), b2 = a.by(0, 0, 2), b3 = a.by(0, 0, 3);
+            s2 = (T(-3) * b0 + T(4) * b1 - b2) * T(c.hz) * op.d2zz +
+                 (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
+        } else if (WALL == 2) {
+            const T b0 = BY, b1 = a.by(0, 0, -1), b2 = a.by(0, 0, -2), b3 = a.by(0, 0, -3);
+            s2 = (T(3) * b0 - T(4) * b1 + b2) * T(c.hz) * op.d2zz +
+                 (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
+        } else {
+            s2 = op.d2z(a.by(0, 0, 1), BY, a.by(0, 0, -1));
+        }
+        s3 = op.d2y(a.bz(0, 1, 0), BZ, a.bz(0, -1, 0));
+        B2 += s2 * ORM;
+        B3 += s3 * ORM;
+        // :691-694
+        s3 = op.d1z(a.bz(0, 0, 1), a.bz(0, 0, -1));
+        B3 -= W0 * s3;
+        s3 = op.d2z(a.bz(0, 0, 1), BZ, a.bz(0, 0, -1));
+        B3 += s3 * ORM;
+    }
+    o.FU = double(FU);
+    o.FV = double(FV);
+    o.FW = double(FW);
+    o.FR = double(FR);
+    o.FT = double(FT);
+    o.B1 = double(B1);
+    o.B2 = double(B2);
+    o.B3 = double(B3);
+}
+
+}  // namespace mhd

@@ -1,0 +1,455 @@
+// Point-parallel kernels: wall planes of a stage, CFL reduction, BCON,
+// stand-alone fluxes, diagnostics and small bookkeeping kernels.
+//
+// The wall kernel evaluates the one-sided stencils of the two z walls and
+// then applies BCON (3dmhdsub.h:720-873) to the freshly updated wall values,
+// reading the new values two planes inward that the interior stage kernel has
+// already written.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "flux_point.hpp"
+#include "mhd3d_types.hpp"
+
+namespace mhd {
+
+// State accessor that reads the register straight from global memory and
+// forms 1/RO, u, v, w and P on the fly (3dmhdsub.h:267-276).
+struct GAcc {
+    const double* __restrict__ U;
+    int64_t fs, plane;
+    int pitch, nPx, nPy, x, y, z;
+    __device__ __forceinline__ double ld(int f, int dx, int dy, int dz) const
+    {
+        const int xx = wrapi(x + dx, nPx), yy = wrapi(y + dy, nPy);
+        return U[f * fs + (int64_t)(z + dz) * plane + (int64_t)yy * pitch + xx];
+    }
+    __device__ __forceinline__ double ru(int a, int b, int c) const { return ld(0, a, b, c); }
+    __device__ __forceinline__ double rv(int a, int b, int c) const { return ld(1, a, b, c); }
+    __device__ __forceinline__ double rw(int a, int b, int c) const { return ld(2, a, b, c); }
+    __device__ __forceinline__ double ro(int a, int b, int c) const { return ld(3, a, b, c); }
+    __device__ __forceinline__ double tt(int a, int b, int c) const { return ld(4, a, b, c); }
+    __device__ __forceinline__ double bx(int a, int b, int c) const { return ld(5, a, b, c); }
+    __device__ __forceinline__ double by(int a, int b, int c) const { return ld(6, a, b, c); }
+    __device__ __forceinline__ double bz(int a, int b, int c) const { return ld(7, a, b, c); }
+    __device__ __forceinline__ double ri(int a, int b, int c) const { return 1.0 / ro(a, b, c); }
+    __device__ __forceinline__ double u(int a, int b, int c) const { return ru(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ double v(int a, int b, int c) const { return rv(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ double w(int a, int b, int c) const { return rw(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ double p(int a, int b, int c) const { return ro(a, b, c) * tt(a, b, c); }
+};
+
+struct StageArgs {
+    Geom g;
+    Phys c;
+    Metric m;
+    const double* __restrict__ Uin;
+    double* __restrict__ Uout;
+    double* __restrict__ Z;
+    const double* __restrict__ dt;
+};
+
+__device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int yg, int zg)
+{
+    PointMetric pm;
+    pm.dxx = m.dxxdx[xg];
+    pm.d2xx = m.d2xxdx2[xg];
+    pm.dyy = m.dyydy[yg];
+    pm.d2yy = m.d2yydy2[yg];
+    pm.dzz = m.dzzdz[zg];
+    pm.d2zz = m.d2zzdz2[zg];
+    pm.kap = m.RKAPA[zg];
+    pm.dkap = m.DKAPA[zg];
+    return pm;
+}
+
+__device__ __forceinline__ double fsel(const FluxOut& o, int f)
+{
+    switch (f) {
+    case 0: return o.FU;
+    case 1: return o.FV;
+    case 2: return o.FW;
+    case 3: return o.FR;
+    case 4: return o.FT;
+    case 5: return o.B1;
+    case 6: return o.B2;
+    default: return o.B3;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interior planes of one RK stage, one thread per point (baseline form).
+// Stage k: F = fluxes(Uin); Uout = Zin + GAM_k*DT*F; Z = Uout + ZETA_k*DT*F
+// (step_block_GAM / step_block_ZETA, 3dmhdsub.h:168-192; Zin = Uin for k = 1).
+template <bool MAG, bool ROT, int STAGE>
+__global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z + 2;   // planes 2 .. nPz-1
+    if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz - 1) return;
+    GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+    const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+    FluxOut o;
+    flux_point<MAG, ROT, 0, double>(acc, a.c, pm, o);
+    const double DT = *a.dt;
+    const double cg = a.c.GAM[STAGE - 1] * DT;
+    const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
+    const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
+    constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double F = fsel(o, f);
+        const double zin = STAGE == 1 ? a.Uin[f * a.g.fs + i] : a.Z[f * a.g.fs + i];
+        const double un = zin + cg * F;
+        a.Uout[f * a.g.fs + i] = un;
+        if (STAGE < 3) a.Z[f * a.g.fs + i] = un + cz * F;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wall planes of one RK stage: fluxes with the one-sided wall stencils, the
+// GAM update, BCON on the updated wall values, then the ZETA register.
+template <bool MAG, bool ROT, int STAGE>
+__global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const bool top = blockIdx.z == 0;
+    if (x >= a.g.nPx || y >= a.g.nPy) return;
+    const int z = top ? 1 : a.g.nPz;
+    const int zi1 = top ? z + 1 : z - 1;   // first plane inward
+    const int zi2 = top ? z + 2 : z - 2;   // second plane inward
+    GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+    const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+    FluxOut o;
+    if (top)
+        flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
+    else
+        flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+    const double DT = *a.dt;
+    const double cg = a.c.GAM[STAGE - 1] * DT;
+    const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
+    const int64_t row = (int64_t)y * a.g.pitch + x;
+    const int64_t i = (int64_t)z * a.g.plane + row;
+    const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
+    const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
+    const int64_t fs = a.g.fs;
+    constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
+    double un[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double zin = STAGE == 1 ? a.Uin[f * fs + i] : a.Z[f * fs + i];
+        un[f] = zin + cg * fsel(o, f);
+    }
+    // BCON (3dmhdsub.h:727-872); the updated planes inward are in Uout
+    const Phys& c = a.c;
+    const double* Un = a.Uout;
+    const double ro1 = Un[3 * fs + i1], ro2 = Un[3 * fs + i2];
+    un[0] = un[3] * (c.C43 * Un[0 * fs + i1] / ro1 - c.C13 * Un[0 * fs + i2] / ro2);
+    un[1] = un[3] * (c.C43 * Un[1 * fs + i1] / ro1 - c.C13 * Un[1 * fs + i2] / ro2);
+    un[2] = 0.0;
+    if (top) {
+        if (c.iTCon == 0 || c.iTCon == 4) {
+            un[4] = 1.0;
+            if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
+        } else if (c.iTCon == 1) {
+            const double DZ = c.ORZF / a.m.dzzdz[z];
+            un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] - c.C23 * DZ * c.THETA;
+        }
+    } else {
+        if (c.izCon == 0) {
+            un[4] = c.TB;
+        } else {
+            const double DZ = c.ORZF / a.m.dzzdz[z];
+            un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] -
+                    c.C23 * DZ * c.THETA / a.m.RKAPA[z];
+        }
+    }
+    if (MAG && c.iBCon == 0) un[7] = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        a.Uout[f * fs + i] = un[f];
+        if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stand-alone fluxes() into F (all interior planes incl. walls).
+template <bool MAG, bool ROT>
+__global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __restrict__ F)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z + 1;   // planes 1 .. nPz
+    if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz) return;
+    GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+    const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+    FluxOut o;
+    if (z == 1)
+        flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
+    else if (z == a.g.nPz)
+        flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+    else
+        flux_point<MAG, ROT, 0, double>(acc, a.c, pm, o);
+    const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
+    constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) F[f * a.g.fs + i] = fsel(o, f);
+}
+
+// ---------------------------------------------------------------------------
+// BCON() on a register in place (3dmhdsub.h:720-873).
+template <bool MAG>
+__global__ void bcon_kernel(Geom g, Phys c, Metric m, double* __restrict__ U)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const bool top = blockIdx.z == 0;
+    if (x >= g.nPx || y >= g.nPy) return;
+    const int z = top ? 1 : g.nPz;
+    const int64_t row = (int64_t)y * g.pitch + x;
+    const int64_t i = (int64_t)z * g.plane + row;
+    const int64_t i1 = (int64_t)(top ? z + 1 : z - 1) * g.plane + row;
+    const int64_t i2 = (int64_t)(top ? z + 2 : z - 2) * g.plane + row;
+    const int64_t fs = g.fs;
+    const double ro = U[3 * fs + i], ro1 = U[3 * fs + i1], ro2 = U[3 * fs + i2];
+    U[0 * fs + i] = ro * (c.C43 * U[0 * fs + i1] / ro1 - c.C13 * U[0 * fs + i2] / ro2);
+    U[1 * fs + i] = ro * (c.C43 * U[1 * fs + i1] / ro1 - c.C13 * U[1 * fs + i2] / ro2);
+    U[2 * fs + i] = 0.0;
+    if (top) {
+        if (c.iTCon == 0 || c.iTCon == 4) {
+            double t = 1.0;
+            if (c.iTCon == 4 && (x + 1) < (g.nPx + 2) / 2) t = 1.0 - c.TP;
+            U[4 * fs + i] = t;
+        } else if (c.iTCon == 1) {
+            const double DZ = c.ORZF / m.dzzdz[z];
+            U[4 * fs + i] = c.C43 * U[4 * fs + i1] - c.C13 * U[4 * fs + i2] - c.C23 * DZ * c.THETA;
+        }
+    } else {
+        if (c.izCon == 0) {
+            U[4 * fs + i] = c.TB;
+        } else {
+            const double DZ = c.ORZF / m.dzzdz[z];
+            U[4 * fs + i] = c.C43 * U[4 * fs + i1] - c.C13 * U[4 * fs + i2] -
+                            c.C23 * DZ * c.THETA / m.RKAPA[z];
+        }
+    }
+    if (MAG && c.iBCon == 0) U[7 * fs + i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// TSTEP (3dmhdsub.h:36-106): per-point CFL limits, wave + block reduction,
+// then one atomic per block on order-preserving keys.  keys[0..3] = minima of
+// WW1, WW2, WW3, VV; keys[4] = max of OGAMMA*UU/TT (UMACH^2).
+struct CflRed {
+    unsigned long long k[5];
+};
+
+__device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Metric& m,
+                                          const double* __restrict__ U, bool mag, int x, int y,
+                                          int z, double w[4], double& um)
+{
+    const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
+    const int64_t fs = g.fs;
+    const double RU = U[i], RV = U[fs + i], RW = U[2 * fs + i], RO = U[3 * fs + i];
+    const double TT = U[4 * fs + i];
+    double UU = (RU * RU + RV * RV + RW * RW) / (RO * RO);
+    um = c.OGAMMA * UU / TT;
+    if (mag) {
+        const double BX = U[5 * fs + i], BY = U[6 * fs + i], BZ = U[7 * fs + i];
+        const double Q = c.GAMMA * TT + (BX * BX + BY * BY + BZ * BZ) * c.OBETA / RO;
+        UU += Q + 2.0 * sqrt(UU * Q);
+    } else {
+        UU += c.GAMMA * TT + 2.0 * sqrt(c.GAMMA * UU * TT);
+    }
+    double DD = ref_min(m.DDY[y + 1], m.DDZ[z]);
+    if (c.xdim) DD = ref_min(DD, m.DDX[x + 1]);
+    w[0] = DD / sqrt(UU);
+    w[1] = 0.5 * DD * DD * c.REPR * c.CV * RO / m.RKAPA[z];
+    w[2] = 0.375 * DD * DD * c.RE * RO;
+    w[3] = mag ? 0.5 * DD * DD * c.RM : 1e99;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = ref_min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = ref_max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <bool MAG>
+__global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
+                                                  const double* __restrict__ U,
+                                                  CflRed* __restrict__ red)
+{
+    __shared__ double sh[5][4];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    double w[4] = {1e99, 1e99, 1e99, 1e99};
+    double um = 0.0;
+    if (x < g.nPx && y < g.nPy) {
+        for (int z = 1 + blockIdx.z; z <= g.nPz; z += gridDim.z) {
+            double wp[4], up;
+            cfl_point(g, c, m, U, MAG, x, y, z, wp, up);
+            // NaN candidates never replace the running value (std::min/max)
+            for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], wp[k]);
+            um = ref_max(um, up);
+        }
+    }
+    for (int k = 0; k < 4; ++k) w[k] = wave_min(w[k]);
+    um = wave_max(um);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        for (int k = 0; k < 4; ++k) sh[k][wv] = w[k];
+        sh[4][wv] = um;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (blockDim.x * blockDim.y + 63) / 64;
+        for (int j = 1; j < nw; ++j) {
+            for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], sh[k][j]);
+            um = ref_max(um, sh[4][j]);
+        }
+        for (int k = 0; k < 4; ++k)
+            if (w[k] == w[k]) atomicMin(&red->k[k], ord_key(w[k]));
+        if (um == um) atomicMax(&red->k[4], ord_key(um));
+    }
+}
+
+// Device scalars of one context.
+struct DevScalars {
+    double dt;          // DT of the current step
+    double timc;        // TIMC (3dmhdsub.h:151)
+    double nit;         // nIt
+    double umach;
+    double wmin[4];
+    CflRed red;
+    unsigned int nonfinite;
+    unsigned int pad;
+};
+
+__global__ void cfl_init_kernel(DevScalars* s)
+{
+    for (int k = 0; k < 4; ++k) s->red.k[k] = ord_key(1e99);
+    s->red.k[4] = ord_key(0.0);
+}
+
+__global__ void cfl_finalize_kernel(DevScalars* s, Phys c, int mag)
+{
+    double w[4];
+    for (int k = 0; k < 4; ++k) w[k] = ord_val(s->red.k[k]);
+    s->umach = sqrt(ord_val(s->red.k[4]));
+    for (int k = 0; k < 4; ++k) s->wmin[k] = w[k];
+    if (mag)
+        s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), ref_min(w[2], w[3]));
+    else
+        s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), w[2]);
+}
+
+__global__ void set_dt_kernel(DevScalars* s, double dt) { s->dt = dt; }
+
+__global__ void clock_kernel(DevScalars* s)
+{
+    s->timc += s->dt;
+    s->nit += 1.0;
+}
+
+// Non-finite scan of a register (interior incl. walls).
+__global__ void nonfinite_kernel(Geom g, int nf, const double* __restrict__ U, DevScalars* s)
+{
+    const int64_t n = (int64_t)g.nPz * g.plane;
+    const int64_t base = g.plane;   // skip the top ghost plane
+    bool bad = false;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(t % g.pitch);
+        if (x >= g.nPx) continue;
+        for (int f = 0; f < nf; ++f) {
+            const double v = U[f * g.fs + base + t];
+            if (!(v - v == 0.0)) bad = true;
+        }
+    }
+    if (bad) atomicOr(&s->nonfinite, 1u);
+}
+
+}  // namespace mhd
+
+namespace mhd {
+
+// Global FP64 diagnostics: total mass, total energy, max |div B|.
+// Deterministic: fixed per-block partials, summed in block order.
+__global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, Metric m, int nf,
+                                                   const double* __restrict__ U,
+                                                   double* __restrict__ part)
+{
+    __shared__ double sh[3][256];
+    double ms = 0.0, en = 0.0, db = 0.0;
+    const int64_t npts = (int64_t)g.nPz * g.nPy * g.nPx;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npts;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(t % g.nPx);
+        const int y = (int)((t / g.nPx) % g.nPy);
+        const int z = (int)(t / ((int64_t)g.nPx * g.nPy)) + 1;
+        const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
+        const double RU = U[i], RV = U[g.fs + i], RW = U[2 * g.fs + i];
+        const double RO = U[3 * g.fs + i], TT = U[4 * g.fs + i];
+        ms += RO;
+        double e = 0.5 * (RU * RU + RV * RV + RW * RW) / RO + c.CV * RO * TT;
+        if (nf == NF_MAG) {
+            const double BX = U[5 * g.fs + i], BY = U[6 * g.fs + i], BZ = U[7 * g.fs + i];
+            e += 0.5 * c.OBETA * (BX * BX + BY * BY + BZ * BZ);
+            if (z > 1 && z < g.nPz) {
+                const int xp = wrapi(x + 1, g.nPx), xm = wrapi(x - 1, g.nPx);
+                const int yp = wrapi(y + 1, g.nPy), ym = wrapi(y - 1, g.nPy);
+                const int64_t r = (int64_t)z * g.plane;
+                const double dbx = (U[5 * g.fs + r + (int64_t)y * g.pitch + xp] -
+                                    U[5 * g.fs + r + (int64_t)y * g.pitch + xm]) * m.dxxdx[x + 1] * c.hx;
+                const double dby = (U[6 * g.fs + r + (int64_t)yp * g.pitch + x] -
+                                    U[6 * g.fs + r + (int64_t)ym * g.pitch + x]) * m.dyydy[y + 1] * c.hy;
+                const double dbz = (U[7 * g.fs + i + g.plane] - U[7 * g.fs + i - g.plane]) *
+                                   m.dzzdz[z] * c.hz;
+                db = fmax(db, fabs(dbx + dby + dbz));
+            }
+        }
+        en += e;
+    }
+    sh[0][threadIdx.x] = ms;
+    sh[1][threadIdx.x] = en;
+    sh[2][threadIdx.x] = db;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = sh[0][0];
+        part[3 * blockIdx.x + 1] = sh[1][0];
+        part[3 * blockIdx.x + 2] = sh[2][0];
+    }
+}
+
+__global__ void diag_finalize_kernel(const double* part, int n, double* out)
+{
+    if (threadIdx.x != 0) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = 0; k < n; ++k) {
+        a += part[3 * k];
+        b += part[3 * k + 1];
+        c = fmax(c, part[3 * k + 2]);
+    }
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+}
+
+}  // namespace mhd

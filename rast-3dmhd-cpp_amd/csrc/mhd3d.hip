@@ -1,0 +1,555 @@
+// C ABI of the MI355X RK3 timestep (include/mhd3d.h).
+//
+// One context = one domain on one GPU: three device registers (two U buffers
+// used as ping-pong, one Z), the metric arrays, a block of device scalars and
+// one HIP stream.  A step is: CFL (unless fixed DT) -> 3 x (interior stage
+// kernel, wall stage kernel) -> clock update, with DT kept on the device so
+// the host never waits inside a step.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd3d.h"
+#include "kernels_basic.hpp"
+#include "kernels_stream.hpp"
+
+using namespace mhd;
+
+struct mhd3d_ctx {
+    mhd3d_desc d;
+    int nf = 5;
+    Geom g{};
+    Phys c{};
+    Metric m{};
+    double* dmet = nullptr;
+    double* U[2] = {nullptr, nullptr};
+    double* Z = nullptr;
+    double* F = nullptr;
+    DevScalars* ds = nullptr;
+    double* host_stage = nullptr;   // pinned, one field
+    hipStream_t stream = nullptr;
+    int cur = 0;
+    int device = 0;
+    int kernel_mode = 0;            // 0 = z-streaming LDS kernel, 1 = point kernel
+    int64_t bytes = 0;
+    std::string err;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double stage_ms = 0.0;
+    int64_t stage_launches = 0;
+    double step_ms = 0.0;
+    int64_t steps_timed = 0;
+};
+
+static std::string g_create_error;
+
+#define HIPCHK(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+            return MHD3D_EHIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+static int fail(mhd3d_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+
+static int validate(const mhd3d_desc* d, std::string& why)
+{
+    if (!d) { why = "null descriptor"; return MHD3D_EINVAL; }
+    if (d->nPx < 1 || d->nPy < 1) { why = "nPx and nPy must be >= 1"; return MHD3D_EINVAL; }
+    if (d->nPz < 5) { why = "nPz must be >= 5 (one-sided wall stencils reach 3 planes in)"; return MHD3D_EINVAL; }
+    if (!d->dxxdx || !d->d2xxdx2 || !d->DDX || !d->dyydy || !d->d2yydy2 || !d->DDY ||
+        !d->dzzdz || !d->d2zzdz2 || !d->DDZ || !d->RKAPA || !d->DKAPA) {
+        why = "metric array pointer missing";
+        return MHD3D_EINVAL;
+    }
+    if (d->ixCon != 0 || d->iyCon != 0) { why = "fluxes: Non-periodic horizontal boundaries (3dmhdsub.h:208)"; return MHD3D_EUNSUPPORTED; }
+    if (d->lRem) { why = "lRem (Rempel relaxation) is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->lShr) { why = "lShr is not built on the GPU path"; return MHD3D_EUNSUPPORTED; }
+    if (d->iD != 0) { why = "density diffusion iD != 0 is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->RLAX != 0.0) { why = "RLAX relaxation is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->iTCon == 2) { why = "iTCon = 2 (embedded heat loss) is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->iTCon < 0 || d->iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
+    if ((d->iTCon == 0 || d->iTCon == 1) && d->TP != 0.0) { why = "iTCon 0/1 with TP != 0 (Gaussian plume wall) is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->izCon != 0 && d->izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
+    if (d->lMag && d->iBCon != 0) { why = "iBCon != 0 is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->precision != MHD3D_FP64 && d->precision != MHD3D_MIXED) { why = "invalid precision"; return MHD3D_EINVAL; }
+    return MHD3D_OK;
+}
+
+static Phys make_phys(const mhd3d_desc* d)
+{
+    Phys c{};
+    c.C13 = d->C13; c.C23 = d->C23; c.C43 = d->C43;
+    c.GAM[0] = d->GAM1; c.GAM[1] = d->GAM2; c.GAM[2] = d->GAM3;
+    c.ZETA[0] = d->ZETA1; c.ZETA[1] = d->ZETA2;
+    c.SFF = d->SFF;
+    c.hx = d->hx; c.h2x = d->h2x; c.hy = d->hy; c.h2y = d->h2y; c.hz = d->hz; c.h2z = d->h2z;
+    c.ORZF = d->ORZF;
+    c.RE = d->RE; c.ORE = d->ORE; c.REPR = d->REPR; c.CV = d->CV; c.OCV = d->OCV;
+    c.GAMMA = d->GAMMA; c.OGAMMA = 1.0 / d->GAMMA; c.THETA = d->THETA; c.GRAV = d->GRAV;
+    c.OMX = d->OMX; c.OMZ = d->OMZ; c.RM = d->RM; c.ORM = d->ORM; c.OBETA = d->OBETA;
+    c.TP = d->TP; c.TB = d->TB;
+    c.izCon = d->izCon; c.iTCon = d->iTCon; c.iBCon = d->iBCon;
+    c.xdim = (d->nPx + 2) > 3;   // `nx > ix + 1` (3dmhdsub.h:77)
+    return c;
+}
+
+extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
+{
+    if (!out) return MHD3D_EINVAL;
+    *out = nullptr;
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc != MHD3D_OK) { g_create_error = why; return rc; }
+    mhd3d_ctx* ctx = new mhd3d_ctx();
+    ctx->d = *desc;
+    ctx->nf = desc->lMag ? NF_MAG : NF_HYDRO;
+    Geom& g = ctx->g;
+    g.nPx = desc->nPx; g.nPy = desc->nPy; g.nPz = desc->nPz;
+    g.nz = desc->nPz + 2;
+    g.pitch = (desc->nPx + 7) / 8 * 8;
+    g.plane = (int64_t)g.pitch * g.nPy;
+    g.fs = (g.plane * g.nz + 31) / 32 * 32;   // 256-byte aligned fields
+    ctx->c = make_phys(desc);
+    auto cleanup = [&](int code, const std::string& m) {
+        g_create_error = m;
+        mhd3d_destroy(ctx);
+        return code;
+    };
+    if (hipGetDevice(&ctx->device) != hipSuccess) return cleanup(MHD3D_EHIP, "hipGetDevice failed");
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(MHD3D_EHIP, "hipStreamCreate failed");
+    const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;
+    // metric block: dxxdx d2xxdx2 DDX | dyydy d2yydy2 DDY | dzzdz d2zzdz2 DDZ RKAPA DKAPA
+    const size_t nmet = 3 * (size_t)nx + 3 * (size_t)ny + 5 * (size_t)nz;
+    std::vector<double> hm(nmet);
+    size_t o = 0;
+    auto put = [&](const double* src, int n) { std::memcpy(&hm[o], src, n * sizeof(double)); o += n; };
+    put(desc->dxxdx, nx); put(desc->d2xxdx2, nx); put(desc->DDX, nx);
+    put(desc->dyydy, ny); put(desc->d2yydy2, ny); put(desc->DDY, ny);
+    put(desc->dzzdz, nz); put(desc->d2zzdz2, nz); put(desc->DDZ, nz);
+    put(desc->RKAPA, nz); put(desc->DKAPA, nz);
+    if (hipMalloc(&ctx->dmet, nmet * sizeof(double)) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(metrics) failed");
+    if (hipMemcpy(ctx->dmet, hm.data(), nmet * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(MHD3D_EHIP, "metric upload failed");
+    double* p = ctx->dmet;
+    ctx->m.dxxdx = p; p += nx; ctx->m.d2xxdx2 = p; p += nx; ctx->m.DDX = p; p += nx;
+    ctx->m.dyydy = p; p += ny; ctx->m.d2yydy2 = p; p += ny; ctx->m.DDY = p; p += ny;
+    ctx->m.dzzdz = p; p += nz; ctx->m.d2zzdz2 = p; p += nz; ctx->m.DDZ = p; p += nz;
+    ctx->m.RKAPA = p; p += nz; ctx->m.DKAPA = p; p += nz;
+    const size_t reg = (size_t)g.fs * ctx->nf * sizeof(double);
+    for (int k = 0; k < 2; ++k)
+        if (hipMalloc(&ctx->U[k], reg) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(U) failed");
+    if (hipMalloc(&ctx->Z, reg) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(Z) failed");
+    if (hipMalloc(&ctx->ds, sizeof(DevScalars)) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(scalars) failed");
+    if (hipMemset(ctx->U[0], 0, reg) != hipSuccess || hipMemset(ctx->U[1], 0, reg) != hipSuccess ||
+        hipMemset(ctx->Z, 0, reg) != hipSuccess || hipMemset(ctx->ds, 0, sizeof(DevScalars)) != hipSuccess)
+        return cleanup(MHD3D_EHIP, "hipMemset failed");
+    if (hipHostMalloc(&ctx->host_stage, (size_t)g.fs * sizeof(double)) != hipSuccess)
+        return cleanup(MHD3D_ENOMEM, "hipHostMalloc failed");
+    ctx->bytes = (int64_t)(3 * reg + nmet * sizeof(double) + sizeof(DevScalars));
+    const char* km = std::getenv("MHD3D_KERNEL");
+    if (km && std::strcmp(km, "point") == 0) ctx->kernel_mode = 1;
+    *out = ctx;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
+{
+    if (!ctx) return MHD3D_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    (void)hipFree(ctx->dmet);
+    (void)hipFree(ctx->U[0]);
+    (void)hipFree(ctx->U[1]);
+    (void)hipFree(ctx->Z);
+    (void)hipFree(ctx->F);
+    (void)hipFree(ctx->ds);
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MHD3D_OK;
+}
+
+extern "C" const char* mhd3d_last_error(const mhd3d_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int64_t mhd3d_device_bytes(const mhd3d_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// host <-> device field transfer
+
+static int upload_field(mhd3d_ctx* ctx, double* dst, const double* src)
+{
+    const Geom& g = ctx->g;
+    const int nx = g.nPx + 2, ny = g.nPy + 2;
+    double* h = ctx->host_stage;
+    std::memset(h, 0, (size_t)g.fs * sizeof(double));
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.nPy; ++y)
+            std::memcpy(h + (size_t)z * g.plane + (size_t)y * g.pitch,
+                        src + ((size_t)z * ny + (y + 1)) * nx + 1, g.nPx * sizeof(double));
+    HIPCHK(ctx, hipMemcpyAsync(dst, h, (size_t)g.fs * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}
+
+static int download_field(mhd3d_ctx* ctx, double* dst, const double* src)
+{
+    const Geom& g = ctx->g;
+    const int nx = g.nPx + 2, ny = g.nPy + 2;
+    double* h = ctx->host_stage;
+    HIPCHK(ctx, hipMemcpyAsync(h, src, (size_t)g.fs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int z = 0; z < g.nz; ++z) {
+        for (int y = 0; y < g.nPy; ++y) {
+            double* row = dst + ((size_t)z * ny + (y + 1)) * nx;
+            std::memcpy(row + 1, h + (size_t)z * g.plane + (size_t)y * g.pitch, g.nPx * sizeof(double));
+            row[0] = row[nx - 2];            // 3dmhdmpi.h:118-121
+            row[nx - 1] = row[1];
+        }
+        double* pl = dst + (size_t)z * ny * nx;
+        std::memcpy(pl, pl + (size_t)(ny - 2) * nx, nx * sizeof(double));   // :65-68,135-142
+        std::memcpy(pl + (size_t)(ny - 1) * nx, pl + (size_t)nx, nx * sizeof(double));
+    }
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
+{
+    if (!ctx || !fields) return MHD3D_EINVAL;
+    for (int f = 0; f < ctx->nf; ++f)
+        if (!fields[f]) return fail(ctx, MHD3D_EINVAL, "null field pointer");
+    for (int f = 0; f < ctx->nf; ++f) {
+        int rc = upload_field(ctx, ctx->U[ctx->cur] + (size_t)f * ctx->g.fs, fields[f]);
+        if (rc) return rc;
+    }
+    // the other U buffer needs the z ghost planes too (stage kernels never write them)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->U[1 - ctx->cur], ctx->U[ctx->cur],
+                               (size_t)ctx->g.fs * ctx->nf * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_get_state(mhd3d_ctx* ctx, double* const* fields)
+{
+    if (!ctx || !fields) return MHD3D_EINVAL;
+    for (int f = 0; f < ctx->nf; ++f) {
+        if (!fields[f]) return fail(ctx, MHD3D_EINVAL, "null field pointer");
+        int rc = download_field(ctx, fields[f], ctx->U[ctx->cur] + (size_t)f * ctx->g.fs);
+        if (rc) return rc;
+    }
+    return MHD3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+
+static StageArgs stage_args(mhd3d_ctx* ctx)
+{
+    StageArgs a;
+    a.g = ctx->g;
+    a.c = ctx->c;
+    a.m = ctx->m;
+    a.Uin = ctx->U[ctx->cur];
+    a.Uout = ctx->U[1 - ctx->cur];
+    a.Z = ctx->Z;
+    a.dt = &ctx->ds->dt;
+    return a;
+}
+
+static hipEvent_t next_event(mhd3d_ctx* ctx)
+{
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+template <bool MAG, bool ROT, int STAGE>
+static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
+{
+    const Geom& g = ctx->g;
+    dim3 blk(64, 4, 1);
+    if (ctx->kernel_mode == 1) {
+        dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
+        stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+    } else {
+        launch_stream_stage<MAG, ROT, STAGE>(a, ctx->stream);
+    }
+}
+
+template <bool MAG, bool ROT, int STAGE>
+static void launch_wall_t(mhd3d_ctx* ctx, const StageArgs& a)
+{
+    const Geom& g = ctx->g;
+    dim3 blk(64, 4, 1);
+    dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
+    stage_wall_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+}
+
+template <int STAGE>
+static void launch_stage_k(mhd3d_ctx* ctx, const StageArgs& a, bool wall)
+{
+    const bool mag = ctx->d.lMag, rot = ctx->d.lRot;
+    if (!wall) {
+        if (mag && rot) launch_stage_t<true, true, STAGE>(ctx, a);
+        else if (mag) launch_stage_t<true, false, STAGE>(ctx, a);
+        else if (rot) launch_stage_t<false, true, STAGE>(ctx, a);
+        else launch_stage_t<false, false, STAGE>(ctx, a);
+    } else {
+        if (mag && rot) launch_wall_t<true, true, STAGE>(ctx, a);
+        else if (mag) launch_wall_t<true, false, STAGE>(ctx, a);
+        else if (rot) launch_wall_t<false, true, STAGE>(ctx, a);
+        else launch_wall_t<false, false, STAGE>(ctx, a);
+    }
+}
+
+static int run_stage(mhd3d_ctx* ctx, int k)
+{
+    StageArgs a = stage_args(ctx);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        e0 = next_event(ctx);
+        e1 = next_event(ctx);
+        if (!e0 || !e1) return fail(ctx, MHD3D_EHIP, "hipEventCreate failed");
+        HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+    }
+    if (k == 1) launch_stage_k<1>(ctx, a, false);
+    else if (k == 2) launch_stage_k<2>(ctx, a, false);
+    else launch_stage_k<3>(ctx, a, false);
+    if (ctx->timing) HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+    if (k == 1) launch_stage_k<1>(ctx, a, true);
+    else if (k == 2) launch_stage_k<2>(ctx, a, true);
+    else launch_stage_k<3>(ctx, a, true);
+    HIPCHK(ctx, hipGetLastError());
+    ctx->cur = 1 - ctx->cur;
+    return MHD3D_OK;
+}
+
+static int run_cfl(mhd3d_ctx* ctx)
+{
+    const Geom& g = ctx->g;
+    cfl_init_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
+    dim3 blk(64, 4, 1);
+    int zchunks = g.nPz < 32 ? g.nPz : 32;
+    dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, zchunks);
+    if (ctx->d.lMag)
+        cfl_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], &ctx->ds->red);
+    else
+        cfl_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], &ctx->ds->red);
+    cfl_finalize_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, ctx->c, ctx->d.lMag ? 1 : 0);
+    HIPCHK(ctx, hipGetLastError());
+    return MHD3D_OK;
+}
+
+static int collect_timing(mhd3d_ctx* ctx)
+{
+    if (!ctx->timing) return MHD3D_OK;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+        ctx->stage_ms += ms;
+        ctx->stage_launches += 1;
+    }
+    ctx->ev_used = 0;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
+{
+    if (!ctx || nsteps < 0) return MHD3D_EINVAL;
+    if (fixed_dt && !dt_inout) return fail(ctx, MHD3D_EINVAL, "fixed_dt needs *dt_inout");
+    if (fixed_dt) {
+        set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, *dt_inout);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    for (int n = 0; n < nsteps; ++n) {
+        if (!fixed_dt) {
+            int rc = run_cfl(ctx);
+            if (rc) return rc;
+        }
+        for (int k = 1; k <= 3; ++k) {
+            int rc = run_stage(ctx, k);
+            if (rc) return rc;
+        }
+        clock_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
+        // bound the event pool: fold timings every 64 steps
+        if (ctx->timing && ctx->ev_used >= 384) {
+            int rc = collect_timing(ctx);
+            if (rc) return rc;
+        }
+    }
+    (void)s0; (void)s1;
+    HIPCHK(ctx, hipGetLastError());
+    ctx->d.reserved = 0;
+    DevScalars hs;
+    HIPCHK(ctx, hipMemsetAsync(&ctx->ds->nonfinite, 0, sizeof(unsigned int), ctx->stream));
+    nonfinite_kernel<<<256, 256, 0, ctx->stream>>>(ctx->g, ctx->nf, ctx->U[ctx->cur], ctx->ds);
+    HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = collect_timing(ctx);
+    if (rc) return rc;
+    if (dt_inout) *dt_inout = hs.dt;
+    if (hs.nonfinite) return fail(ctx, MHD3D_ENONFINITE, "state is not finite after the step");
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
+{
+    if (!ctx || stage < 1 || stage > 3) return MHD3D_EINVAL;
+    set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt);
+    int rc = run_stage(ctx, stage);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return collect_timing(ctx);
+}
+
+extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
+{
+    if (!ctx || !flux) return MHD3D_EINVAL;
+    const Geom& g = ctx->g;
+    const size_t reg = (size_t)g.fs * ctx->nf * sizeof(double);
+    if (!ctx->F) {
+        HIPCHK(ctx, hipMalloc(&ctx->F, reg));
+        ctx->bytes += (int64_t)reg;
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->F, 0, reg, ctx->stream));
+    StageArgs a = stage_args(ctx);
+    dim3 blk(64, 4, 1);
+    dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz);
+    const bool mag = ctx->d.lMag, rot = ctx->d.lRot;
+    if (mag && rot) flux_only_kernel<true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+    else if (mag) flux_only_kernel<true, false><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+    else if (rot) flux_only_kernel<false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+    else flux_only_kernel<false, false><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+    HIPCHK(ctx, hipGetLastError());
+    const int nx = g.nPx + 2, ny = g.nPy + 2;
+    for (int f = 0; f < ctx->nf; ++f) {
+        if (!flux[f]) return fail(ctx, MHD3D_EINVAL, "null flux pointer");
+        double* h = ctx->host_stage;
+        HIPCHK(ctx, hipMemcpyAsync(h, ctx->F + (size_t)f * g.fs, (size_t)g.fs * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        double* dst = flux[f];
+        std::memset(dst, 0, (size_t)g.nz * ny * nx * sizeof(double));
+        for (int z = 1; z <= g.nPz; ++z)
+            for (int y = 0; y < g.nPy; ++y)
+                std::memcpy(dst + ((size_t)z * ny + (y + 1)) * nx + 1,
+                            h + (size_t)z * g.plane + (size_t)y * g.pitch, g.nPx * sizeof(double));
+    }
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_bcon(mhd3d_ctx* ctx)
+{
+    if (!ctx) return MHD3D_EINVAL;
+    const Geom& g = ctx->g;
+    dim3 blk(64, 4, 1);
+    dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
+    if (ctx->d.lMag)
+        bcon_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur]);
+    else
+        bcon_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur]);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_cfl(mhd3d_ctx* ctx, mhd3d_cfl_info* out)
+{
+    if (!ctx || !out) return MHD3D_EINVAL;
+    int rc = run_cfl(ctx);
+    if (rc) return rc;
+    DevScalars hs;
+    HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; ++k) out->wmin[k] = hs.wmin[k];
+    out->umach = hs.umach;
+    out->dt = hs.dt;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_clock(mhd3d_ctx* ctx, int64_t* nIt, double* timt, double* timc)
+{
+    if (!ctx) return MHD3D_EINVAL;
+    DevScalars hs;
+    HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (nIt) *nIt = (int64_t)hs.nit;
+    if (timc) *timc = hs.timc;
+    if (timt) *timt = ctx->d.TIMI + hs.timc;    // TIMT = TIMI + TIMC (3dmhdsub.h:152)
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_diagnostics(mhd3d_ctx* ctx, double out[3])
+{
+    if (!ctx || !out) return MHD3D_EINVAL;
+    const Geom& g = ctx->g;
+    const int nblk = 1024;
+    double* part = nullptr;
+    HIPCHK(ctx, hipMalloc(&part, sizeof(double) * 3 * nblk));
+    diag_kernel<<<nblk, 256, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->nf, ctx->U[ctx->cur], part);
+    diag_finalize_kernel<<<1, 64, 0, ctx->stream>>>(part, nblk, part);
+    double h[3];
+    hipError_t e = hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return fail(ctx, MHD3D_EHIP, hipGetErrorString(e));
+    out[0] = h[0];
+    out[1] = h[1];
+    out[2] = h[2];
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_timing_enable(mhd3d_ctx* ctx, int on)
+{
+    if (!ctx) return MHD3D_EINVAL;
+    ctx->timing = on != 0;
+    ctx->stage_ms = 0.0;
+    ctx->stage_launches = 0;
+    ctx->step_ms = 0.0;
+    ctx->steps_timed = 0;
+    ctx->ev_used = 0;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_timing_read(mhd3d_ctx* ctx, mhd3d_timing* out)
+{
+    if (!ctx || !out) return MHD3D_EINVAL;
+    out->stage_ms_total = ctx->stage_ms;
+    out->stage_launches = ctx->stage_launches;
+    out->step_ms_total = ctx->step_ms;
+    out->steps = ctx->steps_timed;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_synchronize(mhd3d_ctx* ctx)
+{
+    if (!ctx) return MHD3D_EINVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}

@@ -1,0 +1,83 @@
+// Device-side types of the MI355X RK3 timestep.
+//
+// Device layout of one register (U or Z): the fields RU,RV,RW,RO,TT[,BX,BY,BZ]
+// one after another, each [nz][nPy][pitch] doubles with nz = nPz+2.  The z
+// ghost planes of the reference's arrays (3dmhddata.h:8) are kept; the x and
+// y ghost columns are not: x and y are periodic (3dmhdmpi.h:59-142), so the
+// kernels wrap the in-plane indices instead of refilling ghosts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mhd {
+
+constexpr int NF_HYDRO = 5;
+constexpr int NF_MAG = 8;
+
+// Sizes of one domain.
+struct Geom {
+    int nPx, nPy, nPz;   // interior x, y; z planes incl. walls
+    int nz;              // nPz + 2 (with the z ghost planes)
+    int pitch;           // row pitch in doubles (>= nPx)
+    int64_t plane;       // nPy * pitch
+    int64_t fs;          // field stride in doubles
+};
+
+// Scalars of set() (3dmhd.cpp:353-495) the kernels read.
+struct Phys {
+    double C13, C23, C43;
+    double GAM[3], ZETA[2];
+    double SFF;
+    double hx, h2x, hy, h2y, hz, h2z;
+    double ORZF;
+    double RE, ORE, REPR, CV, OCV, GAMMA, OGAMMA, THETA, GRAV;
+    double OMX, OMZ, RM, ORM, OBETA;
+    double TP, TB;
+    int izCon, iTCon, iBCon, xdim;
+};
+
+// 1-D metric arrays on the device, indexed with the reference's ghosted
+// indices (interior x = 1..nPx etc.).
+struct Metric {
+    const double* __restrict__ dxxdx;
+    const double* __restrict__ d2xxdx2;
+    const double* __restrict__ DDX;
+    const double* __restrict__ dyydy;
+    const double* __restrict__ d2yydy2;
+    const double* __restrict__ DDY;
+    const double* __restrict__ dzzdz;
+    const double* __restrict__ d2zzdz2;
+    const double* __restrict__ DDZ;
+    const double* __restrict__ RKAPA;
+    const double* __restrict__ DKAPA;
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Order-preserving map of doubles onto uint64 (for atomic min/max).
+__device__ __forceinline__ unsigned long long ord_key(double v)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double ord_val(unsigned long long k)
+{
+    unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double d;
+    __builtin_memcpy(&d, &u, sizeof(d));
+    return d;
+}
+__host__ __device__ inline unsigned long long ord_key_host(double v)
+{
+    unsigned long long u;
+    __builtin_memcpy(&u, &v, sizeof(u));
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// std::min / std::max of the reference (3dmhdutils.h:61-95): a NaN candidate
+// never replaces the running value.
+__host__ __device__ __forceinline__ double ref_min(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ double ref_max(double a, double b) { return (a < b) ? b : a; }
+
+}  // namespace mhd

@@ -1,0 +1,84 @@
+"""ctypes binding of libmhd3d.so (the C ABI of include/mhd3d.h).
+
+The library is the product: it must exist (built by ``build.py`` /
+``__graft_entry__.build()``); there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libmhd3d.so")
+
+OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENONFINITE = range(6)
+FP64, MIXED = 0, 1
+FIELDS = ("RU", "RV", "RW", "RO", "TT", "BX", "BY", "BZ")
+FLUXES = ("FU", "FV", "FW", "FR", "FT", "WW1", "WW2", "WW3")
+
+# Every entry point the header declares: name -> (restype, argtypes)
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_dpp = C.POINTER(_dp)
+SIGNATURES = {
+    "mhd3d_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "mhd3d_destroy": (C.c_int, [_vp]),
+    "mhd3d_set_state": (C.c_int, [_vp, _dpp]),
+    "mhd3d_get_state": (C.c_int, [_vp, _dpp]),
+    "mhd3d_step": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "mhd3d_fluxes": (C.c_int, [_vp, _dpp]),
+    "mhd3d_stage": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "mhd3d_bcon": (C.c_int, [_vp]),
+    "mhd3d_cfl": (C.c_int, [_vp, _vp]),
+    "mhd3d_diagnostics": (C.c_int, [_vp, _dp]),
+    "mhd3d_clock": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp, _dp]),
+    "mhd3d_timing_enable": (C.c_int, [_vp, C.c_int]),
+    "mhd3d_timing_read": (C.c_int, [_vp, _vp]),
+    "mhd3d_synchronize": (C.c_int, [_vp]),
+    "mhd3d_last_error": (C.c_char_p, [_vp]),
+    "mhd3d_device_bytes": (C.c_int64, [_vp]),
+}
+
+
+class CflInfo(C.Structure):
+    _fields_ = [("wmin", C.c_double * 4), ("umach", C.c_double), ("dt", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("stage_ms_total", C.c_double), ("stage_launches", C.c_int64),
+                ("step_ms_total", C.c_double), ("steps", C.c_int64)]
+
+
+_lib = None
+
+
+class MhdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mhd3d error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libmhd3d.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                               "or rast-3dmhd-cpp_amd/build.py first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _fieldptrs(arrays):
+    arr = (_dp * len(arrays))()
+    for k, a in enumerate(arrays):
+        arr[k] = a.ctypes.data_as(_dp)
+    return arr

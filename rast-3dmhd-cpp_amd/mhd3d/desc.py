@@ -260,7 +260,10 @@ def set_constants(cfg: RunConfig):
     c["RLAX"] = par[59]
     c["TIMI"] = float(cfg.TIMI)
 
-    EXX, dxxdx, d2xxdx2, DDX = _uniform_axis(cfg.nPx, cfg.nPx, xMax, ghosts_zero=True)
+    if cfg.nPx > 1:                                             # :461-468
+        EXX, dxxdx, d2xxdx2, DDX = _uniform_axis(cfg.nPx, cfg.nPx, xMax, ghosts_zero=True)
+    else:
+        EXX, dxxdx, d2xxdx2, DDX = (np.zeros(cfg.nPx + 2) for _ in range(4))
     WYY, dyydy, d2yydy2, DDY = _uniform_axis(cfg.nPy, nPy_g, yMax, ghosts_zero=False)
     if cfg.zgrid[0] == "uniform":
         ZEE, dzzdz, d2zzdz2, DDZ = _uniform_axis(cfg.nPz, nPz_g, zMax, ghosts_zero=False)
