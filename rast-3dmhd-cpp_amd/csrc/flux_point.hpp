@@ -59,9 +59,9 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
 {
     const Ops<T> op{T(m.dxx), T(m.d2xx), T(m.dyy), T(m.d2yy), T(m.dzz), T(m.d2zz),
                     T(c.hx),  T(c.h2x),  T(c.hy),  T(c.h2y),  T(c.hz),  T(c.h2z)};
-    const T ORE = T(c.ORE), OCV = T(c.OCV), C13 = T(c.C13), C43 = T(c.C43);
-    const T ORE_C43 = T(c.ORE * c.C43);
-    const T C13_ORE = T(c.C13 * c.ORE);
+    const T ORE = T(c.ORE), OCV = T(c.OCV);
+    const T ORE_C43 = T(c.ORE_C43);
+    const T C13_ORE = T(c.C13_ORE);
     T FU, FV, FW, FR, FT, tmp;
 
     // ---- continuity :205-227
@@ -106,7 +106,7 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     FT -= dzT * W0;
 
     // ---- conduction :315-319 (func_d2_mul_array_z: ((d2*g)*a[z])*s)
-    tmp = OCV / T(c.REPR);
+    tmp = T(c.OCV_REPR);
     const T kap = T(m.kap);
     FT += ((op.d2x(a.tt(1, 0, 0), T0, a.tt(-1, 0, 0)) * RI) * kap) * tmp;
     FT += ((op.d2y(a.tt(0, 1, 0), T0, a.tt(0, -1, 0)) * RI) * kap) * tmp;
@@ -129,8 +129,8 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     const T dxv = op.d1x(a.v(1, 0, 0), a.v(-1, 0, 0));
     const T dxw = op.d1x(a.w(1, 0, 0), a.w(-1, 0, 0));
     {
-        tmp = OCV * ORE;
-        FT += ((RI * dxu) * dxu) * (tmp * C43);
+        tmp = T(c.OCV_ORE);
+        FT += ((RI * dxu) * dxu) * T(c.OCV_ORE_C43);
         FT += ((RI * dxv) * dxv) * tmp;
         FT += ((RI * dxw) * dxw) * tmp;
         FT -= (dxu * T0) * OCV;
@@ -147,17 +147,17 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     const T dyu = op.d1y(a.u(0, 1, 0), a.u(0, -1, 0));
     const T dyv = op.d1y(a.v(0, 1, 0), a.v(0, -1, 0));
     {
-        tmp = OCV * ORE;
+        tmp = T(c.OCV_ORE);
         FT += ((RI * dyu) * dyu) * tmp;
-        FT += ((RI * dyu) * dxv) * (tmp * T(2));
-        tmp = C43 * OCV * ORE;
+        FT += ((RI * dyu) * dxv) * T(c.OCV_ORE_2);
+        tmp = T(c.C43_OCV_ORE);
         FT += ((RI * dyv) * dyv) * tmp;
         FT -= (dyv * T0) * OCV;
     }
     // D1z(w) at the point and at x+-1, y+-1 (WW3 over FORRANGE_EXT_XY, :505)
     const T dzw = op.d1z(a.w(0, 0, 1), a.w(0, 0, -1));
     {
-        tmp = C43 * OCV * ORE;
+        tmp = T(c.C43_OCV_ORE);
         FT += ((RI * dzw) * dzw) * tmp;
         FT += ((RI * dxu) * dzw) * (-tmp);
         FT += ((RI * dxu) * dyv) * (-tmp);
@@ -177,11 +177,11 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     const T dzv = op.d1z(a.v(0, 0, 1), a.v(0, 0, -1));
     const T dyw = op.d1y(a.w(0, 1, 0), a.w(0, -1, 0));
     {
-        tmp = OCV * ORE;
+        tmp = T(c.OCV_ORE);
         FT += ((RI * dzu) * dzu) * tmp;
         FT += ((RI * dzv) * dzv) * tmp;
         FT += ((RI * dyw) * dyw) * tmp;
-        FT += ((RI * dzv) * dyw) * (T(2) * tmp);
+        FT += ((RI * dzv) * dyw) * T(c.OCV_ORE_2);
     }
     {
         const T dzu_xp = op.d1z(a.u(1, 0, 1), a.u(1, 0, -1));
@@ -192,8 +192,7 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         FW += op.d1y(dzv_yp, dzv_ym) * C13_ORE;
     }
     {
-        tmp = OCV * ORE;
-        FT += ((RI * dzu) * dxw) * (T(2) * tmp);
+        FT += ((RI * dzu) * dxw) * T(c.OCV_ORE_2);
     }
 
     // ---- Coriolis :540-545
@@ -207,9 +206,24 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     T B1 = T(0), B2 = T(0), B3 = T(0);
     if (MAG) {
         // ---- induction, Lorentz force, Joule heating :547-695
+        // The reference recomputes D1x/D1y/D1z of u, v, w here (:548-570);
+        // so does this code, behind a compiler memory fence, instead of
+        // keeping the values above live through the hydro terms (the
+        // register budget of the fused kernel).  Same operands, same results.
+        asm volatile("" ::: "memory");
+        const T dxu = op.d1x(a.u(1, 0, 0), a.u(-1, 0, 0));
+        const T dxv = op.d1x(a.v(1, 0, 0), a.v(-1, 0, 0));
+        const T dxw = op.d1x(a.w(1, 0, 0), a.w(-1, 0, 0));
+        const T dyu = op.d1y(a.u(0, 1, 0), a.u(0, -1, 0));
+        const T dyv = op.d1y(a.v(0, 1, 0), a.v(0, -1, 0));
+        const T dyw = op.d1y(a.w(0, 1, 0), a.w(0, -1, 0));
+        const T dzu = op.d1z(a.u(0, 0, 1), a.u(0, 0, -1));
+        const T dzv = op.d1z(a.v(0, 0, 1), a.v(0, 0, -1));
+        const T dzw = op.d1z(a.w(0, 0, 1), a.w(0, 0, -1));
+        const T U0 = a.u(0, 0, 0), V0 = a.v(0, 0, 0), W0 = a.w(0, 0, 0);
         const T OBETA = T(c.OBETA), ORM = T(c.ORM);
-        const T KJ = T(c.OBETA * c.ORM * c.OCV);
-        const T KJ2 = T(-2.0 * c.OBETA * c.ORM * c.OCV);
+        const T KJ = T(c.KJ);
+        const T KJ2 = T(c.KJ2);
         const T BX = a.bx(0, 0, 0), BY = a.by(0, 0, 0), BZ = a.bz(0, 0, 0);
         T s1, s2, s3, s4;
         // :548-556 (RU,RV,RW = D1x of u,v,w)

@@ -239,12 +239,10 @@ __global__ void bcon_kernel(Geom g, Phys c, Metric m, double* __restrict__ U)
 }
 
 // ---------------------------------------------------------------------------
-// TSTEP (3dmhdsub.h:36-106): per-point CFL limits, wave + block reduction,
-// then one atomic per block on order-preserving keys.  keys[0..3] = minima of
-// WW1, WW2, WW3, VV; keys[4] = max of OGAMMA*UU/TT (UMACH^2).
-struct CflRed {
-    unsigned long long k[5];
-};
+// TSTEP (3dmhdsub.h:36-106): per-point CFL limits, then a wave + block
+// reduction into one partial record per block; cfl_finalize_kernel reduces
+// the partials.  Min and max are exact, so the result does not depend on the
+// reduction order.
 
 __device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Metric& m,
                                           const double* __restrict__ U, bool mag, int x, int y,
@@ -282,16 +280,24 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// One partial record per block: minima of WW1, WW2, WW3, VV and max of
+// OGAMMA*UU/TT over the block's points, plus a non-finite flag.
+struct CflPart {
+    double w[4];
+    double um;
+    double bad;
+};
+
 template <bool MAG>
 __global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
                                                   const double* __restrict__ U,
-                                                  CflRed* __restrict__ red)
+                                                  CflPart* __restrict__ part)
 {
-    __shared__ double sh[5][4];
+    __shared__ double sh[6][4];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     double w[4] = {1e99, 1e99, 1e99, 1e99};
-    double um = 0.0;
+    double um = 0.0, bad = 0.0;
     if (x < g.nPx && y < g.nPy) {
         for (int z = 1 + blockIdx.z; z <= g.nPz; z += gridDim.z) {
             double wp[4], up;
@@ -299,15 +305,18 @@ __global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
             // NaN candidates never replace the running value (std::min/max)
             for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], wp[k]);
             um = ref_max(um, up);
+            if (!(wp[0] - wp[0] == 0.0) || !(wp[1] - wp[1] == 0.0)) bad = 1.0;
         }
     }
     for (int k = 0; k < 4; ++k) w[k] = wave_min(w[k]);
     um = wave_max(um);
+    bad = wave_max(bad);
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63;
     if (lane == 0) {
         for (int k = 0; k < 4; ++k) sh[k][wv] = w[k];
         sh[4][wv] = um;
+        sh[5][wv] = bad;
     }
     __syncthreads();
     if (tid == 0) {
@@ -315,10 +324,12 @@ __global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
         for (int j = 1; j < nw; ++j) {
             for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], sh[k][j]);
             um = ref_max(um, sh[4][j]);
+            bad = ref_max(bad, sh[5][j]);
         }
-        for (int k = 0; k < 4; ++k)
-            if (w[k] == w[k]) atomicMin(&red->k[k], ord_key(w[k]));
-        if (um == um) atomicMax(&red->k[4], ord_key(um));
+        const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        for (int k = 0; k < 4; ++k) part[b].w[k] = w[k];
+        part[b].um = um;
+        part[b].bad = bad;
     }
 }
 
@@ -329,27 +340,48 @@ struct DevScalars {
     double nit;         // nIt
     double umach;
     double wmin[4];
-    CflRed red;
     unsigned int nonfinite;
     unsigned int pad;
 };
 
-__global__ void cfl_init_kernel(DevScalars* s)
+// Reduces the per-block partials in a fixed order (one block of 256 threads:
+// strided per-thread reduction, then the block reduction) and forms
+// DT = SFF * min(...) as 3dmhdsub.h:101-105.
+__global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __restrict__ part,
+                                                           int nparts, DevScalars* s, Phys c,
+                                                           int mag)
 {
-    for (int k = 0; k < 4; ++k) s->red.k[k] = ord_key(1e99);
-    s->red.k[4] = ord_key(0.0);
-}
-
-__global__ void cfl_finalize_kernel(DevScalars* s, Phys c, int mag)
-{
-    double w[4];
-    for (int k = 0; k < 4; ++k) w[k] = ord_val(s->red.k[k]);
-    s->umach = sqrt(ord_val(s->red.k[4]));
-    for (int k = 0; k < 4; ++k) s->wmin[k] = w[k];
-    if (mag)
-        s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), ref_min(w[2], w[3]));
-    else
-        s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), w[2]);
+    __shared__ double sh[6][256];
+    double w[4] = {1e99, 1e99, 1e99, 1e99};
+    double um = 0.0, bad = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], part[b].w[k]);
+        um = ref_max(um, part[b].um);
+        bad = ref_max(bad, part[b].bad);
+    }
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = w[k];
+    sh[4][threadIdx.x] = um;
+    sh[5][threadIdx.x] = bad;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            for (int k = 0; k < 4; ++k)
+                sh[k][threadIdx.x] = ref_min(sh[k][threadIdx.x], sh[k][threadIdx.x + st]);
+            sh[4][threadIdx.x] = ref_max(sh[4][threadIdx.x], sh[4][threadIdx.x + st]);
+            sh[5][threadIdx.x] = ref_max(sh[5][threadIdx.x], sh[5][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; ++k) w[k] = sh[k][0];
+        s->umach = sqrt(sh[4][0]);
+        for (int k = 0; k < 4; ++k) s->wmin[k] = w[k];
+        if (sh[5][0] != 0.0) s->nonfinite = 1u;
+        if (mag)
+            s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), ref_min(w[2], w[3]));
+        else
+            s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), w[2]);
+    }
 }
 
 __global__ void set_dt_kernel(DevScalars* s, double dt) { s->dt = dt; }
@@ -360,23 +392,27 @@ __global__ void clock_kernel(DevScalars* s)
     s->nit += 1.0;
 }
 
-// Non-finite scan of a register (interior incl. walls).
-__global__ void nonfinite_kernel(Geom g, int nf, const double* __restrict__ U, DevScalars* s)
+// Non-finite scan of a register (interior incl. walls): one thread per
+// (x, y) column, looping over z.
+__global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,
+                                                        const double* __restrict__ U,
+                                                        DevScalars* s)
 {
-    const int64_t n = (int64_t)g.nPz * g.plane;
-    const int64_t base = g.plane;   // skip the top ghost plane
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= g.nPx || y >= g.nPy) return;
     bool bad = false;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(t % g.pitch);
-        if (x >= g.nPx) continue;
+    for (int z = 1 + blockIdx.z; z <= g.nPz; z += gridDim.z) {
+        const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
         for (int f = 0; f < nf; ++f) {
-            const double v = U[f * g.fs + base + t];
+            const double v = U[f * g.fs + i];
             if (!(v - v == 0.0)) bad = true;
         }
     }
     if (bad) atomicOr(&s->nonfinite, 1u);
 }
+
+__global__ void clear_flag_kernel(DevScalars* s) { s->nonfinite = 0u; }
 
 }  // namespace mhd
 

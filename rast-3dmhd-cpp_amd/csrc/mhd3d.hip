@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -32,11 +33,15 @@ struct mhd3d_ctx {
     double* Z = nullptr;
     double* F = nullptr;
     DevScalars* ds = nullptr;
+    CflPart* cfl_parts = nullptr;
+    int cfl_parts_cap = 0;
     double* host_stage = nullptr;   // pinned, one field
     hipStream_t stream = nullptr;
     int cur = 0;
     int device = 0;
     int kernel_mode = 0;            // 0 = z-streaming LDS kernel, 1 = point kernel
+    int stream_waves_mag = 1;       // register budget of the MHD streaming kernel
+    int stream_waves_hydro = 2;
     int64_t bytes = 0;
     std::string err;
     // timing
@@ -105,6 +110,15 @@ static Phys make_phys(const mhd3d_desc* d)
     c.GAMMA = d->GAMMA; c.OGAMMA = 1.0 / d->GAMMA; c.THETA = d->THETA; c.GRAV = d->GRAV;
     c.OMX = d->OMX; c.OMZ = d->OMZ; c.RM = d->RM; c.ORM = d->ORM; c.OBETA = d->OBETA;
     c.TP = d->TP; c.TB = d->TB;
+    c.ORE_C43 = d->ORE * d->C43;
+    c.C13_ORE = d->C13 * d->ORE;
+    c.OCV_REPR = d->OCV / d->REPR;
+    c.OCV_ORE = d->OCV * d->ORE;
+    c.OCV_ORE_C43 = c.OCV_ORE * d->C43;
+    c.OCV_ORE_2 = c.OCV_ORE * 2;
+    c.C43_OCV_ORE = d->C43 * d->OCV * d->ORE;
+    c.KJ = d->OBETA * d->ORM * d->OCV;
+    c.KJ2 = -2.0 * d->OBETA * d->ORM * d->OCV;
     c.izCon = d->izCon; c.iTCon = d->iTCon; c.iBCon = d->iBCon;
     c.xdim = (d->nPx + 2) > 3;   // `nx > ix + 1` (3dmhdsub.h:77)
     return c;
@@ -166,6 +180,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     ctx->bytes = (int64_t)(3 * reg + nmet * sizeof(double) + sizeof(DevScalars));
     const char* km = std::getenv("MHD3D_KERNEL");
     if (km && std::strcmp(km, "point") == 0) ctx->kernel_mode = 1;
+    const char* sw = std::getenv("MHD3D_MAG_WAVES");
+    if (sw && std::atoi(sw) == 2) ctx->stream_waves_mag = 2;
     *out = ctx;
     return MHD3D_OK;
 }
@@ -181,6 +197,7 @@ extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
     (void)hipFree(ctx->Z);
     (void)hipFree(ctx->F);
     (void)hipFree(ctx->ds);
+    (void)hipFree(ctx->cfl_parts);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -296,7 +313,8 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
         stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
     } else {
-        launch_stream_stage<MAG, ROT, STAGE>(a, ctx->stream);
+        launch_stream_stage<MAG, ROT, STAGE>(a, ctx->stream,
+                                             MAG ? ctx->stream_waves_mag : ctx->stream_waves_hydro);
     }
 }
 
@@ -351,15 +369,22 @@ static int run_stage(mhd3d_ctx* ctx, int k)
 static int run_cfl(mhd3d_ctx* ctx)
 {
     const Geom& g = ctx->g;
-    cfl_init_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
     dim3 blk(64, 4, 1);
-    int zchunks = g.nPz < 32 ? g.nPz : 32;
+    const int zchunks = g.nPz < 8 ? g.nPz : 8;
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, zchunks);
+    const int nparts = (int)(grd.x * grd.y * grd.z);
+    if (nparts > ctx->cfl_parts_cap) {
+        (void)hipFree(ctx->cfl_parts);
+        ctx->cfl_parts = nullptr;
+        HIPCHK(ctx, hipMalloc(&ctx->cfl_parts, sizeof(CflPart) * nparts));
+        ctx->cfl_parts_cap = nparts;
+    }
     if (ctx->d.lMag)
-        cfl_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], &ctx->ds->red);
+        cfl_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], ctx->cfl_parts);
     else
-        cfl_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], &ctx->ds->red);
-    cfl_finalize_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, ctx->c, ctx->d.lMag ? 1 : 0);
+        cfl_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], ctx->cfl_parts);
+    cfl_finalize_kernel<<<1, 256, 0, ctx->stream>>>(ctx->cfl_parts, nparts, ctx->ds, ctx->c,
+                                                   ctx->d.lMag ? 1 : 0);
     HIPCHK(ctx, hipGetLastError());
     return MHD3D_OK;
 }
@@ -407,8 +432,13 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
     HIPCHK(ctx, hipGetLastError());
     ctx->d.reserved = 0;
     DevScalars hs;
-    HIPCHK(ctx, hipMemsetAsync(&ctx->ds->nonfinite, 0, sizeof(unsigned int), ctx->stream));
-    nonfinite_kernel<<<256, 256, 0, ctx->stream>>>(ctx->g, ctx->nf, ctx->U[ctx->cur], ctx->ds);
+    {
+        const Geom& g = ctx->g;
+        clear_flag_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
+        dim3 blk(64, 4, 1);
+        dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz < 8 ? g.nPz : 8);
+        nonfinite_kernel<<<grd, blk, 0, ctx->stream>>>(g, ctx->nf, ctx->U[ctx->cur], ctx->ds);
+    }
     HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     int rc = collect_timing(ctx);

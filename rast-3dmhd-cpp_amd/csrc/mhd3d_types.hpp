@@ -34,6 +34,18 @@ struct Phys {
     double RE, ORE, REPR, CV, OCV, GAMMA, OGAMMA, THETA, GRAV;
     double OMX, OMZ, RM, ORM, OBETA;
     double TP, TB;
+    // products of the scalars above that fluxes() forms before its sweeps
+    // (3dmhdsub.h:315,462,479-536,604-668), evaluated once on the host with
+    // the same double operations so the kernels read them as constants
+    double ORE_C43;       // ORE * C43
+    double C13_ORE;       // C13 * ORE
+    double OCV_REPR;      // OCV / REPR
+    double OCV_ORE;       // OCV * ORE
+    double OCV_ORE_C43;   // (OCV * ORE) * C43
+    double OCV_ORE_2;     // (OCV * ORE) * 2
+    double C43_OCV_ORE;   // (C43 * OCV) * ORE
+    double KJ;            // (OBETA * ORM) * OCV
+    double KJ2;           // ((-2 * OBETA) * ORM) * OCV
     int izCon, iTCon, iBCon, xdim;
 };
 
