@@ -37,6 +37,15 @@
 #include <omp.h>
 #endif
 
+/* Planted-mistake switches for the tests that show the mathematical checks
+ * catch a dropped term, a wrong sign, a transposed operand or a wrong index
+ * (tests/test_oracle_mutants.py builds the library with -DORC_MUTANT=k).
+ * The default build has ORC_MUTANT == 0 and none of them is active. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(k) (ORC_MUTANT == (k))
+
 enum { SET = 0, ADD = 1, SUB = 2 };
 enum { DZ = 0, DY = 1, DX = 2 };
 
@@ -398,13 +407,13 @@ void orc_fluxes(orc_ctx* c)
 
     /* cross-viscous, viscous heating, pressure work :475-538 */
     f_d1(c, SET, DX, REY, WW1, UU, 1);
-    f_d1(c, SET, DX, REY, WW2, VV, 1);
+    if (MUT(5)) f_d1(c, SET, DY, R, WW2, VV, 1); else f_d1(c, SET, DX, REY, WW2, VV, 1);
     if (!shr) {
         f_d1(c, SET, DX, R, WW3, WW, 1);
         tmp = OCV * ORE;
         f_mul3(c, ADD, R, FT, RI, WW1, WW1, tmp * C43);
         f_mul3(c, ADD, R, FT, RI, WW2, WW2, tmp);
-        f_mul3(c, ADD, R, FT, RI, WW3, WW3, tmp);
+        if (!MUT(4)) f_mul3(c, ADD, R, FT, RI, WW3, WW3, tmp);
         f_mul2(c, SUB, R, FT, WW1, TT, OCV);
     }
     f_d1(c, ADD, DY, R, FU, WW2, C13 * ORE);
@@ -476,7 +485,7 @@ void orc_fluxes(orc_ctx* c)
         f_d1(c, SET, DY, R, S3, WW, 1);
         f_mul2(c, SET, R, WW1, BY, S1, 1);
         f_mul2(c, SUB, R, WW1, BX, S2, 1);
-        f_mul2(c, ADD, R, WW3, BY, S3, 1);
+        f_mul2(c, MUT(3) ? SUB : ADD, R, WW3, BY, S3, 1);
         f_mul2(c, SUB, R, WW3, BZ, S2, 1);
         f_d1(c, SET, DZ, R, S1, UU, 1);
         f_d1(c, SET, DZ, R, S2, VV, 1);
@@ -495,7 +504,7 @@ void orc_fluxes(orc_ctx* c)
         f_d1(c, SET, DX, R, S2, BY, 1);
         f_d1(c, SET, DX, R, S3, BZ, 1);
         f_d1(c, SET, DY, R, S4, BX, 1);
-        f_mul2(c, ADD, R, FU, BY, S4, OBETA);
+        f_mul2(c, ADD, R, FU, MUT(2) ? BZ : BY, S4, OBETA);
         f_mul2(c, SUB, R, FU, BY, S2, OBETA);
         f_mul2(c, SUB, R, FU, BZ, S3, OBETA);
         f_mul2(c, ADD, R, FV, BX, S2, OBETA);
@@ -504,7 +513,7 @@ void orc_fluxes(orc_ctx* c)
         f_mul3(c, ADD, R, FT, RI, S4, S4, KJ);
         f_mul3(c, ADD, R, FT, RI, S2, S2, KJ);
         f_mul3(c, ADD, R, FT, RI, S3, S3, KJ);
-        f_mul3(c, ADD, R, FT, RI, S2, S4, KJ2);
+        f_mul3(c, ADD, R, FT, RI, S2, S4, MUT(1) ? -KJ2 : KJ2);
         f_mul2(c, SUB, R, WW1, UU, S1, 1);
         f_mul2(c, SUB, R, WW1, VV, S4, 1);
         f_mul2(c, SUB, R, WW2, UU, S2, 1);
@@ -537,7 +546,7 @@ void orc_fluxes(orc_ctx* c)
                     S1[IX(zl, y, x)] = (-3.0 * b0 + 4.0 * b1 - b2) * hz * d2t +
                                        (2.0 * b0 - 5.0 * b1 + 4.0 * b2 - b3) * h2z * dt_ * dt_;
                     const double e0 = BX[IX(zr - 1, y, x)], e1 = BX[IX(zr - 2, y, x)];
-                    const double e2 = BX[IX(zr - 3, y, x)], e3 = BX[IX(zr - 4, y, x)];
+                    const double e2 = BX[IX(zr - 3, y, x)], e3 = BX[IX(MUT(6) ? zr - 3 : zr - 4, y, x)];
                     S1[IX(zr - 1, y, x)] = (3.0 * e0 - 4.0 * e1 + e2) * hz * d2b +
                                            (2.0 * e0 - 5.0 * e1 + 4.0 * e2 - e3) * h2z * db * db;
                 }
@@ -650,7 +659,7 @@ void orc_bcon(orc_ctx* c)
         const double DZ = d->ORZF / c->dzzdz[zr - 1];
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++)
-                TT[IX(zr - 1, y, x)] = C43 * TT[IX(zr - 2, y, x)] - C13 * TT[IX(zr - 3, y, x)] -
+                TT[IX(zr - 1, y, x)] = C43 * TT[IX(zr - 2, y, x)] - (MUT(7) ? C23 : C13) * TT[IX(zr - 3, y, x)] -
                                        C23 * DZ * d->THETA / c->RKAPA[zr - 1];
     }
     if (d->lMag && d->iBCon == 0)
@@ -687,7 +696,7 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
         LOOP(R, UU[i] += GAMMA * TT[i] + (BX[i] * BX[i] + BY[i] * BY[i] + BZ[i] * BZ[i]) * OBETA / RO[i] +
                          2.0 * sqrt(UU[i] * (GAMMA * TT[i] + (BX[i] * BX[i] + BY[i] * BY[i] + BZ[i] * BZ[i]) * OBETA / RO[i])));
     } else {
-        LOOP(R, UU[i] += GAMMA * TT[i] + 2.0 * sqrt(GAMMA * UU[i] * TT[i]));
+        LOOP(R, UU[i] += (MUT(8) ? 1.0 : GAMMA) * TT[i] + 2.0 * sqrt(GAMMA * UU[i] * TT[i]));
     }
     const double *DDX = c->DDX, *DDY = c->DDY, *DDZ = c->DDZ, *RKAPA = c->RKAPA;
     const double REPR = d->REPR, CV = d->CV, RE = d->RE, RM = d->RM;

@@ -26,30 +26,43 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        vp = C.c_void_p
-        dp = C.POINTER(C.c_double)
-        L.orc_create.restype = vp
-        L.orc_create.argtypes = [vp]
-        L.orc_destroy.argtypes = [vp]
-        L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_set.argtypes = [vp, C.c_char_p, dp]
-        L.orc_get.argtypes = [vp, C.c_char_p, dp]
-        for fn in ("orc_communicate", "orc_fluxes", "orc_bcon"):
-            getattr(L, fn).argtypes = [vp]
-        L.orc_cfl.restype = C.c_double
-        L.orc_cfl.argtypes = [vp, dp, dp]
-        L.orc_stage.argtypes = [vp, C.c_int, C.c_double]
-        L.orc_step.restype = C.c_double
-        L.orc_step.argtypes = [vp, C.c_int, C.c_double]
-        L.orc_clock.argtypes = [vp, C.POINTER(C.c_longlong), dp, dp]
-        L.orc_ran2.restype = C.c_double
-        L.orc_ran2.argtypes = [C.POINTER(C.c_int)]
-        L.orc_static_profile.restype = C.c_int
-        L.orc_static_profile.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
-        L.orc_static_rnd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def build_variant(out_path, defines=()):
+    """Compile the restatement with extra -D flags (planted-mistake builds)."""
+    cmd = ["gcc", "-O2", "-fPIC", "-std=c11", "-fopenmp", "-ffp-contract=off", "-shared",
+           *("-D" + d for d in defines), os.path.join(HERE, "mhd_oracle.c"), "-o", out_path, "-lm"]
+    subprocess.run(cmd, check=True)
+    return out_path
+
+
+def load(path):
+    """Open a build of the restatement and declare its entry points."""
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.orc_create.restype = vp
+    L.orc_create.argtypes = [vp]
+    L.orc_destroy.argtypes = [vp]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set.argtypes = [vp, C.c_char_p, dp]
+    L.orc_get.argtypes = [vp, C.c_char_p, dp]
+    for fn in ("orc_communicate", "orc_fluxes", "orc_bcon"):
+        getattr(L, fn).argtypes = [vp]
+    L.orc_cfl.restype = C.c_double
+    L.orc_cfl.argtypes = [vp, dp, dp]
+    L.orc_stage.argtypes = [vp, C.c_int, C.c_double]
+    L.orc_step.restype = C.c_double
+    L.orc_step.argtypes = [vp, C.c_int, C.c_double]
+    L.orc_clock.argtypes = [vp, C.POINTER(C.c_longlong), dp, dp]
+    L.orc_ran2.restype = C.c_double
+    L.orc_ran2.argtypes = [C.POINTER(C.c_int)]
+    L.orc_static_profile.restype = C.c_int
+    L.orc_static_profile.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
+    L.orc_static_rnd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp]
+    return L
 
 
 def _dp(a):

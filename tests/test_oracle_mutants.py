@@ -1,0 +1,53 @@
+"""Planted mistakes in the oracle are caught by the mathematical checks.
+
+Each mutant is the oracle rebuilt with -DORC_MUTANT=k (see the switch list in
+oracle/mhd_oracle.c): a flipped sign, a transposed operand, a dropped term, a
+derivative taken along the wrong axis, a wrong wall-stencil plane, a wrong
+BCON coefficient, a dropped factor in the CFL.  Every one must make at least
+one check of test_oracle_math.py fail, otherwise those checks would not pin
+what they claim to pin.  Runs on the CPU only.
+"""
+import pytest
+
+import test_oracle_math as tm
+from oracle import oracle as om
+
+MUTANTS = {
+    1: "Joule heating cross term with the wrong sign",
+    2: "Lorentz force FU with BZ in place of BY",
+    3: "induction WW3 += BY*D1y(w) subtracted",
+    4: "viscous heating (D1x w)^2 term dropped",
+    5: "D1x(v) taken along y",
+    6: "bottom-wall one-sided D2z(BX) reading the wrong plane",
+    7: "izCon=1 bottom temperature with C23 in place of C13",
+    8: "CFL fast speed without GAMMA",
+}
+
+
+def _checks():
+    yield lambda: tm.test_mhd_fluxes_equal_continuum_for_polynomial_fields(1)
+    yield tm.test_wall_one_sided_stencils
+    yield tm.test_d2z_one_sided_is_exact_for_cubics
+    yield tm.test_conservative_fluxes_telescope
+    for iz, it in [(0, 0), (1, 1), (1, 3), (0, 4)]:
+        yield lambda iz=iz, it=it: tm.test_bcon_variants(iz, it)
+    yield lambda: tm.test_cfl_advective_limit(False)
+    yield lambda: tm.test_cfl_advective_limit(True)
+
+
+def test_clean_oracle_passes_every_check():
+    for chk in _checks():
+        chk()
+
+
+@pytest.mark.parametrize("k", sorted(MUTANTS), ids=[MUTANTS[k] for k in sorted(MUTANTS)])
+def test_planted_mistake_is_caught(k, tmp_path, monkeypatch):
+    path = om.build_variant(str(tmp_path / f"libmut{k}.so"), [f"ORC_MUTANT={k}"])
+    monkeypatch.setattr(om, "_lib", om.load(path))
+    caught = 0
+    for chk in _checks():
+        try:
+            chk()
+        except AssertionError:
+            caught += 1
+    assert caught >= 1, MUTANTS[k]
