@@ -39,6 +39,8 @@ struct GAcc {
     __device__ __forceinline__ double p(int a, int b, int c) const { return ro(a, b, c) * tt(a, b, c); }
 };
 
+struct CflPart;
+
 struct StageArgs {
     Geom g;
     Phys c;
@@ -47,6 +49,11 @@ struct StageArgs {
     double* __restrict__ Uout;
     double* __restrict__ Z;
     const double* __restrict__ dt;
+    // stage 3 only: when non-null, the kernel also evaluates the CFL limits
+    // of the new state (the TSTEP of the next step, 3dmhdsub.h:36-106) and
+    // writes one partial record per block at cfl_parts[cfl_base + block]
+    CflPart* cfl_parts;
+    int cfl_base;
 };
 
 __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int yg, int zg)
@@ -74,6 +81,101 @@ __device__ __forceinline__ double fsel(const FluxOut& o, int f)
     case 5: return o.B1;
     case 6: return o.B2;
     default: return o.B3;
+    }
+}
+
+// Per-point TSTEP quantities from the state values (3dmhdsub.h:41-88).
+__device__ __forceinline__ void cfl_values(const Phys& c, const Metric& m, bool mag, double RU,
+                                           double RV, double RW, double RO, double TT,
+                                           double BX, double BY, double BZ, int xg, int yg,
+                                           int z, double w[4], double& um)
+{
+    double UU = (RU * RU + RV * RV + RW * RW) / (RO * RO);
+    um = c.OGAMMA * UU / TT;
+    if (mag) {
+        const double Q = c.GAMMA * TT + (BX * BX + BY * BY + BZ * BZ) * c.OBETA / RO;
+        UU += Q + 2.0 * sqrt(UU * Q);
+    } else {
+        UU += c.GAMMA * TT + 2.0 * sqrt(c.GAMMA * UU * TT);
+    }
+    double DD = ref_min(m.DDY[yg], m.DDZ[z]);
+    if (c.xdim) DD = ref_min(DD, m.DDX[xg]);
+    w[0] = DD / sqrt(UU);
+    w[1] = 0.5 * DD * DD * c.REPR * c.CV * RO / m.RKAPA[z];
+    w[2] = 0.375 * DD * DD * c.RE * RO;
+    w[3] = mag ? 0.5 * DD * DD * c.RM : 1e99;
+}
+
+__device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Metric& m,
+                                          const double* __restrict__ U, bool mag, int x, int y,
+                                          int z, double w[4], double& um)
+{
+    const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
+    const int64_t fs = g.fs;
+    const double bx = mag ? U[5 * fs + i] : 0.0;
+    const double by = mag ? U[6 * fs + i] : 0.0;
+    const double bz = mag ? U[7 * fs + i] : 0.0;
+    cfl_values(c, m, mag, U[i], U[fs + i], U[2 * fs + i], U[3 * fs + i], U[4 * fs + i], bx, by,
+               bz, x + 1, y + 1, z, w, um);
+}
+
+// Running CFL extrema of one thread; NaN candidates never replace the
+// running value (std::min/std::max of 3dmhdutils.h:61-95).
+struct CflAcc {
+    double w[4] = {1e99, 1e99, 1e99, 1e99};
+    double um = 0.0, bad = 0.0;
+    __device__ __forceinline__ void add(const double wp[4], double up)
+    {
+        for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], wp[k]);
+        um = ref_max(um, up);
+        if (!(wp[0] - wp[0] == 0.0) || !(wp[1] - wp[1] == 0.0)) bad = 1.0;
+    }
+};
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = ref_min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = ref_max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One partial record per block: minima of WW1, WW2, WW3, VV and max of
+// OGAMMA*UU/TT over the block's points, plus a non-finite flag.
+struct CflPart {
+    double w[4];
+    double um;
+    double bad;
+};
+
+// Block reduction of CflAcc into part[b] (wave shuffles, then LDS).
+__device__ __forceinline__ void cfl_block_reduce(CflAcc& a, CflPart* part, int b)
+{
+    __shared__ double sh[6][16];
+    for (int k = 0; k < 4; ++k) a.w[k] = wave_min(a.w[k]);
+    a.um = wave_max(a.um);
+    a.bad = wave_max(a.bad);
+    const int tid = (threadIdx.z * blockDim.y + threadIdx.y) * blockDim.x + threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        for (int k = 0; k < 4; ++k) sh[k][wv] = a.w[k];
+        sh[4][wv] = a.um;
+        sh[5][wv] = a.bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) / 64;
+        for (int j = 1; j < nw; ++j) {
+            for (int k = 0; k < 4; ++k) a.w[k] = ref_min(a.w[k], sh[k][j]);
+            a.um = ref_max(a.um, sh[4][j]);
+            a.bad = ref_max(a.bad, sh[5][j]);
+        }
+        for (int k = 0; k < 4; ++k) part[b].w[k] = a.w[k];
+        part[b].um = a.um;
+        part[b].bad = a.bad;
     }
 }
 
@@ -116,61 +218,76 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const bool top = blockIdx.z == 0;
-    if (x >= a.g.nPx || y >= a.g.nPy) return;
-    const int z = top ? 1 : a.g.nPz;
-    const int zi1 = top ? z + 1 : z - 1;   // first plane inward
-    const int zi2 = top ? z + 2 : z - 2;   // second plane inward
-    GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
-    const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
-    FluxOut o;
-    if (top)
-        flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
-    else
-        flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
-    const double DT = *a.dt;
-    const double cg = a.c.GAM[STAGE - 1] * DT;
-    const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
-    const int64_t row = (int64_t)y * a.g.pitch + x;
-    const int64_t i = (int64_t)z * a.g.plane + row;
-    const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
-    const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
-    const int64_t fs = a.g.fs;
+    // every thread reaches the block reduction below; threads outside the
+    // domain only contribute the identity
+    const bool active = x < a.g.nPx && y < a.g.nPy;
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
-    double un[NF];
+    CflAcc cfl;
+    if (active) {
+        const int z = top ? 1 : a.g.nPz;
+        const int zi1 = top ? z + 1 : z - 1;   // first plane inward
+        const int zi2 = top ? z + 2 : z - 2;   // second plane inward
+        GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+        const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+        FluxOut o;
+        if (top)
+            flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
+        else
+            flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+        const double DT = *a.dt;
+        const double cg = a.c.GAM[STAGE - 1] * DT;
+        const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
+        const int64_t row = (int64_t)y * a.g.pitch + x;
+        const int64_t i = (int64_t)z * a.g.plane + row;
+        const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
+        const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
+        const int64_t fs = a.g.fs;
+        double un[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        const double zin = STAGE == 1 ? a.Uin[f * fs + i] : a.Z[f * fs + i];
-        un[f] = zin + cg * fsel(o, f);
-    }
-    // BCON (3dmhdsub.h:727-872); the updated planes inward are in Uout
-    const Phys& c = a.c;
-    const double* Un = a.Uout;
-    const double ro1 = Un[3 * fs + i1], ro2 = Un[3 * fs + i2];
-    un[0] = un[3] * (c.C43 * Un[0 * fs + i1] / ro1 - c.C13 * Un[0 * fs + i2] / ro2);
-    un[1] = un[3] * (c.C43 * Un[1 * fs + i1] / ro1 - c.C13 * Un[1 * fs + i2] / ro2);
-    un[2] = 0.0;
-    if (top) {
-        if (c.iTCon == 0 || c.iTCon == 4) {
-            un[4] = 1.0;
-            if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
-        } else if (c.iTCon == 1) {
-            const double DZ = c.ORZF / a.m.dzzdz[z];
-            un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] - c.C23 * DZ * c.THETA;
+        for (int f = 0; f < NF; ++f) {
+            const double zin = STAGE == 1 ? a.Uin[f * fs + i] : a.Z[f * fs + i];
+            un[f] = zin + cg * fsel(o, f);
         }
-    } else {
-        if (c.izCon == 0) {
-            un[4] = c.TB;
+        // BCON (3dmhdsub.h:727-872); the updated planes inward are in Uout
+        const Phys& c = a.c;
+        const double* Un = a.Uout;
+        const double ro1 = Un[3 * fs + i1], ro2 = Un[3 * fs + i2];
+        un[0] = un[3] * (c.C43 * Un[0 * fs + i1] / ro1 - c.C13 * Un[0 * fs + i2] / ro2);
+        un[1] = un[3] * (c.C43 * Un[1 * fs + i1] / ro1 - c.C13 * Un[1 * fs + i2] / ro2);
+        un[2] = 0.0;
+        if (top) {
+            if (c.iTCon == 0 || c.iTCon == 4) {
+                un[4] = 1.0;
+                if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
+            } else if (c.iTCon == 1) {
+                const double DZ = c.ORZF / a.m.dzzdz[z];
+                un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] - c.C23 * DZ * c.THETA;
+            }
         } else {
-            const double DZ = c.ORZF / a.m.dzzdz[z];
-            un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] -
-                    c.C23 * DZ * c.THETA / a.m.RKAPA[z];
+            if (c.izCon == 0) {
+                un[4] = c.TB;
+            } else {
+                const double DZ = c.ORZF / a.m.dzzdz[z];
+                un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] -
+                        c.C23 * DZ * c.THETA / a.m.RKAPA[z];
+            }
+        }
+        if (MAG && c.iBCon == 0) un[7] = 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            a.Uout[f * fs + i] = un[f];
+            if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
+        }
+        if (STAGE == 3 && a.cfl_parts) {
+            double wp[4], up;
+            cfl_values(c, a.m, MAG, un[0], un[1], un[2], un[3], un[4], MAG ? un[5] : 0.0,
+                       MAG ? un[6] : 0.0, MAG ? un[7] : 0.0, x + 1, y + 1, z, wp, up);
+            cfl.add(wp, up);
         }
     }
-    if (MAG && c.iBCon == 0) un[7] = 0.0;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        a.Uout[f * fs + i] = un[f];
-        if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
+    if (STAGE == 3 && a.cfl_parts) {
+        const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + b);
     }
 }
 
@@ -244,99 +361,30 @@ __global__ void bcon_kernel(Geom g, Phys c, Metric m, double* __restrict__ U)
 // the partials.  Min and max are exact, so the result does not depend on the
 // reduction order.
 
-__device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Metric& m,
-                                          const double* __restrict__ U, bool mag, int x, int y,
-                                          int z, double w[4], double& um)
-{
-    const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
-    const int64_t fs = g.fs;
-    const double RU = U[i], RV = U[fs + i], RW = U[2 * fs + i], RO = U[3 * fs + i];
-    const double TT = U[4 * fs + i];
-    double UU = (RU * RU + RV * RV + RW * RW) / (RO * RO);
-    um = c.OGAMMA * UU / TT;
-    if (mag) {
-        const double BX = U[5 * fs + i], BY = U[6 * fs + i], BZ = U[7 * fs + i];
-        const double Q = c.GAMMA * TT + (BX * BX + BY * BY + BZ * BZ) * c.OBETA / RO;
-        UU += Q + 2.0 * sqrt(UU * Q);
-    } else {
-        UU += c.GAMMA * TT + 2.0 * sqrt(c.GAMMA * UU * TT);
-    }
-    double DD = ref_min(m.DDY[y + 1], m.DDZ[z]);
-    if (c.xdim) DD = ref_min(DD, m.DDX[x + 1]);
-    w[0] = DD / sqrt(UU);
-    w[1] = 0.5 * DD * DD * c.REPR * c.CV * RO / m.RKAPA[z];
-    w[2] = 0.375 * DD * DD * c.RE * RO;
-    w[3] = mag ? 0.5 * DD * DD * c.RM : 1e99;
-}
-
-__device__ __forceinline__ double wave_min(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = ref_min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = ref_max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// One partial record per block: minima of WW1, WW2, WW3, VV and max of
-// OGAMMA*UU/TT over the block's points, plus a non-finite flag.
-struct CflPart {
-    double w[4];
-    double um;
-    double bad;
-};
-
 template <bool MAG>
 __global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
                                                   const double* __restrict__ U,
                                                   CflPart* __restrict__ part)
 {
-    __shared__ double sh[6][4];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    double w[4] = {1e99, 1e99, 1e99, 1e99};
-    double um = 0.0, bad = 0.0;
+    CflAcc acc;
     if (x < g.nPx && y < g.nPy) {
         for (int z = 1 + blockIdx.z; z <= g.nPz; z += gridDim.z) {
             double wp[4], up;
             cfl_point(g, c, m, U, MAG, x, y, z, wp, up);
-            // NaN candidates never replace the running value (std::min/max)
-            for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], wp[k]);
-            um = ref_max(um, up);
-            if (!(wp[0] - wp[0] == 0.0) || !(wp[1] - wp[1] == 0.0)) bad = 1.0;
+            acc.add(wp, up);
         }
     }
-    for (int k = 0; k < 4; ++k) w[k] = wave_min(w[k]);
-    um = wave_max(um);
-    bad = wave_max(bad);
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    const int wv = tid >> 6, lane = tid & 63;
-    if (lane == 0) {
-        for (int k = 0; k < 4; ++k) sh[k][wv] = w[k];
-        sh[4][wv] = um;
-        sh[5][wv] = bad;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int nw = (blockDim.x * blockDim.y + 63) / 64;
-        for (int j = 1; j < nw; ++j) {
-            for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], sh[k][j]);
-            um = ref_max(um, sh[4][j]);
-            bad = ref_max(bad, sh[5][j]);
-        }
-        const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        for (int k = 0; k < 4; ++k) part[b].w[k] = w[k];
-        part[b].um = um;
-        part[b].bad = bad;
-    }
+    const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    cfl_block_reduce(acc, part, b);
 }
 
 // Device scalars of one context.
 struct DevScalars {
     double dt;          // DT of the current step
     double timc;        // TIMC (3dmhdsub.h:151)
+    double dt_last;     // DT of the last completed step
     double nit;         // nIt
     double umach;
     double wmin[4];
@@ -390,6 +438,7 @@ __global__ void clock_kernel(DevScalars* s)
 {
     s->timc += s->dt;
     s->nit += 1.0;
+    s->dt_last = s->dt;
 }
 
 // Non-finite scan of a register (interior incl. walls): one thread per

@@ -14,8 +14,9 @@
 // registers.  The flux is
 // the shared flux_point() (same operation order as the reference), followed by
 // the fused step_block_GAM / step_block_ZETA update (3dmhdsub.h:168-192).
-// Two barriers per plane; global loads of the next plane are issued before
-// the flux evaluation so they overlap it.
+// One barrier per plane (four-slot ring, three centre-plane buffers) or two
+// (three-slot ring, two buffers); global loads of the next plane are issued
+// before the flux evaluation so they overlap it.
 #pragma once
 #pragma clang fp contract(off)
 
@@ -90,18 +91,21 @@ struct SAcc {
 
 // WAVES: minimum waves per SIMD the register allocation must allow (2 keeps
 // 256 VGPRs per wave; 1 lets the MHD variant use AGPRs instead of spilling).
-template <bool MAG, bool ROT, int STAGE, int WAVES>
+template <bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER>
 __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int ntx,
                                                                   int nty, int xcd_remap)
 {
     constexpr int NR = MAG ? 6 : 3;   // ring fields: u v w [BX BY BZ]
     constexpr int NS = 5;             // centre-plane fields: RU RV RW TT P
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
-    // with the 2-wave register budget the MHD variant issues the next-plane
-    // loads after the flux evaluation (registers); otherwise before it
-    constexpr bool PREFETCH_LATE = MAG && WAVES == 2;
-    __shared__ double s_uvw[3][NR][SHP];
-    __shared__ double s_st[2][NS][SHP];
+    // ring depths: NRING planes of u, v, w (BX, BY, BZ) and NSTAR centre
+    // planes, deep enough that one barrier per plane suffices (a thread can be
+    // at most one plane ahead of another, and then it writes slots nobody
+    // still reads)
+    constexpr int NRING = ONE_BARRIER ? 4 : 3;
+    constexpr int NSTAR = ONE_BARRIER ? 3 : 2;
+    __shared__ double s_uvw[NRING][NR][SHP];
+    __shared__ double s_st[NSTAR][NS][SHP];
 
     const Geom& g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -119,7 +123,15 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     const int x0 = tile_x * STX, y0 = tile_y * STY;
     const int zb = 2 + chunk * zchunk;
     const int ze = min(zb + zchunk, g.nPz);   // planes zb .. ze-1 (interior 2 .. nPz-1)
-    if (zb >= ze) return;
+    if (zb >= ze) {
+        if (STAGE == 3 && a.cfl_parts && threadIdx.x == 0 && threadIdx.y == 0) {
+            CflPart& q = a.cfl_parts[a.cfl_base + blockIdx.x];
+            for (int k = 0; k < 4; ++k) q.w[k] = 1e99;
+            q.um = 0.0;
+            q.bad = 0.0;
+        }
+        return;
+    }
 
     // own column and (for t < SNH) one halo point, wrapped periodically
     const int gx = (x0 + tx) % g.nPx, gy = (y0 + ty) % g.nPy;
@@ -162,13 +174,14 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
         for (int f = 0; f < NF; ++f) zpre[f] = a.Z[f * fs + (int64_t)zb * g.plane + go];
     }
 
+    CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
     for (int L = zb - 1; L <= ze; ++L) {
         // -- derive plane L and publish it
         {
             const double ri = 1.0 / own.ro;
             const double u = own.ru * ri, v = own.rv * ri, w = own.rw * ri, P = own.ro * own.tt;
-            double* su = &s_uvw[L % 3][0][0];
-            double* ss = &s_st[L & 1][0][0];
+            double* su = &s_uvw[L % NRING][0][0];
+            double* ss = &s_st[L % NSTAR][0][0];
             su[0 * SHP + pc] = u;
             su[1 * SHP + pc] = v;
             su[2 * SHP + pc] = w;
@@ -209,9 +222,8 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
             }
         }
         __syncthreads();
-        // -- prefetch plane L+1 (before the flux evaluation, so the loads overlap
-        //    it; PREFETCH_LATE issues them after it to keep registers free)
-        if (!PREFETCH_LATE && L < ze) {
+        // -- prefetch plane L+1 (before the flux evaluation, so the loads overlap it)
+        if (L < ze) {
             load_raw<MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
             if (has_halo) load_raw<MAG>(halo_n, U, fs, (int64_t)(L + 1) * g.plane + gh);
         }
@@ -226,10 +238,10 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
             }
         }
         if (z >= zb) {
-            acc.st = &s_st[z & 1][0][pc];
-            acc.um = &s_uvw[(z + 2) % 3][0][pc];
-            acc.u0 = &s_uvw[z % 3][0][pc];
-            acc.up = &s_uvw[(z + 1) % 3][0][pc];
+            acc.st = &s_st[z % NSTAR][0][pc];
+            acc.um = &s_uvw[(z - 1) % NRING][0][pc];
+            acc.u0 = &s_uvw[z % NRING][0][pc];
+            acc.up = &s_uvw[(z + 1) % NRING][0][pc];
             acc.ro0 = row[1];
             acc.ri0 = rinv[1];
             const PointMetric pm = point_metric(a.m, gx + 1, gy + 1, z);
@@ -244,24 +256,32 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
                     uc[6] = acc.u0[4 * SHP];
                     uc[7] = acc.u0[5 * SHP];
                 }
+                double un[NF];
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
                     const double F = fsel(o, f);
                     const double zin = STAGE == 1 ? uc[f] : zcur[f];
-                    const double un = zin + cg * F;
-                    a.Uout[f * fs + i] = un;
-                    if (STAGE < 3) a.Z[f * fs + i] = un + cz * F;
+                    un[f] = zin + cg * F;
+                    a.Uout[f * fs + i] = un[f];
+                    if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * F;
+                }
+                if (STAGE == 3 && a.cfl_parts) {
+                    double wp[4], up;
+                    if constexpr (MAG)
+                        cfl_values(a.c, a.m, true, un[0], un[1], un[2], un[3], un[4], un[5],
+                                   un[6], un[7], gx + 1, gy + 1, z, wp, up);
+                    else
+                        cfl_values(a.c, a.m, false, un[0], un[1], un[2], un[3], un[4], 0.0,
+                                   0.0, 0.0, gx + 1, gy + 1, z, wp, up);
+                    cfl.add(wp, up);
                 }
             }
         }
-        if (PREFETCH_LATE && L < ze) {
-            load_raw<MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
-            if (has_halo) load_raw<MAG>(halo_n, U, fs, (int64_t)(L + 1) * g.plane + gh);
-        }
-        __syncthreads();
+        if (!ONE_BARRIER) __syncthreads();
         own = own_n;
         if (has_halo) halo = halo_n;
     }
+    if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
 }
 
 struct StreamPlan {
@@ -287,16 +307,22 @@ inline StreamPlan stream_plan(const Geom& g, int target_blocks = 1024)
 }
 
 template <bool MAG, bool ROT, int STAGE>
-static void launch_stream_stage(const StageArgs& a, hipStream_t s, int waves)
+static void launch_stream_stage(const StageArgs& a, hipStream_t s, int waves, bool one_barrier)
 {
     const StreamPlan p = stream_plan(a.g);
     dim3 blk(STX, STY, 1);
-    if (waves == 1)
-        stage_stream_kernel<MAG, ROT, STAGE, 1><<<p.nblocks, blk, 0, s>>>(a, p.zchunk, p.ntx,
-                                                                           p.nty, p.xcd_remap);
+    if (waves == 1 && one_barrier)
+        stage_stream_kernel<MAG, ROT, STAGE, 1, true><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+    else if (waves == 1)
+        stage_stream_kernel<MAG, ROT, STAGE, 1, false><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+    else if (one_barrier)
+        stage_stream_kernel<MAG, ROT, STAGE, 2, true><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
     else
-        stage_stream_kernel<MAG, ROT, STAGE, 2><<<p.nblocks, blk, 0, s>>>(a, p.zchunk, p.ntx,
-                                                                           p.nty, p.xcd_remap);
+        stage_stream_kernel<MAG, ROT, STAGE, 2, false><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
 }
 
 }  // namespace mhd

@@ -35,6 +35,7 @@ struct mhd3d_ctx {
     DevScalars* ds = nullptr;
     CflPart* cfl_parts = nullptr;
     int cfl_parts_cap = 0;
+    bool dt_ready = false;          // ds->dt holds TSTEP of the current state
     double* host_stage = nullptr;   // pinned, one field
     hipStream_t stream = nullptr;
     int cur = 0;
@@ -42,6 +43,9 @@ struct mhd3d_ctx {
     int kernel_mode = 0;            // 0 = z-streaming LDS kernel, 1 = point kernel
     int stream_waves_mag = 1;       // register budget of the MHD streaming kernel
     int stream_waves_hydro = 2;
+    bool one_barrier_mag = true;    // streaming kernel synchronisation scheme (MHD)
+    bool one_barrier_hydro = false; //   (hydro: two barriers, 2 workgroups per CU)
+    bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
     int64_t bytes = 0;
     std::string err;
     // timing
@@ -182,6 +186,11 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (km && std::strcmp(km, "point") == 0) ctx->kernel_mode = 1;
     const char* sw = std::getenv("MHD3D_MAG_WAVES");
     if (sw && std::atoi(sw) == 2) ctx->stream_waves_mag = 2;
+    const char* ob = std::getenv("MHD3D_BARRIERS");
+    if (ob && std::atoi(ob) == 2) ctx->one_barrier_mag = false;
+    if (ob && std::atoi(ob) == 1) ctx->one_barrier_hydro = true;
+    const char* fc = std::getenv("MHD3D_FUSE_CFL");
+    if (fc && std::atoi(fc) == 0) ctx->fuse_cfl = false;
     *out = ctx;
     return MHD3D_OK;
 }
@@ -253,6 +262,7 @@ static int download_field(mhd3d_ctx* ctx, double* dst, const double* src)
 extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
 {
     if (!ctx || !fields) return MHD3D_EINVAL;
+    ctx->dt_ready = false;
     for (int f = 0; f < ctx->nf; ++f)
         if (!fields[f]) return fail(ctx, MHD3D_EINVAL, "null field pointer");
     for (int f = 0; f < ctx->nf; ++f) {
@@ -291,6 +301,8 @@ static StageArgs stage_args(mhd3d_ctx* ctx)
     a.Uout = ctx->U[1 - ctx->cur];
     a.Z = ctx->Z;
     a.dt = &ctx->ds->dt;
+    a.cfl_parts = nullptr;
+    a.cfl_base = 0;
     return a;
 }
 
@@ -314,7 +326,8 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
         stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
     } else {
         launch_stream_stage<MAG, ROT, STAGE>(a, ctx->stream,
-                                             MAG ? ctx->stream_waves_mag : ctx->stream_waves_hydro);
+                                             MAG ? ctx->stream_waves_mag : ctx->stream_waves_hydro,
+                                             MAG ? ctx->one_barrier_mag : ctx->one_barrier_hydro);
     }
 }
 
@@ -344,9 +357,39 @@ static void launch_stage_k(mhd3d_ctx* ctx, const StageArgs& a, bool wall)
     }
 }
 
-static int run_stage(mhd3d_ctx* ctx, int k)
+// Number of CFL partial records the fused stage-3 launch writes: one per
+// block of the streaming kernel (or of the point kernel) and of the wall kernel.
+static void cfl_part_layout(const mhd3d_ctx* ctx, int& nstream, int& nwall)
+{
+    const Geom& g = ctx->g;
+    if (ctx->kernel_mode == 1)
+        nstream = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * (g.nPz - 2);
+    else
+        nstream = stream_plan(g).nblocks;
+    nwall = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * 2;
+}
+
+static int ensure_cfl_parts(mhd3d_ctx* ctx, int n)
+{
+    if (n > ctx->cfl_parts_cap) {
+        (void)hipFree(ctx->cfl_parts);
+        ctx->cfl_parts = nullptr;
+        HIPCHK(ctx, hipMalloc(&ctx->cfl_parts, sizeof(CflPart) * n));
+        ctx->cfl_parts_cap = n;
+    }
+    return MHD3D_OK;
+}
+
+static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
 {
     StageArgs a = stage_args(ctx);
+    if (fused_cfl && k == 3 && ctx->kernel_mode == 0) {
+        int ns, nw;
+        cfl_part_layout(ctx, ns, nw);
+        int rc = ensure_cfl_parts(ctx, ns + nw);
+        if (rc) return rc;
+        a.cfl_parts = ctx->cfl_parts;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
         e0 = next_event(ctx);
@@ -358,6 +401,11 @@ static int run_stage(mhd3d_ctx* ctx, int k)
     else if (k == 2) launch_stage_k<2>(ctx, a, false);
     else launch_stage_k<3>(ctx, a, false);
     if (ctx->timing) HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+    if (a.cfl_parts) {
+        int ns, nw;
+        cfl_part_layout(ctx, ns, nw);
+        a.cfl_base = ns;
+    }
     if (k == 1) launch_stage_k<1>(ctx, a, true);
     else if (k == 2) launch_stage_k<2>(ctx, a, true);
     else launch_stage_k<3>(ctx, a, true);
@@ -373,12 +421,8 @@ static int run_cfl(mhd3d_ctx* ctx)
     const int zchunks = g.nPz < 8 ? g.nPz : 8;
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, zchunks);
     const int nparts = (int)(grd.x * grd.y * grd.z);
-    if (nparts > ctx->cfl_parts_cap) {
-        (void)hipFree(ctx->cfl_parts);
-        ctx->cfl_parts = nullptr;
-        HIPCHK(ctx, hipMalloc(&ctx->cfl_parts, sizeof(CflPart) * nparts));
-        ctx->cfl_parts_cap = nparts;
-    }
+    int rc = ensure_cfl_parts(ctx, nparts);
+    if (rc) return rc;
     if (ctx->d.lMag)
         cfl_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], ctx->cfl_parts);
     else
@@ -411,24 +455,33 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
         set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, *dt_inout);
         HIPCHK(ctx, hipGetLastError());
     }
-    hipEvent_t s0 = nullptr, s1 = nullptr;
+    // TSTEP of each step is evaluated on the state the previous step's
+    // stage 3 produced, inside that launch (fused); a standalone pass runs
+    // only when no fused value for the current state exists.
+    const bool fuse = !fixed_dt && ctx->kernel_mode == 0 && ctx->fuse_cfl;
     for (int n = 0; n < nsteps; ++n) {
-        if (!fixed_dt) {
+        if (!fixed_dt && !(fuse && ctx->dt_ready)) {
             int rc = run_cfl(ctx);
             if (rc) return rc;
         }
         for (int k = 1; k <= 3; ++k) {
-            int rc = run_stage(ctx, k);
+            int rc = run_stage(ctx, k, fuse);
             if (rc) return rc;
         }
         clock_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
+        if (fuse) {
+            int ns, nw;
+            cfl_part_layout(ctx, ns, nw);
+            cfl_finalize_kernel<<<1, 256, 0, ctx->stream>>>(ctx->cfl_parts, ns + nw, ctx->ds,
+                                                           ctx->c, ctx->d.lMag ? 1 : 0);
+        }
+        ctx->dt_ready = fuse;
         // bound the event pool: fold timings every 64 steps
         if (ctx->timing && ctx->ev_used >= 384) {
             int rc = collect_timing(ctx);
             if (rc) return rc;
         }
     }
-    (void)s0; (void)s1;
     HIPCHK(ctx, hipGetLastError());
     ctx->d.reserved = 0;
     DevScalars hs;
@@ -443,7 +496,7 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     int rc = collect_timing(ctx);
     if (rc) return rc;
-    if (dt_inout) *dt_inout = hs.dt;
+    if (dt_inout && nsteps > 0) *dt_inout = hs.dt_last;
     if (hs.nonfinite) return fail(ctx, MHD3D_ENONFINITE, "state is not finite after the step");
     return MHD3D_OK;
 }
@@ -451,6 +504,7 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
 extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
 {
     if (!ctx || stage < 1 || stage > 3) return MHD3D_EINVAL;
+    ctx->dt_ready = false;
     set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt);
     int rc = run_stage(ctx, stage);
     if (rc) return rc;
@@ -497,6 +551,7 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
 extern "C" int mhd3d_bcon(mhd3d_ctx* ctx)
 {
     if (!ctx) return MHD3D_EINVAL;
+    ctx->dt_ready = false;
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
