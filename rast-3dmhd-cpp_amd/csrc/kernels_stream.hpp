@@ -25,6 +25,17 @@
 namespace mhd {
 
 constexpr int STX = 32, STY = 8;
+
+// Workgroup barrier ordering LDS only.  Threads of a workgroup exchange data
+// through LDS alone (every global location is read or written by one
+// thread), so the barrier need not wait for outstanding global loads and
+// stores, which __syncthreads() may do.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 constexpr int SHW = STX + 2, SHH = STY + 2, SHP = SHW * SHH;   // 34 x 10 = 340 points
 constexpr int SNH = 2 * SHW + 2 * STY;                          // 84 halo points
 
@@ -221,7 +232,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         // -- prefetch plane L+1 (before the flux evaluation, so the loads overlap it)
         if (L < ze) {
             load_raw<MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
@@ -277,7 +288,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
                 }
             }
         }
-        if (!ONE_BARRIER) __syncthreads();
+        if (!ONE_BARRIER) lds_barrier();
         own = own_n;
         if (has_halo) halo = halo_n;
     }
