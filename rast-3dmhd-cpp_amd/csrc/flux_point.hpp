@@ -16,6 +16,17 @@
 //   ri = 1/RO, u = RU*ri, v = RV*ri, w = RW*ri, p = RO*TT   (3dmhdsub.h:267-276)
 // WALL = 0 (interior plane), 1 (top wall, z = zl), 2 (bottom wall, z = zr-1):
 // the one-sided stencils of 3dmhdsub.h:211-226, 572-584, 640-654, 673-687.
+//
+// DEV = true (the mixed form only): the accessor's ro/tt/p stencil values are
+// deviations from a fixed z-only background RO0(z), TT0(z), P0 = RO0*TT0, so
+// FP32 differences of a nearly hydrostatic state do not cancel.  x and y
+// differences of the background vanish; its z differences come from the
+// accessor, precomputed in FP64: dzt0() = D1z(TT0), d2zt0() = D2z(TT0) and
+// hb() = RO0*GRAV - D1z(P0) (the hydrostatic residual).  rodev() is the
+// centre density deviation and ttfull() the full centre temperature.  BX, BY,
+// BZ stencil values are deviations from BX0(z), BY0(z), BZ0(z) likewise, with
+// bfull(k) the full centre field and dzb(k), d2zb(k) the background's D1z and
+// D2z (one-sided at the walls for BX and BY, as the stencils below).
 #pragma once
 #pragma clang fp contract(off)
 
@@ -23,9 +34,12 @@
 
 namespace mhd {
 
-struct FluxOut {
-    double FU, FV, FW, FR, FT, B1, B2, B3;
+// The eight tendencies of one point, in the type the fluxes are evaluated in.
+template <typename T>
+struct FluxOutT {
+    T FU, FV, FW, FR, FT, B1, B2, B3;
 };
+using FluxOut = FluxOutT<double>;
 
 struct PointMetric {
     double dxx, d2xx, dyy, d2yy, dzz, d2zz, kap, dkap;
@@ -53,9 +67,9 @@ struct Ops {
     }
 };
 
-template <bool MAG, bool ROT, int WALL, typename T, typename A>
+template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, typename A>
 __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const PointMetric& m,
-                                           FluxOut& o)
+                                           FluxOutT<T>& o)
 {
     const Ops<T> op{T(m.dxx), T(m.d2xx), T(m.dyy), T(m.d2yy), T(m.dzz), T(m.d2zz),
                     T(c.hx),  T(c.h2x),  T(c.hy),  T(c.h2y),  T(c.hz),  T(c.h2z)};
@@ -77,7 +91,10 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     FR -= dzrw;
 
     // ---- gravity and pressure :266-272
-    FW = a.ro(0, 0, 0) * T(c.GRAV);
+    if constexpr (DEV)
+        FW = a.hb() + a.rodev() * T(c.GRAV);
+    else
+        FW = a.ro(0, 0, 0) * T(c.GRAV);
     FW -= op.d1z(a.p(0, 0, 1), a.p(0, 0, -1));
     FV = op.d1y(a.p(0, 1, 0), a.p(0, -1, 0)) * T(-1);
     FU = op.d1x(a.p(1, 0, 0), a.p(-1, 0, 0)) * T(-1);
@@ -95,12 +112,15 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
 
     const T RI = a.ri(0, 0, 0);
     const T U0 = a.u(0, 0, 0), V0 = a.v(0, 0, 0), W0 = a.w(0, 0, 0);
-    const T T0 = a.tt(0, 0, 0);
+    const T T0 = a.tt(0, 0, 0);   // stencil value (a deviation when DEV)
+    T T0f = T0;                    // full temperature, for products
+    if constexpr (DEV) T0f = a.ttfull();
 
     // ---- temperature advection :294-296 (func_d1_mul: (((df*du)*h)*g)*s)
     const T dxT = op.d1x(a.tt(1, 0, 0), a.tt(-1, 0, 0));
     const T dyT = op.d1y(a.tt(0, 1, 0), a.tt(0, -1, 0));
-    const T dzT = op.d1z(a.tt(0, 0, 1), a.tt(0, 0, -1));
+    T dzT = op.d1z(a.tt(0, 0, 1), a.tt(0, 0, -1));
+    if constexpr (DEV) dzT += a.dzt0();
     FT = (dxT * U0) * T(-1);
     FT -= dyT * V0;
     FT -= dzT * W0;
@@ -110,7 +130,9 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
     const T kap = T(m.kap);
     FT += ((op.d2x(a.tt(1, 0, 0), T0, a.tt(-1, 0, 0)) * RI) * kap) * tmp;
     FT += ((op.d2y(a.tt(0, 1, 0), T0, a.tt(0, -1, 0)) * RI) * kap) * tmp;
-    FT += ((op.d2z(a.tt(0, 0, 1), T0, a.tt(0, 0, -1)) * RI) * kap) * tmp;
+    T d2zT = op.d2z(a.tt(0, 0, 1), T0, a.tt(0, 0, -1));
+    if constexpr (DEV) d2zT += a.d2zt0();
+    FT += ((d2zT * RI) * kap) * tmp;
     FT += ((dzT * RI) * T(m.dkap)) * tmp;
 
     // ---- viscous :462-472
@@ -133,7 +155,7 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         FT += ((RI * dxu) * dxu) * T(c.OCV_ORE_C43);
         FT += ((RI * dxv) * dxv) * tmp;
         FT += ((RI * dxw) * dxw) * tmp;
-        FT -= (dxu * T0) * OCV;
+        FT -= (dxu * T0f) * OCV;
     }
     {
         // D1x(v), D1x(u) at y+-1 (WW2, WW1 over FORRANGE_EXT_Y, :475-476)
@@ -152,7 +174,7 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         FT += ((RI * dyu) * dxv) * T(c.OCV_ORE_2);
         tmp = T(c.C43_OCV_ORE);
         FT += ((RI * dyv) * dyv) * tmp;
-        FT -= (dyv * T0) * OCV;
+        FT -= (dyv * T0f) * OCV;
     }
     // D1z(w) at the point and at x+-1, y+-1 (WW3 over FORRANGE_EXT_XY, :505)
     const T dzw = op.d1z(a.w(0, 0, 1), a.w(0, 0, -1));
@@ -162,7 +184,7 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         FT += ((RI * dxu) * dzw) * (-tmp);
         FT += ((RI * dxu) * dyv) * (-tmp);
         FT += ((RI * dyv) * dzw) * (-tmp);
-        FT -= (dzw * T0) * OCV;
+        FT -= (dzw * T0f) * OCV;
     }
     {
         const T dzw_xp = op.d1z(a.w(1, 0, 1), a.w(1, 0, -1));
@@ -224,7 +246,14 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         const T OBETA = T(c.OBETA), ORM = T(c.ORM);
         const T KJ = T(c.KJ);
         const T KJ2 = T(c.KJ2);
-        const T BX = a.bx(0, 0, 0), BY = a.by(0, 0, 0), BZ = a.bz(0, 0, 0);
+        // centre stencil values (deviations when DEV) and full fields for products
+        const T BXs = a.bx(0, 0, 0), BYs = a.by(0, 0, 0), BZs = a.bz(0, 0, 0);
+        T BX = BXs, BY = BYs, BZ = BZs;
+        if constexpr (DEV) {
+            BX = a.bfull(0);
+            BY = a.bfull(1);
+            BZ = a.bfull(2);
+        }
         T s1, s2, s3, s4;
         // :548-556 (RU,RV,RW = D1x of u,v,w)
         B2 = BX * dxv;
@@ -271,16 +300,17 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         B3 -= U0 * s3;
         const T dxBZ = s3;
         // :614-624
-        s1 = op.d2x(a.bx(1, 0, 0), BX, a.bx(-1, 0, 0));
-        s2 = op.d2x(a.by(1, 0, 0), BY, a.by(-1, 0, 0));
-        s4 = op.d2y(a.bx(0, 1, 0), BX, a.bx(0, -1, 0));
+        s1 = op.d2x(a.bx(1, 0, 0), BXs, a.bx(-1, 0, 0));
+        s2 = op.d2x(a.by(1, 0, 0), BYs, a.by(-1, 0, 0));
+        s4 = op.d2y(a.bx(0, 1, 0), BXs, a.bx(0, -1, 0));
         B1 += s1 * ORM;
         B1 += s4 * ORM;
         B2 += s2 * ORM;
-        s4 = op.d2x(a.bz(1, 0, 0), BZ, a.bz(-1, 0, 0));
+        s4 = op.d2x(a.bz(1, 0, 0), BZs, a.bz(-1, 0, 0));
         B3 += s4 * ORM;
         // :626-636
         s1 = op.d1z(a.bx(0, 0, 1), a.bx(0, 0, -1));
+        if constexpr (DEV) s1 += a.dzb(0);
         s2 = op.d1y(a.by(0, 1, 0), a.by(0, -1, 0));
         FU += (BZ * s1) * OBETA;
         FW -= (BX * s1) * OBETA;
@@ -290,21 +320,23 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         B2 -= V0 * s2;
         // :638-658, one-sided D2z(BX) at the walls
         if (WALL == 1) {
-            const T b0 = BX, b1 = a.bx(0, 0, 1), b2 = a.bx(0, 0, 2), b3 = a.bx(0, 0, 3);
+            const T b0 = BXs, b1 = a.bx(0, 0, 1), b2 = a.bx(0, 0, 2), b3 = a.bx(0, 0, 3);
             s1 = (T(-3) * b0 + T(4) * b1 - b2) * T(c.hz) * op.d2zz +
                  (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
         } else if (WALL == 2) {
-            const T b0 = BX, b1 = a.bx(0, 0, -1), b2 = a.bx(0, 0, -2), b3 = a.bx(0, 0, -3);
+            const T b0 = BXs, b1 = a.bx(0, 0, -1), b2 = a.bx(0, 0, -2), b3 = a.bx(0, 0, -3);
             s1 = (T(3) * b0 - T(4) * b1 + b2) * T(c.hz) * op.d2zz +
                  (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
         } else {
-            s1 = op.d2z(a.bx(0, 0, 1), BX, a.bx(0, 0, -1));
+            s1 = op.d2z(a.bx(0, 0, 1), BXs, a.bx(0, 0, -1));
         }
-        s2 = op.d2y(a.by(0, 1, 0), BY, a.by(0, -1, 0));
+        if constexpr (DEV) s1 += a.d2zb(0);
+        s2 = op.d2y(a.by(0, 1, 0), BYs, a.by(0, -1, 0));
         B1 += s1 * ORM;
         B2 += s2 * ORM;
         // :659-671
         s2 = op.d1z(a.by(0, 0, 1), a.by(0, 0, -1));
+        if constexpr (DEV) s2 += a.dzb(1);
         s3 = op.d1y(a.bz(0, 1, 0), a.bz(0, -1, 0));
         s1 = s2;
         s1 -= s3;
@@ -317,33 +349,36 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         B3 -= V0 * s3;
         // :672-690, one-sided D2z(BY) at the walls
         if (WALL == 1) {
-            const T b0 = BY, b1 = a.by(0, 0, 1), b2 = a.by(0, 0, 2), b3 = a.by(0, 0, 3);
+            const T b0 = BYs, b1 = a.by(0, 0, 1), b2 = a.by(0, 0, 2), b3 = a.by(0, 0, 3);
             s2 = (T(-3) * b0 + T(4) * b1 - b2) * T(c.hz) * op.d2zz +
                  (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
         } else if (WALL == 2) {
-            const T b0 = BY, b1 = a.by(0, 0, -1), b2 = a.by(0, 0, -2), b3 = a.by(0, 0, -3);
+            const T b0 = BYs, b1 = a.by(0, 0, -1), b2 = a.by(0, 0, -2), b3 = a.by(0, 0, -3);
             s2 = (T(3) * b0 - T(4) * b1 + b2) * T(c.hz) * op.d2zz +
                  (T(2) * b0 - T(5) * b1 + T(4) * b2 - b3) * T(c.h2z) * op.dzz * op.dzz;
         } else {
-            s2 = op.d2z(a.by(0, 0, 1), BY, a.by(0, 0, -1));
+            s2 = op.d2z(a.by(0, 0, 1), BYs, a.by(0, 0, -1));
         }
-        s3 = op.d2y(a.bz(0, 1, 0), BZ, a.bz(0, -1, 0));
+        if constexpr (DEV) s2 += a.d2zb(1);
+        s3 = op.d2y(a.bz(0, 1, 0), BZs, a.bz(0, -1, 0));
         B2 += s2 * ORM;
         B3 += s3 * ORM;
         // :691-694
         s3 = op.d1z(a.bz(0, 0, 1), a.bz(0, 0, -1));
+        if constexpr (DEV) s3 += a.dzb(2);
         B3 -= W0 * s3;
-        s3 = op.d2z(a.bz(0, 0, 1), BZ, a.bz(0, 0, -1));
+        s3 = op.d2z(a.bz(0, 0, 1), BZs, a.bz(0, 0, -1));
+        if constexpr (DEV) s3 += a.d2zb(2);
         B3 += s3 * ORM;
     }
-    o.FU = double(FU);
-    o.FV = double(FV);
-    o.FW = double(FW);
-    o.FR = double(FR);
-    o.FT = double(FT);
-    o.B1 = double(B1);
-    o.B2 = double(B2);
-    o.B3 = double(B3);
+    o.FU = FU;
+    o.FV = FV;
+    o.FW = FW;
+    o.FR = FR;
+    o.FT = FT;
+    o.B1 = B1;
+    o.B2 = B2;
+    o.B3 = B3;
 }
 
 }  // namespace mhd

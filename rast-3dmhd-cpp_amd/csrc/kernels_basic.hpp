@@ -8,38 +8,107 @@
 #pragma once
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "flux_point.hpp"
 #include "mhd3d_types.hpp"
 
 namespace mhd {
 
 // State accessor that reads the register straight from global memory and
-// forms 1/RO, u, v, w and P on the fly (3dmhdsub.h:267-276).
+// forms 1/RO, u, v, w and P on the fly (3dmhdsub.h:267-276).  TI is the type
+// the register is stored in, T the type the fluxes are evaluated in.
+template <typename T, typename TI = T>
 struct GAcc {
-    const double* __restrict__ U;
+    const TI* __restrict__ U;
     int64_t fs, plane;
     int pitch, nPx, nPy, x, y, z;
-    __device__ __forceinline__ double ld(int f, int dx, int dy, int dz) const
+    __device__ __forceinline__ T ld(int f, int dx, int dy, int dz) const
     {
         const int xx = wrapi(x + dx, nPx), yy = wrapi(y + dy, nPy);
-        return U[f * fs + (int64_t)(z + dz) * plane + (int64_t)yy * pitch + xx];
+        return T(U[f * fs + (int64_t)(z + dz) * plane + (int64_t)yy * pitch + xx]);
     }
-    __device__ __forceinline__ double ru(int a, int b, int c) const { return ld(0, a, b, c); }
-    __device__ __forceinline__ double rv(int a, int b, int c) const { return ld(1, a, b, c); }
-    __device__ __forceinline__ double rw(int a, int b, int c) const { return ld(2, a, b, c); }
-    __device__ __forceinline__ double ro(int a, int b, int c) const { return ld(3, a, b, c); }
-    __device__ __forceinline__ double tt(int a, int b, int c) const { return ld(4, a, b, c); }
-    __device__ __forceinline__ double bx(int a, int b, int c) const { return ld(5, a, b, c); }
-    __device__ __forceinline__ double by(int a, int b, int c) const { return ld(6, a, b, c); }
-    __device__ __forceinline__ double bz(int a, int b, int c) const { return ld(7, a, b, c); }
-    __device__ __forceinline__ double ri(int a, int b, int c) const { return 1.0 / ro(a, b, c); }
-    __device__ __forceinline__ double u(int a, int b, int c) const { return ru(a, b, c) * ri(a, b, c); }
-    __device__ __forceinline__ double v(int a, int b, int c) const { return rv(a, b, c) * ri(a, b, c); }
-    __device__ __forceinline__ double w(int a, int b, int c) const { return rw(a, b, c) * ri(a, b, c); }
-    __device__ __forceinline__ double p(int a, int b, int c) const { return ro(a, b, c) * tt(a, b, c); }
+    __device__ __forceinline__ T ru(int a, int b, int c) const { return ld(0, a, b, c); }
+    __device__ __forceinline__ T rv(int a, int b, int c) const { return ld(1, a, b, c); }
+    __device__ __forceinline__ T rw(int a, int b, int c) const { return ld(2, a, b, c); }
+    __device__ __forceinline__ T ro(int a, int b, int c) const { return ld(3, a, b, c); }
+    __device__ __forceinline__ T tt(int a, int b, int c) const { return ld(4, a, b, c); }
+    __device__ __forceinline__ T bx(int a, int b, int c) const { return ld(5, a, b, c); }
+    __device__ __forceinline__ T by(int a, int b, int c) const { return ld(6, a, b, c); }
+    __device__ __forceinline__ T bz(int a, int b, int c) const { return ld(7, a, b, c); }
+    __device__ __forceinline__ T ri(int a, int b, int c) const { return T(1) / ro(a, b, c); }
+    __device__ __forceinline__ T u(int a, int b, int c) const { return ru(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ T v(int a, int b, int c) const { return rv(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ T w(int a, int b, int c) const { return rw(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ T p(int a, int b, int c) const { return ro(a, b, c) * tt(a, b, c); }
 };
 
 struct CflPart;
+
+// Background profiles of the mixed form (index k * nz + z).
+// FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
+// their D1z and their D2z (one-sided on the wall planes for BX0 and BY0).
+enum { BG_R0 = 0, BG_T0 = 1, BG_DZT0 = 2, BG_D2ZT0 = 3, BG_HB = 4, BG_B0 = 5, BG_DZB = 8,
+       BG_D2ZB = 11, NBGF = 14 };
+// FP64: RO0, TT0, BX0 BY0 BZ0.
+enum { BG64_R0 = 0, BG64_T0 = 1, BG64_B0 = 2, NBG64 = 5 };
+
+// Index of the FP64 background of field f (RO, TT, BX, BY, BZ), or -1.
+__host__ __device__ __forceinline__ int bg64_slot(int f)
+{
+    return f == 3 ? BG64_R0 : (f == 4 ? BG64_T0 : (f >= 5 ? BG64_B0 + (f - 5) : -1));
+}
+
+// Global-memory accessor of the mixed form's wall kernel: reads the FP64
+// state (TI = double, stage 1; deviations are formed in FP64, then rounded)
+// or the FP32 deviation copy (TI = float), and gives the deviation-form
+// values flux_point<..., DEV = true> asks for.
+template <typename TI>
+struct GAccDev {
+    const TI* __restrict__ U;
+    int64_t fs, plane;
+    int pitch, nPx, nPy, x, y, z, nz;
+    const float* __restrict__ bgf;
+    const double* __restrict__ bg64;
+    __device__ __forceinline__ float raw(int f, int dx, int dy, int dz) const
+    {
+        const int xx = wrapi(x + dx, nPx), yy = wrapi(y + dy, nPy);
+        const TI v = U[f * fs + (int64_t)(z + dz) * plane + (int64_t)yy * pitch + xx];
+        if constexpr (std::is_same<TI, double>::value) {
+            const int k = bg64_slot(f);
+            return k < 0 ? float(v) : float(v - bg64[k * nz + z + dz]);
+        } else {
+            return v;
+        }
+    }
+    __device__ __forceinline__ float ru(int a, int b, int c) const { return raw(0, a, b, c); }
+    __device__ __forceinline__ float rv(int a, int b, int c) const { return raw(1, a, b, c); }
+    __device__ __forceinline__ float rw(int a, int b, int c) const { return raw(2, a, b, c); }
+    __device__ __forceinline__ float ro(int a, int b, int c) const { return bgf[BG_R0 * nz + z + c] + raw(3, a, b, c); }
+    __device__ __forceinline__ float tt(int a, int b, int c) const { return raw(4, a, b, c); }
+    __device__ __forceinline__ float bx(int a, int b, int c) const { return raw(5, a, b, c); }
+    __device__ __forceinline__ float by(int a, int b, int c) const { return raw(6, a, b, c); }
+    __device__ __forceinline__ float bz(int a, int b, int c) const { return raw(7, a, b, c); }
+    __device__ __forceinline__ float ri(int a, int b, int c) const { return 1.0f / ro(a, b, c); }
+    __device__ __forceinline__ float u(int a, int b, int c) const { return ru(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ float v(int a, int b, int c) const { return rv(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ float w(int a, int b, int c) const { return rw(a, b, c) * ri(a, b, c); }
+    __device__ __forceinline__ float p(int a, int b, int c) const
+    {
+        const float R0 = bgf[BG_R0 * nz + z + c], T0 = bgf[BG_T0 * nz + z + c];
+        const float r = raw(3, a, b, c), t = raw(4, a, b, c);
+        return (R0 * t + r * T0) + r * t;
+    }
+    __device__ __forceinline__ float hb() const { return bgf[BG_HB * nz + z]; }
+    __device__ __forceinline__ float dzt0() const { return bgf[BG_DZT0 * nz + z]; }
+    __device__ __forceinline__ float d2zt0() const { return bgf[BG_D2ZT0 * nz + z]; }
+    __device__ __forceinline__ float rodev() const { return raw(3, 0, 0, 0); }
+    __device__ __forceinline__ float ttfull() const { return bgf[BG_T0 * nz + z] + raw(4, 0, 0, 0); }
+    __device__ __forceinline__ float bfull(int k) const { return bgf[(BG_B0 + k) * nz + z] + raw(5 + k, 0, 0, 0); }
+    __device__ __forceinline__ float dzb(int k) const { return bgf[(BG_DZB + k) * nz + z]; }
+    __device__ __forceinline__ float d2zb(int k) const { return bgf[(BG_D2ZB + k) * nz + z]; }
+};
+
 
 struct StageArgs {
     Geom g;
@@ -49,12 +118,30 @@ struct StageArgs {
     double* __restrict__ Uout;
     double* __restrict__ Z;
     const double* __restrict__ dt;
+    // mixed form only: the FP32 copies of the stage values that stages 2 and
+    // 3 take their stencils from (Uin32) and that stages 1 and 2 write
+    // (Uout32); Uin (read by stage 1) and Uout (written by stage 3) are then
+    // the FP64 state, and stages 1-2 write no FP64 U
+    const float* __restrict__ Uin32;
+    float* __restrict__ Uout32;
+    // mixed form only: the z-only background, bgf[k * nz + z] (BG_*) and
+    // bg64[k * nz + z] (BG64_*); the FP32 copies hold RO - RO0(z), TT - TT0(z)
+    // and B - B0(z) in place of RO, TT and B
+    const float* __restrict__ bgf;
+    const double* __restrict__ bg64;
     // stage 3 only: when non-null, the kernel also evaluates the CFL limits
     // of the new state (the TSTEP of the next step, 3dmhdsub.h:36-106) and
     // writes one partial record per block at cfl_parts[cfl_base + block]
     CflPart* cfl_parts;
     int cfl_base;
 };
+
+// Background subtracted from field f of plane z when the FP32 copy is written.
+__device__ __forceinline__ double bg_sub(const StageArgs& a, int f, int z)
+{
+    const int k = bg64_slot(f);
+    return k < 0 ? 0.0 : a.bg64[k * a.g.nz + z];
+}
 
 __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int yg, int zg)
 {
@@ -70,7 +157,8 @@ __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int
     return pm;
 }
 
-__device__ __forceinline__ double fsel(const FluxOut& o, int f)
+template <typename T>
+__device__ __forceinline__ double fsel(const FluxOutT<T>& o, int f)
 {
     switch (f) {
     case 0: return o.FU;
@@ -190,7 +278,7 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z + 2;   // planes 2 .. nPz-1
     if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz - 1) return;
-    GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+    GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
     FluxOut o;
     flux_point<MAG, ROT, 0, double>(acc, a.c, pm, o);
@@ -212,9 +300,15 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
 // ---------------------------------------------------------------------------
 // Wall planes of one RK stage: fluxes with the one-sided wall stencils, the
 // GAM update, BCON on the updated wall values, then the ZETA register.
-template <bool MAG, bool ROT, int STAGE>
+// T = double: the FP64 form.  T = float: the mixed form (fluxes in FP32, in
+// deviation form, from the FP64 state at stage 1 and from the FP32 stage copy
+// at stages 2-3; the update, BCON and the Z register in FP64; BCON reads the
+// new inward planes from the FP32 copy at stages 1-2 and from the FP64 state
+// at stage 3).
+template <typename T, bool MAG, bool ROT, int STAGE>
 __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
 {
+    constexpr bool F64 = std::is_same<T, double>::value;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const bool top = blockIdx.z == 0;
@@ -227,13 +321,29 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
         const int z = top ? 1 : a.g.nPz;
         const int zi1 = top ? z + 1 : z - 1;   // first plane inward
         const int zi2 = top ? z + 2 : z - 2;   // second plane inward
-        GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
         const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
-        FluxOut o;
-        if (top)
-            flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
-        else
-            flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+        FluxOutT<T> o;
+        if constexpr (F64) {
+            GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+            if (top)
+                flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
+            else
+                flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+        } else if constexpr (STAGE == 1) {
+            GAccDev<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
+                                a.g.nz, a.bgf, a.bg64};
+            if (top)
+                flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
+            else
+                flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
+        } else {
+            GAccDev<float> acc{a.Uin32, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
+                               a.g.nz, a.bgf, a.bg64};
+            if (top)
+                flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
+            else
+                flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
+        }
         const double DT = *a.dt;
         const double cg = a.c.GAM[STAGE - 1] * DT;
         const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
@@ -248,12 +358,16 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
             const double zin = STAGE == 1 ? a.Uin[f * fs + i] : a.Z[f * fs + i];
             un[f] = zin + cg * fsel(o, f);
         }
-        // BCON (3dmhdsub.h:727-872); the updated planes inward are in Uout
+        // BCON (3dmhdsub.h:727-872) on the updated planes inward
+        const bool new64 = F64 || STAGE == 3;
+        auto Un = [&](int f, int64_t j, int zj) -> double {
+            if (new64) return a.Uout[f * fs + j];
+            return double(a.Uout32[f * fs + j]) + bg_sub(a, f, zj);
+        };
         const Phys& c = a.c;
-        const double* Un = a.Uout;
-        const double ro1 = Un[3 * fs + i1], ro2 = Un[3 * fs + i2];
-        un[0] = un[3] * (c.C43 * Un[0 * fs + i1] / ro1 - c.C13 * Un[0 * fs + i2] / ro2);
-        un[1] = un[3] * (c.C43 * Un[1 * fs + i1] / ro1 - c.C13 * Un[1 * fs + i2] / ro2);
+        const double ro1 = Un(3, i1, zi1), ro2 = Un(3, i2, zi2);
+        un[0] = un[3] * (c.C43 * Un(0, i1, zi1) / ro1 - c.C13 * Un(0, i2, zi2) / ro2);
+        un[1] = un[3] * (c.C43 * Un(1, i1, zi1) / ro1 - c.C13 * Un(1, i2, zi2) / ro2);
         un[2] = 0.0;
         if (top) {
             if (c.iTCon == 0 || c.iTCon == 4) {
@@ -261,21 +375,24 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
                 if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
             } else if (c.iTCon == 1) {
                 const double DZ = c.ORZF / a.m.dzzdz[z];
-                un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] - c.C23 * DZ * c.THETA;
+                un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) - c.C23 * DZ * c.THETA;
             }
         } else {
             if (c.izCon == 0) {
                 un[4] = c.TB;
             } else {
                 const double DZ = c.ORZF / a.m.dzzdz[z];
-                un[4] = c.C43 * Un[4 * fs + i1] - c.C13 * Un[4 * fs + i2] -
+                un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) -
                         c.C23 * DZ * c.THETA / a.m.RKAPA[z];
             }
         }
         if (MAG && c.iBCon == 0) un[7] = 0.0;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            a.Uout[f * fs + i] = un[f];
+            if (new64)
+                a.Uout[f * fs + i] = un[f];
+            else
+                a.Uout32[f * fs + i] = float(un[f] - bg_sub(a, f, z));
             if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
         }
         if (STAGE == 3 && a.cfl_parts) {
@@ -300,7 +417,7 @@ __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __r
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z + 1;   // planes 1 .. nPz
     if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz) return;
-    GAcc acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+    GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
     FluxOut o;
     if (z == 1)
@@ -462,6 +579,25 @@ __global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,
 }
 
 __global__ void clear_flag_kernel(DevScalars* s) { s->nonfinite = 0u; }
+
+// FP32 copy of a register in the mixed form's deviation layout:
+// dst = (float)(src - background) for RO, TT, BX, BY, BZ, (float)src otherwise.
+__global__ void __launch_bounds__(256) to_f32_dev_kernel(const double* __restrict__ src,
+                                                         float* __restrict__ dst, Geom g, int nf,
+                                                         const double* __restrict__ bg64)
+{
+    const int64_t n = g.fs * nf;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int f = (int)(i / g.fs);
+        const int64_t j = i - (int64_t)f * g.fs;
+        const int z = (int)(j / g.plane);
+        double v = src[i];
+        const int k = bg64_slot(f);
+        if (z < g.nz && k >= 0) v -= bg64[k * g.nz + z];
+        dst[i] = float(v);
+    }
+}
 
 }  // namespace mhd
 

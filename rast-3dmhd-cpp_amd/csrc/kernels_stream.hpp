@@ -20,6 +20,8 @@
 #pragma once
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "kernels_basic.hpp"
 
 namespace mhd {
@@ -39,15 +41,15 @@ __device__ __forceinline__ void lds_barrier()
 constexpr int SHW = STX + 2, SHH = STY + 2, SHP = SHW * SHH;   // 34 x 10 = 340 points
 constexpr int SNH = 2 * SHW + 2 * STY;                          // 84 halo points
 
-// Raw values of one point of one plane.
-template <bool MAG>
+// Values of one point of one plane as stored in the stencil input (type TI).
+template <typename TI, bool MAG>
 struct RawPt {
-    double ru = 0.0, rv = 0.0, rw = 0.0, ro = 1.0, tt = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+    TI ru = TI(0), rv = TI(0), rw = TI(0), ro = TI(1), tt = TI(0), bx = TI(0), by = TI(0), bz = TI(0);
 };
 
-template <bool MAG>
-__device__ __forceinline__ void load_raw(RawPt<MAG>& r, const double* __restrict__ U,
-                                         int64_t fs, int64_t off)
+template <typename TI, bool MAG>
+__device__ __forceinline__ void load_raw(RawPt<TI, MAG>& r, const TI* __restrict__ U, int64_t fs,
+                                         int64_t off)
 {
     r.ru = U[off];
     r.rv = U[fs + off];
@@ -61,51 +63,125 @@ __device__ __forceinline__ void load_raw(RawPt<MAG>& r, const double* __restrict
     }
 }
 
-// LDS-backed accessor for an interior plane.  Star fields are
-// [RU, RV, RW, TT, P, BX, BY, BZ]; ws[f][k] is the own-column window
-// (k = 0, 1, 2 for z-1, z, z+1).
-template <bool MAG>
+// Stencil values of one point in the type T: RU RV RW, the full density, the
+// TT and P stencil values (deviations from the background in the mixed
+// form), the density deviation and BX BY BZ.
+template <typename T>
+struct PrepPt {
+    T ru, rv, rw, ro, tt, p, rdev, bx, by, bz;
+};
+
+// FP64 form: the stored values.  Mixed form: the stored FP32 deviations of
+// RO, TT, BX, BY, BZ (or, when the input is the FP64 state, its deviations
+// from the FP64 background rounded to FP32), with the full density RO0 + r
+// and P - P0 = (RO0*t + r*TT0) + r*t formed from the FP32 background of
+// plane z.
+template <typename T, bool MAG, typename TI>
+__device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const StageArgs& a, int z)
+{
+    PrepPt<T> q;
+    q.ru = T(raw.ru);
+    q.rv = T(raw.rv);
+    q.rw = T(raw.rw);
+    if constexpr (!std::is_same<T, double>::value && std::is_same<TI, double>::value && MAG) {
+        const int nz = a.g.nz;
+        q.bx = T(raw.bx - a.bg64[(BG64_B0 + 0) * nz + z]);
+        q.by = T(raw.by - a.bg64[(BG64_B0 + 1) * nz + z]);
+        q.bz = T(raw.bz - a.bg64[(BG64_B0 + 2) * nz + z]);
+    } else {
+        q.bx = T(raw.bx);
+        q.by = T(raw.by);
+        q.bz = T(raw.bz);
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        q.ro = raw.ro;
+        q.tt = raw.tt;
+        q.p = raw.ro * raw.tt;
+        q.rdev = T(0);
+    } else {
+        const int nz = a.g.nz;
+        const T R0 = a.bgf[BG_R0 * nz + z], T0 = a.bgf[BG_T0 * nz + z];
+        T r, t;
+        if constexpr (std::is_same<TI, double>::value) {
+            r = T(raw.ro - a.bg64[BG64_R0 * nz + z]);
+            t = T(raw.tt - a.bg64[BG64_T0 * nz + z]);
+        } else {
+            r = raw.ro;
+            t = raw.tt;
+        }
+        q.ro = R0 + r;
+        q.tt = t;
+        q.p = (R0 * t + r * T0) + r * t;
+        q.rdev = r;
+    }
+    return q;
+}
+
+// LDS-backed accessor for an interior plane.  Centre-plane fields are
+// [RU, RV, RW, TT, P]; ws[f][k] is the own-column window (k = 0, 1, 2 for
+// z-1, z, z+1).
+template <typename T, bool MAG>
 struct SAcc {
     static constexpr int NW = 5;               // RU RV RW TT P
-    const double* st;                          // star plane z, at this thread's centre
-    const double* um;                          // u/v/w [BX BY BZ] ring, plane z-1, at centre
-    const double* u0;                          // plane z
-    const double* up;                          // plane z+1
-    double ws[NW][3];
-    double ro0, ri0;
+    const T* st;                               // centre plane z, at this thread's centre
+    const T* um;                               // u/v/w [BX BY BZ] ring, plane z-1, at centre
+    const T* u0;                               // plane z
+    const T* up;                               // plane z+1
+    T ws[NW][3];
+    T ro0, ri0;
 
-    __device__ __forceinline__ double star(int f, int dx, int dy, int dz) const
+    __device__ __forceinline__ T star(int f, int dx, int dy, int dz) const
     {
         if (dx == 0 && dy == 0) return ws[f][dz + 1];
         return st[f * SHP + dy * SHW + dx];
     }
-    // u, v, w (and BX, BY, BZ) come from the three-plane LDS ring
-    __device__ __forceinline__ double ring(int f, int dx, int dy, int dz) const
+    // u, v, w (and BX, BY, BZ) come from the ring
+    __device__ __forceinline__ T ring(int f, int dx, int dy, int dz) const
     {
-        const double* b = dz < 0 ? um : (dz > 0 ? up : u0);
+        const T* b = dz < 0 ? um : (dz > 0 ? up : u0);
         return b[f * SHP + dy * SHW + dx];
     }
-    __device__ __forceinline__ double ru(int a, int b, int c) const { return star(0, a, b, c); }
-    __device__ __forceinline__ double rv(int a, int b, int c) const { return star(1, a, b, c); }
-    __device__ __forceinline__ double rw(int a, int b, int c) const { return star(2, a, b, c); }
-    __device__ __forceinline__ double tt(int a, int b, int c) const { return star(3, a, b, c); }
-    __device__ __forceinline__ double p(int a, int b, int c) const { return star(4, a, b, c); }
-    __device__ __forceinline__ double u(int a, int b, int c) const { return ring(0, a, b, c); }
-    __device__ __forceinline__ double v(int a, int b, int c) const { return ring(1, a, b, c); }
-    __device__ __forceinline__ double w(int a, int b, int c) const { return ring(2, a, b, c); }
-    __device__ __forceinline__ double bx(int a, int b, int c) const { return ring(3, a, b, c); }
-    __device__ __forceinline__ double by(int a, int b, int c) const { return ring(4, a, b, c); }
-    __device__ __forceinline__ double bz(int a, int b, int c) const { return ring(5, a, b, c); }
-    __device__ __forceinline__ double ro(int, int, int) const { return ro0; }
-    __device__ __forceinline__ double ri(int, int, int) const { return ri0; }
+    __device__ __forceinline__ T ru(int a, int b, int c) const { return star(0, a, b, c); }
+    __device__ __forceinline__ T rv(int a, int b, int c) const { return star(1, a, b, c); }
+    __device__ __forceinline__ T rw(int a, int b, int c) const { return star(2, a, b, c); }
+    __device__ __forceinline__ T tt(int a, int b, int c) const { return star(3, a, b, c); }
+    __device__ __forceinline__ T p(int a, int b, int c) const { return star(4, a, b, c); }
+    __device__ __forceinline__ T u(int a, int b, int c) const { return ring(0, a, b, c); }
+    __device__ __forceinline__ T v(int a, int b, int c) const { return ring(1, a, b, c); }
+    __device__ __forceinline__ T w(int a, int b, int c) const { return ring(2, a, b, c); }
+    __device__ __forceinline__ T bx(int a, int b, int c) const { return ring(3, a, b, c); }
+    __device__ __forceinline__ T by(int a, int b, int c) const { return ring(4, a, b, c); }
+    __device__ __forceinline__ T bz(int a, int b, int c) const { return ring(5, a, b, c); }
+    __device__ __forceinline__ T ro(int, int, int) const { return ro0; }
+    __device__ __forceinline__ T ri(int, int, int) const { return ri0; }
+    // deviation form (mixed): background terms of the centre plane
+    T hb_ = T(0), dzt0_ = T(0), d2zt0_ = T(0), t0_ = T(0), rdev_ = T(0);
+    T b0_[3] = {T(0), T(0), T(0)}, dzb_[3] = {T(0), T(0), T(0)}, d2zb_[3] = {T(0), T(0), T(0)};
+    __device__ __forceinline__ T bfull(int k) const { return b0_[k] + u0[(3 + k) * SHP]; }
+    __device__ __forceinline__ T dzb(int k) const { return dzb_[k]; }
+    __device__ __forceinline__ T d2zb(int k) const { return d2zb_[k]; }
+    __device__ __forceinline__ T hb() const { return hb_; }
+    __device__ __forceinline__ T dzt0() const { return dzt0_; }
+    __device__ __forceinline__ T d2zt0() const { return d2zt0_; }
+    __device__ __forceinline__ T rodev() const { return rdev_; }
+    __device__ __forceinline__ T ttfull() const { return t0_ + ws[3][1]; }
 };
 
+// T = double: the plain FP64 form, reading U (FP64) and bit-identical to the
+//   reference's arithmetic.
+// T = float: the mixed form, evaluating the fluxes in FP32 while the RK
+//   registers stay FP64.  Stage 1 takes its stencils (rounded to FP32) and
+//   Z = U from the FP64 state; stages 1-2 write the FP32 copy of their result
+//   (the next stage's stencil input) and the FP64 Z; stage 3 reads the FP32
+//   copy and Z and writes the FP64 state (and the FP64 CFL of the next step).
 // WAVES: minimum waves per SIMD the register allocation must allow (2 keeps
-// 256 VGPRs per wave; 1 lets the MHD variant use AGPRs instead of spilling).
-template <bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER>
+// 256 VGPRs per wave; 1 lets the FP64 MHD variant use AGPRs instead of
+// spilling).
+template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER>
 __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int ntx,
                                                                   int nty, int xcd_remap)
 {
+    constexpr bool F64 = std::is_same<T, double>::value;
     constexpr int NR = MAG ? 6 : 3;   // ring fields: u v w [BX BY BZ]
     constexpr int NS = 5;             // centre-plane fields: RU RV RW TT P
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
@@ -115,8 +191,13 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     // still reads)
     constexpr int NRING = ONE_BARRIER ? 4 : 3;
     constexpr int NSTAR = ONE_BARRIER ? 3 : 2;
-    __shared__ double s_uvw[NRING][NR][SHP];
-    __shared__ double s_st[NSTAR][NS][SHP];
+    // Z-type operand of the update read from HBM: Z for stages 2-3, and in
+    // the mixed form also the FP64 state for stage 1
+    constexpr bool ZLOAD = STAGE > 1 || !F64;
+    // storage type of the stencil input: the FP64 state, or the FP32 copy
+    using TI = typename std::conditional<F64 || STAGE == 1, double, float>::type;
+    __shared__ T s_uvw[NRING][NR][SHP];
+    __shared__ T s_st[NSTAR][NS][SHP];
 
     const Geom& g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -164,88 +245,139 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
         ph = ly * SHW + lx;
     }
 
-    const double* __restrict__ U = a.Uin;
+    const TI* __restrict__ U;
+    if constexpr (std::is_same<TI, double>::value)
+        U = a.Uin;
+    else
+        U = a.Uin32;
+    const double* __restrict__ zsrc = (STAGE == 1) ? a.Uin : a.Z;
     const int64_t fs = g.fs;
-    SAcc<MAG> acc;
-    double row[3] = {0.0, 0.0, 0.0};
-    double rinv[3] = {0.0, 0.0, 0.0};
-    RawPt<MAG> own, halo, own_n, halo_n;
-    load_raw<MAG>(own, U, fs, (int64_t)(zb - 1) * g.plane + go);
-    if (has_halo) load_raw<MAG>(halo, U, fs, (int64_t)(zb - 1) * g.plane + gh);
+    SAcc<T, MAG> acc;
+    T row[3] = {T(0), T(0), T(0)};
+    T rinv[3] = {T(0), T(0), T(0)};
+    T rdv[3] = {T(0), T(0), T(0)};
+    RawPt<TI, MAG> own, halo, own_n, halo_n;
+    load_raw<TI, MAG>(own, U, fs, (int64_t)(zb - 1) * g.plane + go);
+    if (has_halo) load_raw<TI, MAG>(halo, U, fs, (int64_t)(zb - 1) * g.plane + gh);
 
     const double DT = *a.dt;
     const double cg = a.c.GAM[STAGE - 1] * DT;
     const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
-    // Z of the next plane to update, loaded one iteration ahead (stages 2, 3)
+    // Z-type operand of the next plane to update, loaded one iteration ahead
     double zpre[NF], zcur[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) zpre[f] = zcur[f] = 0.0;
-    if (STAGE > 1) {
+    if (ZLOAD && !DEFER) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) zpre[f] = a.Z[f * fs + (int64_t)zb * g.plane + go];
+        for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)zb * g.plane + go];
     }
 
     CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
+    // RK update of plane zq from its tendencies o and its Z-type operand zin
+    // (Uout = zin + GAM*DT*F, Z = Uout + ZETA*DT*F; stage 3 also the CFL terms)
+    auto update = [&](int zq, const FluxOutT<T>& o, const double* zin) {
+        if (!valid) return;
+        const int64_t i = (int64_t)zq * g.plane + go;
+        double un[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const double F = fsel(o, f);
+            un[f] = zin[f] + cg * F;
+            if (F64 || STAGE == 3)
+                a.Uout[f * fs + i] = un[f];
+            else
+                a.Uout32[f * fs + i] = float(un[f] - bg_sub(a, f, zq));
+            if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * F;
+        }
+        if (STAGE == 3 && a.cfl_parts) {
+            double wp[4], up;
+            if constexpr (MAG)
+                cfl_values(a.c, a.m, true, un[0], un[1], un[2], un[3], un[4], un[5], un[6], un[7],
+                           gx + 1, gy + 1, zq, wp, up);
+            else
+                cfl_values(a.c, a.m, false, un[0], un[1], un[2], un[3], un[4], 0.0, 0.0, 0.0,
+                           gx + 1, gy + 1, zq, wp, up);
+            cfl.add(wp, up);
+        }
+    };
+    // DEFER: the update of plane z runs one iteration later, after the next
+    // plane is published, with its Z operand loaded right after the flux
+    // evaluation -- the operand's load latency then overlaps the next
+    // plane's LDS work, and nothing but the 8 tendencies and the operand is
+    // carried through it.  Otherwise the operand of plane z+1 is prefetched
+    // one iteration ahead and plane z is updated at once.
+    FluxOutT<T> fpend;
+    double zpend[NF];
+    int zp = -1;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) zpend[f] = 0.0;
     for (int L = zb - 1; L <= ze; ++L) {
         // -- derive plane L and publish it
         {
-            const double ri = 1.0 / own.ro;
-            const double u = own.ru * ri, v = own.rv * ri, w = own.rw * ri, P = own.ro * own.tt;
-            double* su = &s_uvw[L % NRING][0][0];
-            double* ss = &s_st[L % NSTAR][0][0];
+            const PrepPt<T> q = prep_point<T, MAG, TI>(own, a, L);
+            const T ri = T(1) / q.ro;
+            const T u = q.ru * ri, v = q.rv * ri, w = q.rw * ri;
+            T* su = &s_uvw[L % NRING][0][0];
+            T* ss = &s_st[L % NSTAR][0][0];
             su[0 * SHP + pc] = u;
             su[1 * SHP + pc] = v;
             su[2 * SHP + pc] = w;
-            ss[0 * SHP + pc] = own.ru;
-            ss[1 * SHP + pc] = own.rv;
-            ss[2 * SHP + pc] = own.rw;
-            ss[3 * SHP + pc] = own.tt;
-            ss[4 * SHP + pc] = P;
+            ss[0 * SHP + pc] = q.ru;
+            ss[1 * SHP + pc] = q.rv;
+            ss[2 * SHP + pc] = q.rw;
+            ss[3 * SHP + pc] = q.tt;
+            ss[4 * SHP + pc] = q.p;
             if (MAG) {
-                su[3 * SHP + pc] = own.bx;
-                su[4 * SHP + pc] = own.by;
-                su[5 * SHP + pc] = own.bz;
+                su[3 * SHP + pc] = q.bx;
+                su[4 * SHP + pc] = q.by;
+                su[5 * SHP + pc] = q.bz;
             }
-            const double nw[5] = {own.ru, own.rv, own.rw, own.tt, P};
+            const T nw[5] = {q.ru, q.rv, q.rw, q.tt, q.p};
 #pragma unroll
-            for (int k = 0; k < SAcc<MAG>::NW; ++k) {
+            for (int k = 0; k < SAcc<T, MAG>::NW; ++k) {
                 acc.ws[k][0] = acc.ws[k][1];
                 acc.ws[k][1] = acc.ws[k][2];
                 acc.ws[k][2] = nw[k];
             }
-            row[0] = row[1]; row[1] = row[2]; row[2] = own.ro;
+            row[0] = row[1]; row[1] = row[2]; row[2] = q.ro;
             rinv[0] = rinv[1]; rinv[1] = rinv[2]; rinv[2] = ri;
+            if (!F64) { rdv[0] = rdv[1]; rdv[1] = rdv[2]; rdv[2] = q.rdev; }
             if (has_halo) {
-                const double hri = 1.0 / halo.ro;
-                su[0 * SHP + ph] = halo.ru * hri;
-                su[1 * SHP + ph] = halo.rv * hri;
-                su[2 * SHP + ph] = halo.rw * hri;
-                ss[0 * SHP + ph] = halo.ru;
-                ss[1 * SHP + ph] = halo.rv;
-                ss[2 * SHP + ph] = halo.rw;
-                ss[3 * SHP + ph] = halo.tt;
-                ss[4 * SHP + ph] = halo.ro * halo.tt;
+                const PrepPt<T> h = prep_point<T, MAG, TI>(halo, a, L);
+                const T hri = T(1) / h.ro;
+                su[0 * SHP + ph] = h.ru * hri;
+                su[1 * SHP + ph] = h.rv * hri;
+                su[2 * SHP + ph] = h.rw * hri;
+                ss[0 * SHP + ph] = h.ru;
+                ss[1 * SHP + ph] = h.rv;
+                ss[2 * SHP + ph] = h.rw;
+                ss[3 * SHP + ph] = h.tt;
+                ss[4 * SHP + ph] = h.p;
                 if (MAG) {
-                    su[3 * SHP + ph] = halo.bx;
-                    su[4 * SHP + ph] = halo.by;
-                    su[5 * SHP + ph] = halo.bz;
+                    su[3 * SHP + ph] = h.bx;
+                    su[4 * SHP + ph] = h.by;
+                    su[5 * SHP + ph] = h.bz;
                 }
             }
         }
         lds_barrier();
         // -- prefetch plane L+1 (before the flux evaluation, so the loads overlap it)
         if (L < ze) {
-            load_raw<MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
-            if (has_halo) load_raw<MAG>(halo_n, U, fs, (int64_t)(L + 1) * g.plane + gh);
+            load_raw<TI, MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
+            if (has_halo) load_raw<TI, MAG>(halo_n, U, fs, (int64_t)(L + 1) * g.plane + gh);
+        }
+        if (DEFER && zp >= 0) {
+            update(zp, fpend, zpend);
+            zp = -1;
         }
         const int z = L - 1;
-        if (STAGE > 1 && z >= zb) {
-            // this iteration updates plane z; fetch Z of plane z+1 for the next one
+        if (!DEFER && ZLOAD && z >= zb) {
+            // this iteration updates plane z; fetch the operand of plane z+1
 #pragma unroll
             for (int f = 0; f < NF; ++f) zcur[f] = zpre[f];
             if (z + 1 < ze) {
 #pragma unroll
-                for (int f = 0; f < NF; ++f) zpre[f] = a.Z[f * fs + (int64_t)(z + 1) * g.plane + go];
+                for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)(z + 1) * g.plane + go];
             }
         }
         if (z >= zb) {
@@ -255,43 +387,54 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
             acc.up = &s_uvw[(z + 1) % NRING][0][pc];
             acc.ro0 = row[1];
             acc.ri0 = rinv[1];
+            if constexpr (!F64) {
+                const int nz = g.nz;
+                acc.rdev_ = rdv[1];
+                acc.hb_ = a.bgf[BG_HB * nz + z];
+                acc.dzt0_ = a.bgf[BG_DZT0 * nz + z];
+                acc.d2zt0_ = a.bgf[BG_D2ZT0 * nz + z];
+                acc.t0_ = a.bgf[BG_T0 * nz + z];
+                if (MAG) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        acc.b0_[k] = a.bgf[(BG_B0 + k) * nz + z];
+                        acc.dzb_[k] = a.bgf[(BG_DZB + k) * nz + z];
+                        acc.d2zb_[k] = a.bgf[(BG_D2ZB + k) * nz + z];
+                    }
+                }
+            }
             const PointMetric pm = point_metric(a.m, gx + 1, gy + 1, z);
-            FluxOut o;
-            flux_point<MAG, ROT, 0, double>(acc, a.c, pm, o);
-            if (valid) {
-                const int64_t i = (int64_t)z * g.plane + go;
-                double uc[8] = {acc.ws[0][1], acc.ws[1][1], acc.ws[2][1], row[1],
-                                acc.ws[3][1], 0.0, 0.0, 0.0};
+            FluxOutT<T> o;
+            flux_point<MAG, ROT, 0, T, !F64>(acc, a.c, pm, o);
+            // FP64 stage 1: Z = U, the centre values of plane z
+            double uc[NF];
+            if constexpr (!ZLOAD) {
+                uc[0] = acc.ws[0][1];
+                uc[1] = acc.ws[1][1];
+                uc[2] = acc.ws[2][1];
+                uc[3] = row[1];
+                uc[4] = acc.ws[3][1];
                 if constexpr (MAG) {
                     uc[5] = acc.u0[3 * SHP];
                     uc[6] = acc.u0[4 * SHP];
                     uc[7] = acc.u0[5 * SHP];
                 }
-                double un[NF];
+            }
+            if constexpr (DEFER) {
+                fpend = o;
+                zp = z;
 #pragma unroll
-                for (int f = 0; f < NF; ++f) {
-                    const double F = fsel(o, f);
-                    const double zin = STAGE == 1 ? uc[f] : zcur[f];
-                    un[f] = zin + cg * F;
-                    a.Uout[f * fs + i] = un[f];
-                    if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * F;
-                }
-                if (STAGE == 3 && a.cfl_parts) {
-                    double wp[4], up;
-                    if constexpr (MAG)
-                        cfl_values(a.c, a.m, true, un[0], un[1], un[2], un[3], un[4], un[5],
-                                   un[6], un[7], gx + 1, gy + 1, z, wp, up);
-                    else
-                        cfl_values(a.c, a.m, false, un[0], un[1], un[2], un[3], un[4], 0.0,
-                                   0.0, 0.0, gx + 1, gy + 1, z, wp, up);
-                    cfl.add(wp, up);
-                }
+                for (int f = 0; f < NF; ++f)
+                    zpend[f] = ZLOAD ? zsrc[f * fs + (int64_t)z * g.plane + go] : uc[f];
+            } else {
+                update(z, o, ZLOAD ? zcur : uc);
             }
         }
         if (!ONE_BARRIER) lds_barrier();
         own = own_n;
         if (has_halo) halo = halo_n;
     }
+    if (DEFER && zp >= 0) update(zp, fpend, zpend);
     if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
 }
 
@@ -317,23 +460,40 @@ inline StreamPlan stream_plan(const Geom& g, int target_blocks = 1024)
     return p;
 }
 
-template <bool MAG, bool ROT, int STAGE>
-static void launch_stream_stage(const StageArgs& a, hipStream_t s, int waves, bool one_barrier)
+// Launch of one interior stage.  sched bit 0: one barrier per plane
+// (ONE_BARRIER), bit 1: deferred update (DEFER); waves: register budget.
+template <typename T, bool MAG, bool ROT, int STAGE, int WAVES>
+static void launch_stream_w(const StageArgs& a, hipStream_t s, const StreamPlan& p, int sched)
+{
+    dim3 blk(STX, STY, 1);
+    switch (sched & 3) {
+    case 0:
+        stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, false, false><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+        break;
+    case 1:
+        stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, true, false><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+        break;
+    case 2:
+        stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, false, true><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+        break;
+    default:
+        stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, true, true><<<p.nblocks, blk, 0, s>>>(
+            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+        break;
+    }
+}
+
+template <typename T, bool MAG, bool ROT, int STAGE>
+static void launch_stream_stage(const StageArgs& a, hipStream_t s, int waves, int sched)
 {
     const StreamPlan p = stream_plan(a.g);
-    dim3 blk(STX, STY, 1);
-    if (waves == 1 && one_barrier)
-        stage_stream_kernel<MAG, ROT, STAGE, 1, true><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
-    else if (waves == 1)
-        stage_stream_kernel<MAG, ROT, STAGE, 1, false><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
-    else if (one_barrier)
-        stage_stream_kernel<MAG, ROT, STAGE, 2, true><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+    if (waves == 1)
+        launch_stream_w<T, MAG, ROT, STAGE, 1>(a, s, p, sched);
     else
-        stage_stream_kernel<MAG, ROT, STAGE, 2, false><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+        launch_stream_w<T, MAG, ROT, STAGE, 2>(a, s, p, sched);
 }
 
 }  // namespace mhd

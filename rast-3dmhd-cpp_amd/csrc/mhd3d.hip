@@ -1,10 +1,13 @@
 // C ABI of the MI355X RK3 timestep (include/mhd3d.h).
 //
-// One context = one domain on one GPU: three device registers (two U buffers
-// used as ping-pong, one Z), the metric arrays, a block of device scalars and
-// one HIP stream.  A step is: CFL (unless fixed DT) -> 3 x (interior stage
-// kernel, wall stage kernel) -> clock update, with DT kept on the device so
-// the host never waits inside a step.
+// One context = one domain on one GPU: the device registers, the metric
+// arrays, a block of device scalars and one HIP stream.  FP64 form: two U
+// buffers used as ping-pong and one Z.  Mixed form: one FP64 U (the state,
+// read by stage 1 and rewritten by stage 3), the FP64 Z and two FP32 copies
+// of the stage values (written by stages 1 and 2, read by stages 2 and 3).
+// A step is: CFL (unless fixed DT or already evaluated inside the previous
+// stage 3) -> 3 x (interior stage kernel, wall stage kernel) -> clock update,
+// with DT kept on the device so the host never waits inside a step.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -31,6 +34,11 @@ struct mhd3d_ctx {
     double* dmet = nullptr;
     double* U[2] = {nullptr, nullptr};
     double* Z = nullptr;
+    bool mixed = false;             // MHD3D_MIXED: FP32 stencils, FP64 registers
+    float* U32[2] = {nullptr, nullptr};
+    float* bgf = nullptr;           // mixed: z-only background (BG_*), FP32
+    double* bg64 = nullptr;         //        and FP64 (BG64_*)
+    std::vector<double> h_dzz, h_d2zz;   // host copies of dzzdz, d2zzdz2
     double* F = nullptr;
     DevScalars* ds = nullptr;
     CflPart* cfl_parts = nullptr;
@@ -41,10 +49,12 @@ struct mhd3d_ctx {
     int cur = 0;
     int device = 0;
     int kernel_mode = 0;            // 0 = z-streaming LDS kernel, 1 = point kernel
-    int stream_waves_mag = 1;       // register budget of the MHD streaming kernel
-    int stream_waves_hydro = 2;
-    bool one_barrier_mag = true;    // streaming kernel synchronisation scheme (MHD)
-    bool one_barrier_hydro = false; //   (hydro: two barriers, 2 workgroups per CU)
+    // streaming-kernel variant per [form][physics] (form 0 = FP64, 1 = mixed;
+    // physics 0 = hydro, 1 = MHD): register budget in waves per SIMD, and
+    // schedule bits (1: one barrier per plane, 2: deferred RK update).
+    // Defaults: the fastest variants measured on MI355X at 256x256x128.
+    int tune_waves[2][2] = {{2, 1}, {2, 2}};
+    int tune_sched[2][2] = {{2, 0}, {1, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
     int64_t bytes = 0;
     std::string err;
@@ -171,26 +181,53 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     ctx->m.dyydy = p; p += ny; ctx->m.d2yydy2 = p; p += ny; ctx->m.DDY = p; p += ny;
     ctx->m.dzzdz = p; p += nz; ctx->m.d2zzdz2 = p; p += nz; ctx->m.DDZ = p; p += nz;
     ctx->m.RKAPA = p; p += nz; ctx->m.DKAPA = p; p += nz;
+    ctx->mixed = desc->precision == MHD3D_MIXED;
     const size_t reg = (size_t)g.fs * ctx->nf * sizeof(double);
-    for (int k = 0; k < 2; ++k)
+    const size_t reg32 = (size_t)g.fs * ctx->nf * sizeof(float);
+    const int nU64 = ctx->mixed ? 1 : 2;
+    for (int k = 0; k < nU64; ++k) {
         if (hipMalloc(&ctx->U[k], reg) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(U) failed");
+        if (hipMemset(ctx->U[k], 0, reg) != hipSuccess) return cleanup(MHD3D_EHIP, "hipMemset failed");
+    }
+    if (ctx->mixed) {
+        for (int k = 0; k < 2; ++k) {
+            if (hipMalloc(&ctx->U32[k], reg32) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(U32) failed");
+            if (hipMemset(ctx->U32[k], 0, reg32) != hipSuccess) return cleanup(MHD3D_EHIP, "hipMemset failed");
+        }
+        if (hipMalloc(&ctx->bgf, sizeof(float) * NBGF * nz) != hipSuccess ||
+            hipMalloc(&ctx->bg64, sizeof(double) * NBG64 * nz) != hipSuccess)
+            return cleanup(MHD3D_ENOMEM, "hipMalloc(background) failed");
+        if (hipMemset(ctx->bgf, 0, sizeof(float) * NBGF * nz) != hipSuccess ||
+            hipMemset(ctx->bg64, 0, sizeof(double) * NBG64 * nz) != hipSuccess)
+            return cleanup(MHD3D_EHIP, "hipMemset failed");
+        ctx->h_dzz.assign(desc->dzzdz, desc->dzzdz + nz);
+        ctx->h_d2zz.assign(desc->d2zzdz2, desc->d2zzdz2 + nz);
+    }
     if (hipMalloc(&ctx->Z, reg) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(Z) failed");
     if (hipMalloc(&ctx->ds, sizeof(DevScalars)) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(scalars) failed");
-    if (hipMemset(ctx->U[0], 0, reg) != hipSuccess || hipMemset(ctx->U[1], 0, reg) != hipSuccess ||
-        hipMemset(ctx->Z, 0, reg) != hipSuccess || hipMemset(ctx->ds, 0, sizeof(DevScalars)) != hipSuccess)
+    if (hipMemset(ctx->Z, 0, reg) != hipSuccess || hipMemset(ctx->ds, 0, sizeof(DevScalars)) != hipSuccess)
         return cleanup(MHD3D_EHIP, "hipMemset failed");
     if (hipHostMalloc(&ctx->host_stage, (size_t)g.fs * sizeof(double)) != hipSuccess)
         return cleanup(MHD3D_ENOMEM, "hipHostMalloc failed");
-    ctx->bytes = (int64_t)(3 * reg + nmet * sizeof(double) + sizeof(DevScalars));
+    ctx->bytes = (int64_t)((nU64 + 1) * reg + (ctx->mixed ? 2 * reg32 : 0) + nmet * sizeof(double) +
+                           sizeof(DevScalars));
     const char* km = std::getenv("MHD3D_KERNEL");
     if (km && std::strcmp(km, "point") == 0) ctx->kernel_mode = 1;
-    const char* sw = std::getenv("MHD3D_MAG_WAVES");
-    if (sw && std::atoi(sw) == 2) ctx->stream_waves_mag = 2;
-    const char* ob = std::getenv("MHD3D_BARRIERS");
-    if (ob && std::atoi(ob) == 2) ctx->one_barrier_mag = false;
-    if (ob && std::atoi(ob) == 1) ctx->one_barrier_hydro = true;
+    // MHD3D_TUNE_<FP64|MIXED>_<HYDRO|MHD>=waves,sched overrides a variant
+    static const char* tune_env[2][2] = {{"MHD3D_TUNE_FP64_HYDRO", "MHD3D_TUNE_FP64_MHD"},
+                                         {"MHD3D_TUNE_MIXED_HYDRO", "MHD3D_TUNE_MIXED_MHD"}};
+    for (int fm = 0; fm < 2; ++fm)
+        for (int ph = 0; ph < 2; ++ph) {
+            const char* v = std::getenv(tune_env[fm][ph]);
+            int w = 0, sc = 0;
+            if (v && std::sscanf(v, "%d,%d", &w, &sc) == 2 && (w == 1 || w == 2) && sc >= 0 && sc <= 3) {
+                ctx->tune_waves[fm][ph] = w;
+                ctx->tune_sched[fm][ph] = sc;
+            }
+        }
     const char* fc = std::getenv("MHD3D_FUSE_CFL");
     if (fc && std::atoi(fc) == 0) ctx->fuse_cfl = false;
+    if (ctx->mixed) ctx->kernel_mode = 0;   // the mixed form has only the streaming kernel
     *out = ctx;
     return MHD3D_OK;
 }
@@ -204,6 +241,10 @@ extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
     (void)hipFree(ctx->U[0]);
     (void)hipFree(ctx->U[1]);
     (void)hipFree(ctx->Z);
+    (void)hipFree(ctx->U32[0]);
+    (void)hipFree(ctx->U32[1]);
+    (void)hipFree(ctx->bgf);
+    (void)hipFree(ctx->bg64);
     (void)hipFree(ctx->F);
     (void)hipFree(ctx->ds);
     (void)hipFree(ctx->cfl_parts);
@@ -259,6 +300,84 @@ static int download_field(mhd3d_ctx* ctx, double* dst, const double* src)
     return MHD3D_OK;
 }
 
+// Background of the mixed form from the state being set: RO0(z), TT0(z) (and
+// BX0, BY0, BZ0 with lMag) are the horizontal means of the fields on every
+// plane (ghost planes included), P0 = RO0*TT0.  For the planes 1..nPz the
+// z differences the deviation-form fluxes add back are evaluated here in FP64
+// with the stencils of 3dmhdutils.h:195-220 (and, for D2z(BX0), D2z(BY0) on
+// the wall planes, the one-sided forms of 3dmhdsub.h:638-690); HB is the
+// hydrostatic residual RO0*GRAV - D1z(P0).  The background stays fixed until
+// the next set_state.
+static int set_background(mhd3d_ctx* ctx, const double* const* fields)
+{
+    const Geom& g = ctx->g;
+    const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;
+    const bool mag = ctx->nf == NF_MAG;
+    // means[k][z], k = RO, TT, BX, BY, BZ
+    std::vector<double> mean[5];
+    const int src_field[5] = {3, 4, 5, 6, 7};
+    for (int k = 0; k < 5; ++k) {
+        mean[k].assign(nz, 0.0);
+        if (k >= 2 && !mag) continue;
+        const double* F = fields[src_field[k]];
+        for (int z = 0; z < nz; ++z) {
+            double sum = 0.0;
+            for (int y = 1; y <= g.nPy; ++y)
+                for (int x = 1; x <= g.nPx; ++x) sum += F[((size_t)z * ny + y) * nx + x];
+            mean[k][z] = sum / ((double)g.nPx * g.nPy);
+        }
+    }
+    const std::vector<double>& R0 = mean[0];
+    const std::vector<double>& T0 = mean[1];
+    std::vector<float> bf((size_t)NBGF * nz, 0.0f);
+    std::vector<double> b64((size_t)NBG64 * nz, 0.0);
+    const Phys& c = ctx->c;
+    auto d1z = [&](const std::vector<double>& f, int z) {
+        return ((f[z + 1] - f[z - 1]) * ctx->h_dzz[z]) * c.hz;
+    };
+    auto d2z = [&](const std::vector<double>& f, int z) {
+        const double dzz = ctx->h_dzz[z];
+        return (f[z + 1] - f[z - 1]) * ctx->h_d2zz[z] * c.hz +
+               ((f[z + 1] - 2.0 * f[z]) + f[z - 1]) * (dzz * dzz) * c.h2z;
+    };
+    auto d2z_wall = [&](const std::vector<double>& f, int z, bool top) {
+        const double dzz = ctx->h_dzz[z];
+        const int s = top ? 1 : -1;
+        const double b0 = f[z], b1 = f[z + s], b2 = f[z + 2 * s], b3 = f[z + 3 * s];
+        const double first = top ? (-3.0 * b0 + 4.0 * b1 - b2) : (3.0 * b0 - 4.0 * b1 + b2);
+        return first * c.hz * ctx->h_d2zz[z] +
+               (2.0 * b0 - 5.0 * b1 + 4.0 * b2 - b3) * c.h2z * dzz * dzz;
+    };
+    std::vector<double> P0(nz);
+    for (int z = 0; z < nz; ++z) P0[z] = R0[z] * T0[z];
+    for (int z = 0; z < nz; ++z) {
+        bf[BG_R0 * nz + z] = (float)R0[z];
+        bf[BG_T0 * nz + z] = (float)T0[z];
+        b64[BG64_R0 * nz + z] = R0[z];
+        b64[BG64_T0 * nz + z] = T0[z];
+        for (int k = 0; k < 3; ++k) {
+            bf[(BG_B0 + k) * nz + z] = (float)mean[2 + k][z];
+            b64[(BG64_B0 + k) * nz + z] = mean[2 + k][z];
+        }
+        if (z >= 1 && z <= g.nPz) {
+            bf[BG_DZT0 * nz + z] = (float)d1z(T0, z);
+            bf[BG_D2ZT0 * nz + z] = (float)d2z(T0, z);
+            bf[BG_HB * nz + z] = (float)(R0[z] * c.GRAV - d1z(P0, z));
+            const bool wall = z == 1 || z == g.nPz;
+            for (int k = 0; k < 3; ++k) {
+                const std::vector<double>& B = mean[2 + k];
+                bf[(BG_DZB + k) * nz + z] = (float)d1z(B, z);
+                bf[(BG_D2ZB + k) * nz + z] =
+                    (float)((wall && k < 2) ? d2z_wall(B, z, z == 1) : d2z(B, z));
+            }
+        }
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->bgf, bf.data(), bf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->bg64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}
+
 extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
 {
     if (!ctx || !fields) return MHD3D_EINVAL;
@@ -269,10 +388,20 @@ extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
         int rc = upload_field(ctx, ctx->U[ctx->cur] + (size_t)f * ctx->g.fs, fields[f]);
         if (rc) return rc;
     }
-    // the other U buffer needs the z ghost planes too (stage kernels never write them)
-    HIPCHK(ctx, hipMemcpyAsync(ctx->U[1 - ctx->cur], ctx->U[ctx->cur],
-                               (size_t)ctx->g.fs * ctx->nf * sizeof(double),
-                               hipMemcpyDeviceToDevice, ctx->stream));
+    // the other U buffers need the z ghost planes too (stage kernels never
+    // write them)
+    const int64_t n = ctx->g.fs * ctx->nf;
+    if (ctx->mixed) {
+        int rc = set_background(ctx, fields);
+        if (rc) return rc;
+        for (int k = 0; k < 2; ++k)
+            to_f32_dev_kernel<<<1024, 256, 0, ctx->stream>>>(ctx->U[0], ctx->U32[k], ctx->g, ctx->nf,
+                                                            ctx->bg64);
+        HIPCHK(ctx, hipGetLastError());
+    } else {
+        HIPCHK(ctx, hipMemcpyAsync(ctx->U[1 - ctx->cur], ctx->U[ctx->cur], (size_t)n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
 }
@@ -291,16 +420,27 @@ extern "C" int mhd3d_get_state(mhd3d_ctx* ctx, double* const* fields)
 // ---------------------------------------------------------------------------
 // launches
 
-static StageArgs stage_args(mhd3d_ctx* ctx)
+// Operands of stage k (0: no stage, the current state only).
+static StageArgs stage_args(mhd3d_ctx* ctx, int k = 0)
 {
     StageArgs a;
     a.g = ctx->g;
     a.c = ctx->c;
     a.m = ctx->m;
     a.Uin = ctx->U[ctx->cur];
-    a.Uout = ctx->U[1 - ctx->cur];
+    a.Uout = ctx->mixed ? ctx->U[0] : ctx->U[1 - ctx->cur];
     a.Z = ctx->Z;
     a.dt = &ctx->ds->dt;
+    a.Uin32 = nullptr;
+    a.Uout32 = nullptr;
+    a.bgf = ctx->bgf;
+    a.bg64 = ctx->bg64;
+    if (ctx->mixed) {
+        // stage 1: U64 -> U32[0]; stage 2: U32[0] -> U32[1]; stage 3: U32[1] -> U64
+        if (k == 1) a.Uout32 = ctx->U32[0];
+        if (k == 2) { a.Uin32 = ctx->U32[0]; a.Uout32 = ctx->U32[1]; }
+        if (k == 3) a.Uin32 = ctx->U32[1];
+    }
     a.cfl_parts = nullptr;
     a.cfl_base = 0;
     return a;
@@ -321,13 +461,16 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
 {
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
-    if (ctx->kernel_mode == 1) {
+    const int fm = ctx->mixed ? 1 : 0, ph = MAG ? 1 : 0;
+    if (ctx->mixed) {
+        launch_stream_stage<float, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
+                                                    ctx->tune_sched[fm][ph]);
+    } else if (ctx->kernel_mode == 1) {
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
         stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
     } else {
-        launch_stream_stage<MAG, ROT, STAGE>(a, ctx->stream,
-                                             MAG ? ctx->stream_waves_mag : ctx->stream_waves_hydro,
-                                             MAG ? ctx->one_barrier_mag : ctx->one_barrier_hydro);
+        launch_stream_stage<double, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
+                                                     ctx->tune_sched[fm][ph]);
     }
 }
 
@@ -337,7 +480,10 @@ static void launch_wall_t(mhd3d_ctx* ctx, const StageArgs& a)
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
-    stage_wall_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+    if (ctx->mixed)
+        stage_wall_kernel<float, MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+    else
+        stage_wall_kernel<double, MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
 }
 
 template <int STAGE>
@@ -382,7 +528,7 @@ static int ensure_cfl_parts(mhd3d_ctx* ctx, int n)
 
 static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
 {
-    StageArgs a = stage_args(ctx);
+    StageArgs a = stage_args(ctx, k);
     if (fused_cfl && k == 3 && ctx->kernel_mode == 0) {
         int ns, nw;
         cfl_part_layout(ctx, ns, nw);
@@ -410,7 +556,7 @@ static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
     else if (k == 2) launch_stage_k<2>(ctx, a, true);
     else launch_stage_k<3>(ctx, a, true);
     HIPCHK(ctx, hipGetLastError());
-    ctx->cur = 1 - ctx->cur;
+    if (!ctx->mixed) ctx->cur = 1 - ctx->cur;
     return MHD3D_OK;
 }
 
@@ -504,6 +650,9 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
 extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
 {
     if (!ctx || stage < 1 || stage > 3) return MHD3D_EINVAL;
+    if (ctx->mixed)
+        return fail(ctx, MHD3D_EUNSUPPORTED,
+                    "mhd3d_stage: the mixed form keeps stage values in FP32 copies; use mhd3d_step");
     ctx->dt_ready = false;
     set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt);
     int rc = run_stage(ctx, stage);

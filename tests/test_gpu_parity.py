@@ -103,3 +103,18 @@ def test_unsupported_flags_fail_loudly():
     with pytest.raises(L.MhdError) as ei:
         Domain(Descriptor(cfg))
     assert ei.value.code == L.EUNSUPPORTED
+
+
+@pytest.mark.parametrize("tune", ["1,0", "1,1", "1,2", "1,3", "2,0", "2,1", "2,2", "2,3"])
+@pytest.mark.parametrize("case", ["hydro", "mhd_rot"])
+def test_schedule_variants_bit_identical(case, tune, monkeypatch):
+    """Every streaming-kernel variant (register budget, barriers, deferred
+    update) of the FP64 form is bit-identical to the oracle."""
+    env = "MHD3D_TUNE_FP64_" + ("MHD" if "mhd" in case else "HYDRO")
+    monkeypatch.setenv(env, tune)
+    desc, dom, orc, _ = make_pair((40, 24, 12), case, seed=31)
+    for _ in range(3):
+        orc.step()
+    dom.step(3)
+    for name, g in dom.state().items():
+        assert bit_identical_fraction(g, orc.get(name)) == 1.0, (case, tune, name)
