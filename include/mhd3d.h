@@ -161,6 +161,43 @@ const char* mhd3d_last_error(const mhd3d_ctx* ctx);
 /* Bytes of device memory the context holds. */
 int64_t mhd3d_device_bytes(const mhd3d_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: n independent domains (sizes, flags and parameters may differ;
+ * one precision form for all) advanced together -- one persistent launch per
+ * RK3 stage for the planes between the walls and one for the wall planes of
+ * every member, each member with its own CFL time step.  A member advances
+ * bit-for-bit as the same domain does alone through mhd3d_step.  No
+ * counterpart in the reference (its step() advances one domain, 3dmhdsub.h:31).
+ * ------------------------------------------------------------------------ */
+typedef struct mhd3d_ensemble mhd3d_ensemble;
+
+int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3d_ensemble** out);
+int mhd3d_ensemble_destroy(mhd3d_ensemble* e);
+const char* mhd3d_ensemble_last_error(const mhd3d_ensemble* e);   /* e == NULL: last create error */
+int mhd3d_ensemble_size(const mhd3d_ensemble* e);
+int64_t mhd3d_ensemble_cells(const mhd3d_ensemble* e);
+/* fields as for mhd3d_set_state / mhd3d_get_state, for member `member`;
+ * set_state also resets that member's clock */
+int mhd3d_ensemble_set_state(mhd3d_ensemble* e, int member, const double* const* fields);
+int mhd3d_ensemble_get_state(mhd3d_ensemble* e, int member, double* const* fields);
+/* nsteps RK3 steps of every member, DT from each member's CFL (TSTEP) */
+int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps);
+int mhd3d_ensemble_clock(mhd3d_ensemble* e, int member, int64_t* nIt, double* timt,
+                         double* timc, double* dt_last);
+/* on != 0: each step is replayed from a captured hipGraph */
+int mhd3d_ensemble_graph(mhd3d_ensemble* e, int on);
+/* kernel variant: waves (1, 2) = register budget, sched bits (1: one barrier
+ * per plane, 2: deferred RK update), chunk = planes per work item (0 = auto);
+ * the arithmetic does not depend on it */
+int mhd3d_ensemble_set_variant(mhd3d_ensemble* e, int waves, int sched, int chunk);
+int mhd3d_ensemble_get_variant(const mhd3d_ensemble* e, int* waves, int* sched, int* chunk);
+int mhd3d_ensemble_synchronize(mhd3d_ensemble* e);
+/* Autotuner: times `steps` steps of every variant (waves x sched x chunk, 32
+ * candidates) on the current states, restores states and clocks after each
+ * trial and keeps the fastest.  report: NULL or room for 4*32 doubles
+ * (waves, sched, chunk, ms per step); *ncand = number of trials. */
+int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* report, int* ncand);
+
 #ifdef __cplusplus
 }
 #endif

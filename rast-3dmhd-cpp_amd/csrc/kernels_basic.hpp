@@ -298,110 +298,117 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Wall planes of one RK stage: fluxes with the one-sided wall stencils, the
-// GAM update, BCON on the updated wall values, then the ZETA register.
+// One wall point of one RK stage: fluxes with the one-sided wall stencils, the
+// GAM update, BCON on the updated wall values, then the ZETA register; stage 3
+// adds the point to the CFL accumulator.  BCON reads the new values one and
+// two planes inward in the same column, so the caller must have updated them.
 // T = double: the FP64 form.  T = float: the mixed form (fluxes in FP32, in
 // deviation form, from the FP64 state at stage 1 and from the FP32 stage copy
 // at stages 2-3; the update, BCON and the Z register in FP64; BCON reads the
 // new inward planes from the FP32 copy at stages 1-2 and from the FP64 state
 // at stage 3).
 template <typename T, bool MAG, bool ROT, int STAGE>
-__global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
+__device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, bool top, CflAcc& cfl)
 {
     constexpr bool F64 = std::is_same<T, double>::value;
+    constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
+    const int z = top ? 1 : a.g.nPz;
+    const int zi1 = top ? z + 1 : z - 1;   // first plane inward
+    const int zi2 = top ? z + 2 : z - 2;   // second plane inward
+    const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+    FluxOutT<T> o;
+    if constexpr (F64) {
+        GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+        if (top)
+            flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
+        else
+            flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+    } else if constexpr (STAGE == 1) {
+        GAccDev<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
+                            a.g.nz, a.bgf, a.bg64};
+        if (top)
+            flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
+        else
+            flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
+    } else {
+        GAccDev<float> acc{a.Uin32, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
+                           a.g.nz, a.bgf, a.bg64};
+        if (top)
+            flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
+        else
+            flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
+    }
+    const double DT = *a.dt;
+    const double cg = a.c.GAM[STAGE - 1] * DT;
+    const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
+    const int64_t row = (int64_t)y * a.g.pitch + x;
+    const int64_t i = (int64_t)z * a.g.plane + row;
+    const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
+    const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
+    const int64_t fs = a.g.fs;
+    double un[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double zin = STAGE == 1 ? a.Uin[f * fs + i] : a.Z[f * fs + i];
+        un[f] = zin + cg * fsel(o, f);
+    }
+    // BCON (3dmhdsub.h:727-872) on the updated planes inward
+    const bool new64 = F64 || STAGE == 3;
+    auto Un = [&](int f, int64_t j, int zj) -> double {
+        if (new64) return a.Uout[f * fs + j];
+        return double(a.Uout32[f * fs + j]) + bg_sub(a, f, zj);
+    };
+    const Phys& c = a.c;
+    const double ro1 = Un(3, i1, zi1), ro2 = Un(3, i2, zi2);
+    un[0] = un[3] * (c.C43 * Un(0, i1, zi1) / ro1 - c.C13 * Un(0, i2, zi2) / ro2);
+    un[1] = un[3] * (c.C43 * Un(1, i1, zi1) / ro1 - c.C13 * Un(1, i2, zi2) / ro2);
+    un[2] = 0.0;
+    if (top) {
+        if (c.iTCon == 0 || c.iTCon == 4) {
+            un[4] = 1.0;
+            if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
+        } else if (c.iTCon == 1) {
+            const double DZ = c.ORZF / a.m.dzzdz[z];
+            un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) - c.C23 * DZ * c.THETA;
+        }
+    } else {
+        if (c.izCon == 0) {
+            un[4] = c.TB;
+        } else {
+            const double DZ = c.ORZF / a.m.dzzdz[z];
+            un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) -
+                    c.C23 * DZ * c.THETA / a.m.RKAPA[z];
+        }
+    }
+    if (MAG && c.iBCon == 0) un[7] = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        if (new64)
+            a.Uout[f * fs + i] = un[f];
+        else
+            a.Uout32[f * fs + i] = float(un[f] - bg_sub(a, f, z));
+        if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
+    }
+    if (STAGE == 3 && a.cfl_parts) {
+        double wp[4], up;
+        cfl_values(c, a.m, MAG, un[0], un[1], un[2], un[3], un[4], MAG ? un[5] : 0.0,
+                   MAG ? un[6] : 0.0, MAG ? un[7] : 0.0, x + 1, y + 1, z, wp, up);
+        cfl.add(wp, up);
+    }
+}
+
+// Wall planes of one RK stage as a separate launch (used with the point
+// kernel; the streaming kernel handles the walls of its own columns).
+template <typename T, bool MAG, bool ROT, int STAGE>
+__global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
+{
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const bool top = blockIdx.z == 0;
     // every thread reaches the block reduction below; threads outside the
     // domain only contribute the identity
-    const bool active = x < a.g.nPx && y < a.g.nPy;
-    constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
     CflAcc cfl;
-    if (active) {
-        const int z = top ? 1 : a.g.nPz;
-        const int zi1 = top ? z + 1 : z - 1;   // first plane inward
-        const int zi2 = top ? z + 2 : z - 2;   // second plane inward
-        const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
-        FluxOutT<T> o;
-        if constexpr (F64) {
-            GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
-            if (top)
-                flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
-            else
-                flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
-        } else if constexpr (STAGE == 1) {
-            GAccDev<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
-                                a.g.nz, a.bgf, a.bg64};
-            if (top)
-                flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
-            else
-                flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
-        } else {
-            GAccDev<float> acc{a.Uin32, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
-                               a.g.nz, a.bgf, a.bg64};
-            if (top)
-                flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
-            else
-                flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
-        }
-        const double DT = *a.dt;
-        const double cg = a.c.GAM[STAGE - 1] * DT;
-        const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
-        const int64_t row = (int64_t)y * a.g.pitch + x;
-        const int64_t i = (int64_t)z * a.g.plane + row;
-        const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
-        const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
-        const int64_t fs = a.g.fs;
-        double un[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const double zin = STAGE == 1 ? a.Uin[f * fs + i] : a.Z[f * fs + i];
-            un[f] = zin + cg * fsel(o, f);
-        }
-        // BCON (3dmhdsub.h:727-872) on the updated planes inward
-        const bool new64 = F64 || STAGE == 3;
-        auto Un = [&](int f, int64_t j, int zj) -> double {
-            if (new64) return a.Uout[f * fs + j];
-            return double(a.Uout32[f * fs + j]) + bg_sub(a, f, zj);
-        };
-        const Phys& c = a.c;
-        const double ro1 = Un(3, i1, zi1), ro2 = Un(3, i2, zi2);
-        un[0] = un[3] * (c.C43 * Un(0, i1, zi1) / ro1 - c.C13 * Un(0, i2, zi2) / ro2);
-        un[1] = un[3] * (c.C43 * Un(1, i1, zi1) / ro1 - c.C13 * Un(1, i2, zi2) / ro2);
-        un[2] = 0.0;
-        if (top) {
-            if (c.iTCon == 0 || c.iTCon == 4) {
-                un[4] = 1.0;
-                if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
-            } else if (c.iTCon == 1) {
-                const double DZ = c.ORZF / a.m.dzzdz[z];
-                un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) - c.C23 * DZ * c.THETA;
-            }
-        } else {
-            if (c.izCon == 0) {
-                un[4] = c.TB;
-            } else {
-                const double DZ = c.ORZF / a.m.dzzdz[z];
-                un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) -
-                        c.C23 * DZ * c.THETA / a.m.RKAPA[z];
-            }
-        }
-        if (MAG && c.iBCon == 0) un[7] = 0.0;
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            if (new64)
-                a.Uout[f * fs + i] = un[f];
-            else
-                a.Uout32[f * fs + i] = float(un[f] - bg_sub(a, f, z));
-            if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
-        }
-        if (STAGE == 3 && a.cfl_parts) {
-            double wp[4], up;
-            cfl_values(c, a.m, MAG, un[0], un[1], un[2], un[3], un[4], MAG ? un[5] : 0.0,
-                       MAG ? un[6] : 0.0, MAG ? un[7] : 0.0, x + 1, y + 1, z, wp, up);
-            cfl.add(wp, up);
-        }
-    }
+    if (x < a.g.nPx && y < a.g.nPy) wall_point<T, MAG, ROT, STAGE>(a, x, y, top, cfl);
     if (STAGE == 3 && a.cfl_parts) {
         const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + b);
@@ -512,9 +519,8 @@ struct DevScalars {
 // Reduces the per-block partials in a fixed order (one block of 256 threads:
 // strided per-thread reduction, then the block reduction) and forms
 // DT = SFF * min(...) as 3dmhdsub.h:101-105.
-__global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __restrict__ part,
-                                                           int nparts, DevScalars* s, Phys c,
-                                                           int mag)
+__device__ __forceinline__ void cfl_finalize_block(const CflPart* __restrict__ part, int nparts,
+                                                   DevScalars* s, const Phys& c, int mag)
 {
     __shared__ double sh[6][256];
     double w[4] = {1e99, 1e99, 1e99, 1e99};
@@ -547,6 +553,13 @@ __global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __rest
         else
             s->dt = c.SFF * ref_min(ref_min(w[0], w[1]), w[2]);
     }
+}
+
+__global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __restrict__ part,
+                                                           int nparts, DevScalars* s, Phys c,
+                                                           int mag)
+{
+    cfl_finalize_block(part, nparts, s, c, mag);
 }
 
 __global__ void set_dt_kernel(DevScalars* s, double dt) { s->dt = dt; }

@@ -177,9 +177,25 @@ struct SAcc {
 // WAVES: minimum waves per SIMD the register allocation must allow (2 keeps
 // 256 VGPRs per wave; 1 lets the FP64 MHD variant use AGPRs instead of
 // spilling).
-template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER>
-__global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int ntx,
-                                                                  int nty, int xcd_remap)
+// LDS of one work item: NRING planes of u, v, w [BX, BY, BZ] and NSTAR
+// centre planes of RU, RV, RW, TT, P.
+template <typename T, bool MAG, bool ONE_BARRIER>
+struct StreamLds {
+    static constexpr int NR = MAG ? 6 : 3;
+    static constexpr int NS = 5;
+    static constexpr int NRING = ONE_BARRIER ? 4 : 3;
+    static constexpr int NSTAR = ONE_BARRIER ? 3 : 2;
+    static constexpr int bytes = (int)sizeof(T) * (NRING * NR + NSTAR * NS) * SHP;
+};
+
+// One work item of a stage: the 32 x 8 tile (tile_x, tile_y) over the
+// interior planes zb .. ze-1 (the wall planes are a separate launch: their
+// one-sided stencils from global memory would raise this kernel's register
+// budget).  lds: StreamLds::bytes of shared memory.  Stage 3 accumulates the
+// CFL terms of every point it updates into cfl.
+template <typename T, bool MAG, bool ROT, int STAGE, bool ONE_BARRIER, bool DEFER>
+__device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int tile_y, int zb,
+                                           int ze, char* lds, CflAcc& cfl)
 {
     constexpr bool F64 = std::is_same<T, double>::value;
     constexpr int NR = MAG ? 6 : 3;   // ring fields: u v w [BX BY BZ]
@@ -196,34 +212,14 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     constexpr bool ZLOAD = STAGE > 1 || !F64;
     // storage type of the stencil input: the FP64 state, or the FP32 copy
     using TI = typename std::conditional<F64 || STAGE == 1, double, float>::type;
-    __shared__ T s_uvw[NRING][NR][SHP];
-    __shared__ T s_st[NSTAR][NS][SHP];
+    T (*s_uvw)[NR][SHP] = reinterpret_cast<T (*)[NR][SHP]>(lds);
+    T (*s_st)[NS][SHP] = reinterpret_cast<T (*)[NS][SHP]>(lds + sizeof(T) * NRING * NR * SHP);
 
     const Geom& g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int t = ty * STX + tx;
 
-    // Block -> tile: with xcd_remap, consecutive tiles go to the same XCD
-    // (blocks are dealt to the 8 XCDs round-robin) so halo re-reads hit its L2.
-    const int nb = gridDim.x;
-    const int bid = blockIdx.x;
-    const int logical = xcd_remap ? (bid % 8) * (nb / 8) + bid / 8 : bid;
-    const int tile_x = logical % ntx;
-    const int rest = logical / ntx;
-    const int tile_y = rest % nty;
-    const int chunk = rest / nty;
     const int x0 = tile_x * STX, y0 = tile_y * STY;
-    const int zb = 2 + chunk * zchunk;
-    const int ze = min(zb + zchunk, g.nPz);   // planes zb .. ze-1 (interior 2 .. nPz-1)
-    if (zb >= ze) {
-        if (STAGE == 3 && a.cfl_parts && threadIdx.x == 0 && threadIdx.y == 0) {
-            CflPart& q = a.cfl_parts[a.cfl_base + blockIdx.x];
-            for (int k = 0; k < 4; ++k) q.w[k] = 1e99;
-            q.um = 0.0;
-            q.bad = 0.0;
-        }
-        return;
-    }
 
     // own column and (for t < SNH) one halo point, wrapped periodically
     const int gx = (x0 + tx) % g.nPx, gy = (y0 + ty) % g.nPy;
@@ -272,7 +268,6 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
         for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)zb * g.plane + go];
     }
 
-    CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
     // RK update of plane zq from its tendencies o and its Z-type operand zin
     // (Uout = zin + GAM*DT*F, Z = Uout + ZETA*DT*F; stage 3 also the CFL terms)
     auto update = [&](int zq, const FluxOutT<T>& o, const double* zin) {
@@ -435,6 +430,28 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
         if (has_halo) halo = halo_n;
     }
     if (DEFER && zp >= 0) update(zp, fpend, zpend);
+}
+
+// One stage of one domain: block -> (tile, z chunk) work item.  With
+// xcd_remap, consecutive tiles go to the same XCD (blocks are dealt to the 8
+// XCDs round-robin) so halo re-reads hit its L2.  Chunk k covers the planes
+// 2 + k*zchunk .. (the last chunk runs to nPz-1).
+template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER>
+__global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int nch,
+                                                                  int ntx, int nty, int xcd_remap)
+{
+    __shared__ __align__(16) char lds[StreamLds<T, MAG, ONE_BARRIER>::bytes];
+    const int nb = gridDim.x;
+    const int bid = blockIdx.x;
+    const int logical = xcd_remap ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+    const int tile_x = logical % ntx;
+    const int rest = logical / ntx;
+    const int tile_y = rest % nty;
+    const int chunk = rest / nty;
+    const int zb = 2 + chunk * zchunk;
+    const int ze = chunk == nch - 1 ? a.g.nPz : zb + zchunk;
+    CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
+    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER>(a, tile_x, tile_y, zb, ze, lds, cfl);
     if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
 }
 
@@ -442,6 +459,8 @@ struct StreamPlan {
     int zchunk, nchunks, ntx, nty, nblocks, xcd_remap;
 };
 
+// Work decomposition of one domain: 32 x 8 tiles times z chunks of at least
+// 8 planes (the last chunk takes the remainder), about target_blocks items.
 inline StreamPlan stream_plan(const Geom& g, int target_blocks = 1024)
 {
     StreamPlan p;
@@ -453,8 +472,8 @@ inline StreamPlan stream_plan(const Geom& g, int target_blocks = 1024)
     const int maxch = nint / 8 > 0 ? nint / 8 : 1;
     if (nch > maxch) nch = maxch;
     if (nch < 1) nch = 1;
-    p.zchunk = (nint + nch - 1) / nch;
-    p.nchunks = (nint + p.zchunk - 1) / p.zchunk;
+    p.zchunk = nint / nch;
+    p.nchunks = nch;
     p.nblocks = tiles * p.nchunks;
     p.xcd_remap = (p.nblocks % 8 == 0) ? 1 : 0;
     return p;
@@ -469,19 +488,19 @@ static void launch_stream_w(const StageArgs& a, hipStream_t s, const StreamPlan&
     switch (sched & 3) {
     case 0:
         stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, false, false><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+            a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
         break;
     case 1:
         stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, true, false><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+            a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
         break;
     case 2:
         stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, false, true><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+            a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
         break;
     default:
         stage_stream_kernel<T, MAG, ROT, STAGE, WAVES, true, true><<<p.nblocks, blk, 0, s>>>(
-            a, p.zchunk, p.ntx, p.nty, p.xcd_remap);
+            a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
         break;
     }
 }

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -22,6 +23,7 @@
 #include "../../include/mhd3d.h"
 #include "kernels_basic.hpp"
 #include "kernels_stream.hpp"
+#include "kernels_ensemble.hpp"
 
 using namespace mhd;
 
@@ -786,4 +788,515 @@ extern "C" int mhd3d_synchronize(mhd3d_ctx* ctx)
     if (!ctx) return MHD3D_EINVAL;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
+}
+
+// ===========================================================================
+// Ensembles: independent domains advanced together (kernels_ensemble.hpp).
+
+struct mhd3d_ensemble {
+    std::vector<mhd3d_ctx*> mem;
+    int n = 0;
+    bool mixed = false;
+    hipStream_t stream = nullptr;
+    DevScalars* ds = nullptr;          // one per member
+    StageArgs* d_args = nullptr;       // [parity at step start][stage][member]
+    WorkItem* d_items = nullptr;
+    WallItem* d_walls = nullptr;
+    FinishArgs* d_fin = nullptr;
+    int nitems = 0, nwalls = 0;
+    int grid_int = 0, grid_wall = 0;
+    int waves = 1, sched = 0, chunk = 0;   // kernel variant; chunk: planes per item (0 = auto)
+    int parity = 0;                    // FP64: U buffer holding every member's state
+    bool dt_ready = false;
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int64_t cells = 0;
+    std::string err;
+};
+
+static std::string g_ens_create_error;
+
+static int ens_fail(mhd3d_ensemble* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+#define ENSCHK(e, call)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (e)->err = std::string(#call) + ": " + hipGetErrorString(e_);             \
+            return MHD3D_EHIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+template <typename T, int STAGE, int WAVES>
+static void ens_stage_launch(const mhd3d_ensemble* e, const StageArgs* args, hipStream_t s, int sched)
+{
+    dim3 blk(STX, STY, 1);
+    switch (sched & 3) {
+    case 0: ensemble_stage_kernel<T, STAGE, WAVES, false, false><<<e->grid_int, blk, 0, s>>>(args, e->d_items, e->nitems); break;
+    case 1: ensemble_stage_kernel<T, STAGE, WAVES, true, false><<<e->grid_int, blk, 0, s>>>(args, e->d_items, e->nitems); break;
+    case 2: ensemble_stage_kernel<T, STAGE, WAVES, false, true><<<e->grid_int, blk, 0, s>>>(args, e->d_items, e->nitems); break;
+    default: ensemble_stage_kernel<T, STAGE, WAVES, true, true><<<e->grid_int, blk, 0, s>>>(args, e->d_items, e->nitems); break;
+    }
+}
+
+template <typename T, int STAGE, int WAVES>
+static const void* ens_stage_fn(int sched)
+{
+    switch (sched & 3) {
+    case 0: return (const void*)ensemble_stage_kernel<T, STAGE, WAVES, false, false>;
+    case 1: return (const void*)ensemble_stage_kernel<T, STAGE, WAVES, true, false>;
+    case 2: return (const void*)ensemble_stage_kernel<T, STAGE, WAVES, false, true>;
+    default: return (const void*)ensemble_stage_kernel<T, STAGE, WAVES, true, true>;
+    }
+}
+
+template <typename T>
+static void ens_stage(const mhd3d_ensemble* e, int k, const StageArgs* args)
+{
+    if (e->waves == 1) {
+        if (k == 1) ens_stage_launch<T, 1, 1>(e, args, e->stream, e->sched);
+        else if (k == 2) ens_stage_launch<T, 2, 1>(e, args, e->stream, e->sched);
+        else ens_stage_launch<T, 3, 1>(e, args, e->stream, e->sched);
+    } else {
+        if (k == 1) ens_stage_launch<T, 1, 2>(e, args, e->stream, e->sched);
+        else if (k == 2) ens_stage_launch<T, 2, 2>(e, args, e->stream, e->sched);
+        else ens_stage_launch<T, 3, 2>(e, args, e->stream, e->sched);
+    }
+    dim3 wblk(64, 4, 1);
+    if (k == 1) ensemble_wall_kernel<T, 1><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+    else if (k == 2) ensemble_wall_kernel<T, 2><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+    else ensemble_wall_kernel<T, 3><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+}
+
+// Resident blocks of the chosen interior variant on the whole device.
+static int ens_resident_blocks(const mhd3d_ensemble* e)
+{
+    const void* fn;
+    if (e->mixed)
+        fn = e->waves == 1 ? ens_stage_fn<float, 2, 1>(e->sched) : ens_stage_fn<float, 2, 2>(e->sched);
+    else
+        fn = e->waves == 1 ? ens_stage_fn<double, 2, 1>(e->sched) : ens_stage_fn<double, 2, 2>(e->sched);
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, STX * STY, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
+    return per_cu * ncu;
+}
+
+static void ens_drop_graphs(mhd3d_ensemble* e)
+{
+    for (int p = 0; p < 2; ++p)
+        if (e->gexec[p]) {
+            (void)hipGraphExecDestroy(e->gexec[p]);
+            e->gexec[p] = nullptr;
+        }
+}
+
+// Work lists, operand tables and grid sizes for the current variant.
+static int ens_build(mhd3d_ensemble* e)
+{
+    ens_drop_graphs(e);
+    std::vector<WorkItem> items;
+    std::vector<WallItem> walls;
+    std::vector<FinishArgs> fin(e->n);
+    std::vector<double> cost;
+    for (int m = 0; m < e->n; ++m) {
+        mhd3d_ctx* c = e->mem[m];
+        const Geom& g = c->g;
+        const int flags = (c->d.lMag ? 1 : 0) | (c->d.lRot ? 2 : 0);
+        StreamPlan p = stream_plan(g, 256);
+        if (e->chunk > 0) {
+            const int nint = g.nPz - 2;
+            p.nchunks = nint / e->chunk > 0 ? nint / e->chunk : 1;
+            p.zchunk = nint / p.nchunks;
+        }
+        int part = 0;
+        for (int ch = 0; ch < p.nchunks; ++ch)
+            for (int ty = 0; ty < p.nty; ++ty)
+                for (int tx = 0; tx < p.ntx; ++tx) {
+                    WorkItem w{};
+                    w.member = m; w.flags = flags; w.tile_x = tx; w.tile_y = ty;
+                    w.zb = 2 + ch * p.zchunk;
+                    w.ze = ch == p.nchunks - 1 ? g.nPz : w.zb + p.zchunk;
+                    w.part = part++;
+                    items.push_back(w);
+                }
+        for (int top = 1; top >= 0; --top)
+            for (int y0 = 0; y0 < g.nPy; y0 += 4)
+                for (int x0 = 0; x0 < g.nPx; x0 += 64) {
+                    WallItem w{};
+                    w.member = m; w.flags = flags; w.x0 = x0; w.y0 = y0; w.top = top;
+                    w.part = part++;
+                    walls.push_back(w);
+                }
+        // room for the fused partials and for the standalone TSTEP pass
+        const int nsa = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * (g.nPz < 8 ? g.nPz : 8);
+        int rc = ensure_cfl_parts(c, part > nsa ? part : nsa);
+        if (rc) return ens_fail(e, rc, c->err);
+        fin[m].parts = c->cfl_parts;
+        fin[m].nparts = part;
+        fin[m].mag = c->d.lMag ? 1 : 0;
+        fin[m].ds = e->ds + m;
+        fin[m].c = c->c;
+    }
+    // largest items first (a z chunk of an MHD member costs about twice a hydro one)
+    std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) {
+        const int ca = (a.ze - a.zb) * ((a.flags & 1) ? 2 : 1);
+        const int cb = (b.ze - b.zb) * ((b.flags & 1) ? 2 : 1);
+        return ca > cb;
+    });
+    // operand tables: [p0][k-1][m], member cur during stage k = p0 ^ ((k-1) & 1)
+    std::vector<StageArgs> args((size_t)2 * 3 * e->n);
+    for (int p0 = 0; p0 < 2; ++p0)
+        for (int k = 1; k <= 3; ++k)
+            for (int m = 0; m < e->n; ++m) {
+                mhd3d_ctx* c = e->mem[m];
+                const int save = c->cur;
+                c->cur = e->mixed ? 0 : (p0 ^ ((k - 1) & 1));
+                StageArgs a = stage_args(c, k);
+                c->cur = save;
+                a.dt = &e->ds[m].dt;
+                a.cfl_parts = k == 3 ? c->cfl_parts : nullptr;
+                a.cfl_base = 0;
+                args[((size_t)p0 * 3 + (k - 1)) * e->n + m] = a;
+            }
+    (void)hipFree(e->d_items);
+    (void)hipFree(e->d_walls);
+    (void)hipFree(e->d_fin);
+    (void)hipFree(e->d_args);
+    e->d_items = nullptr; e->d_walls = nullptr; e->d_fin = nullptr; e->d_args = nullptr;
+    ENSCHK(e, hipMalloc(&e->d_items, sizeof(WorkItem) * items.size()));
+    ENSCHK(e, hipMalloc(&e->d_walls, sizeof(WallItem) * walls.size()));
+    ENSCHK(e, hipMalloc(&e->d_fin, sizeof(FinishArgs) * fin.size()));
+    ENSCHK(e, hipMalloc(&e->d_args, sizeof(StageArgs) * args.size()));
+    ENSCHK(e, hipMemcpy(e->d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
+    ENSCHK(e, hipMemcpy(e->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice));
+    ENSCHK(e, hipMemcpy(e->d_fin, fin.data(), sizeof(FinishArgs) * fin.size(), hipMemcpyHostToDevice));
+    ENSCHK(e, hipMemcpy(e->d_args, args.data(), sizeof(StageArgs) * args.size(), hipMemcpyHostToDevice));
+    e->nitems = (int)items.size();
+    e->nwalls = (int)walls.size();
+    const int res = ens_resident_blocks(e);
+    e->grid_int = e->nitems < res ? e->nitems : res;
+    e->grid_wall = e->nwalls < 2048 ? e->nwalls : 2048;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3d_ensemble** out)
+{
+    if (!out || !descs || n < 1) return MHD3D_EINVAL;
+    *out = nullptr;
+    mhd3d_ensemble* e = new mhd3d_ensemble();
+    auto bail = [&](int code, const std::string& m) {
+        g_ens_create_error = m;
+        mhd3d_ensemble_destroy(e);
+        return code;
+    };
+    e->n = n;
+    e->mixed = descs[0] && descs[0]->precision == MHD3D_MIXED;
+    for (int m = 0; m < n; ++m) {
+        if (!descs[m]) return bail(MHD3D_EINVAL, "null descriptor");
+        if ((descs[m]->precision == MHD3D_MIXED) != e->mixed)
+            return bail(MHD3D_EINVAL, "ensemble members must share one precision form");
+        mhd3d_ctx* c = nullptr;
+        int rc = mhd3d_create(descs[m], &c);
+        if (rc) return bail(rc, "member " + std::to_string(m) + ": " + g_create_error);
+        c->kernel_mode = 0;
+        e->mem.push_back(c);
+        e->cells += (int64_t)descs[m]->nPx * descs[m]->nPy * descs[m]->nPz;
+    }
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(MHD3D_EHIP, "hipStreamCreate failed");
+    if (hipMalloc(&e->ds, sizeof(DevScalars) * n) != hipSuccess)
+        return bail(MHD3D_ENOMEM, "hipMalloc(scalars) failed");
+    if (hipMemset(e->ds, 0, sizeof(DevScalars) * n) != hipSuccess)
+        return bail(MHD3D_EHIP, "hipMemset failed");
+    e->waves = e->mixed ? 2 : 1;
+    e->sched = e->mixed ? 2 : 0;
+    const char* tv = std::getenv(e->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
+    int w = 0, sc = 0, ch = 0;
+    if (tv && std::sscanf(tv, "%d,%d,%d", &w, &sc, &ch) >= 2 && (w == 1 || w == 2) && sc >= 0 && sc <= 3) {
+        e->waves = w;
+        e->sched = sc;
+        e->chunk = ch > 0 ? ch : 0;
+    }
+    int rc = ens_build(e);
+    if (rc) return bail(rc, e->err);
+    *out = e;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_destroy(mhd3d_ensemble* e)
+{
+    if (!e) return MHD3D_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_drop_graphs(e);
+    for (auto* c : e->mem) mhd3d_destroy(c);
+    (void)hipFree(e->ds);
+    (void)hipFree(e->d_items);
+    (void)hipFree(e->d_walls);
+    (void)hipFree(e->d_fin);
+    (void)hipFree(e->d_args);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return MHD3D_OK;
+}
+
+extern "C" const char* mhd3d_ensemble_last_error(const mhd3d_ensemble* e)
+{
+    return e ? e->err.c_str() : g_ens_create_error.c_str();
+}
+
+extern "C" int mhd3d_ensemble_size(const mhd3d_ensemble* e) { return e ? e->n : 0; }
+
+extern "C" int mhd3d_ensemble_set_state(mhd3d_ensemble* e, int member, const double* const* fields)
+{
+    if (!e || member < 0 || member >= e->n) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    mhd3d_ctx* c = e->mem[member];
+    c->cur = e->mixed ? 0 : e->parity;
+    int rc = mhd3d_set_state(c, fields);
+    if (rc) return ens_fail(e, rc, c->err);
+    // a fresh state needs its TSTEP; reset the member's clock
+    e->dt_ready = false;
+    DevScalars z{};
+    ENSCHK(e, hipMemcpy(e->ds + member, &z, sizeof(z), hipMemcpyHostToDevice));
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_get_state(mhd3d_ensemble* e, int member, double* const* fields)
+{
+    if (!e || member < 0 || member >= e->n) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    mhd3d_ctx* c = e->mem[member];
+    c->cur = e->mixed ? 0 : e->parity;
+    int rc = mhd3d_get_state(c, fields);
+    if (rc) return ens_fail(e, rc, c->err);
+    return MHD3D_OK;
+}
+
+// TSTEP of every member on its current state (the first step of a call
+// after set_state; later steps take it from the fused stage-3 partials).
+static int ens_cfl_all(mhd3d_ensemble* e)
+{
+    for (int m = 0; m < e->n; ++m) {
+        mhd3d_ctx* c = e->mem[m];
+        const Geom& g = c->g;
+        dim3 blk(64, 4, 1);
+        const int zchunks = g.nPz < 8 ? g.nPz : 8;
+        dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, zchunks);
+        const int nparts = (int)(grd.x * grd.y * grd.z);   // within the capacity ens_build ensured
+        const double* U = c->U[e->mixed ? 0 : e->parity];
+        if (c->d.lMag)
+            cfl_kernel<true><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
+        else
+            cfl_kernel<false><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
+        cfl_finalize_kernel<<<1, 256, 0, e->stream>>>(c->cfl_parts, nparts, e->ds + m, c->c,
+                                                      c->d.lMag ? 1 : 0);
+    }
+    ENSCHK(e, hipGetLastError());
+    return MHD3D_OK;
+}
+
+static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
+{
+    for (int k = 1; k <= 3; ++k) {
+        const StageArgs* args = e->d_args + ((size_t)p0 * 3 + (k - 1)) * e->n;
+        if (e->mixed) ens_stage<float>(e, k, args);
+        else ens_stage<double>(e, k, args);
+    }
+    ensemble_finish_kernel<<<e->n, 256, 0, e->stream>>>(e->d_fin);
+}
+
+extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
+{
+    if (!e || nsteps < 0) return MHD3D_EINVAL;
+    if (!e->dt_ready) {
+        int rc = ens_cfl_all(e);
+        if (rc) return rc;
+        e->dt_ready = true;
+    }
+    ensemble_clear_kernel<<<(e->n + 255) / 256, 256, 0, e->stream>>>(e->d_fin, e->n);
+    for (int s = 0; s < nsteps; ++s) {
+        const int p0 = e->mixed ? 0 : e->parity;
+        if (e->use_graph) {
+            if (!e->gexec[p0]) {
+                hipGraph_t gr = nullptr;
+                ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                ens_enqueue_step(e, p0);
+                ENSCHK(e, hipStreamEndCapture(e->stream, &gr));
+                hipError_t ie = hipGraphInstantiate(&e->gexec[p0], gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (ie != hipSuccess) return ens_fail(e, MHD3D_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            }
+            ENSCHK(e, hipGraphLaunch(e->gexec[p0], e->stream));
+        } else {
+            ens_enqueue_step(e, p0);
+        }
+        if (!e->mixed) e->parity ^= 1;
+    }
+    ENSCHK(e, hipGetLastError());
+    std::vector<DevScalars> hs(e->n);
+    ENSCHK(e, hipMemcpyAsync(hs.data(), e->ds, sizeof(DevScalars) * e->n, hipMemcpyDeviceToHost, e->stream));
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    for (int m = 0; m < e->n; ++m) {
+        e->mem[m]->cur = e->mixed ? 0 : e->parity;
+        if (hs[m].nonfinite)
+            return ens_fail(e, MHD3D_ENONFINITE, "member " + std::to_string(m) + ": state is not finite after the step");
+    }
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_clock(mhd3d_ensemble* e, int member, int64_t* nIt, double* timt,
+                                    double* timc, double* dt_last)
+{
+    if (!e || member < 0 || member >= e->n) return MHD3D_EINVAL;
+    DevScalars hs;
+    ENSCHK(e, hipMemcpyAsync(&hs, e->ds + member, sizeof(hs), hipMemcpyDeviceToHost, e->stream));
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    if (nIt) *nIt = (int64_t)hs.nit;
+    if (timc) *timc = hs.timc;
+    if (timt) *timt = e->mem[member]->d.TIMI + hs.timc;
+    if (dt_last) *dt_last = hs.dt_last;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_graph(mhd3d_ensemble* e, int on)
+{
+    if (!e) return MHD3D_EINVAL;
+    e->use_graph = on != 0;
+    if (!e->use_graph) ens_drop_graphs(e);
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_set_variant(mhd3d_ensemble* e, int waves, int sched, int chunk)
+{
+    if (!e || (waves != 1 && waves != 2) || sched < 0 || sched > 3 || chunk < 0) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    e->waves = waves;
+    e->sched = sched;
+    e->chunk = chunk;
+    return ens_build(e);
+}
+
+extern "C" int mhd3d_ensemble_get_variant(const mhd3d_ensemble* e, int* waves, int* sched, int* chunk)
+{
+    if (!e) return MHD3D_EINVAL;
+    if (waves) *waves = e->waves;
+    if (sched) *sched = e->sched;
+    if (chunk) *chunk = e->chunk;
+    return MHD3D_OK;
+}
+
+extern "C" int64_t mhd3d_ensemble_cells(const mhd3d_ensemble* e) { return e ? e->cells : 0; }
+
+extern "C" int mhd3d_ensemble_synchronize(mhd3d_ensemble* e)
+{
+    if (!e) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    return MHD3D_OK;
+}
+
+// Autotuner: times `steps` ensemble steps for every kernel variant in the
+// candidate set (register budget x barrier/update schedule x planes per work
+// item) on the current states, restores the states and clocks after each
+// trial, and keeps the fastest variant.  Results do not depend on the
+// variant; only the time does.  report (optional, 4 * ncand doubles): waves,
+// sched, chunk, ms per step of each trial.
+extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* report, int* ncand_out)
+{
+    if (!e || steps < 1) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    // snapshot: every member's state register, and the device scalars
+    std::vector<double*> snap(e->n, nullptr);
+    const int p_save = e->parity;
+    const bool ready_save = e->dt_ready;
+    DevScalars* ds_snap = nullptr;
+    auto cleanup = [&]() {
+        for (double* p : snap) (void)hipFree(p);
+        (void)hipFree(ds_snap);
+    };
+    for (int m = 0; m < e->n; ++m) {
+        mhd3d_ctx* c = e->mem[m];
+        const size_t reg = (size_t)c->g.fs * c->nf * sizeof(double);
+        if (hipMalloc(&snap[m], reg) != hipSuccess) { cleanup(); return ens_fail(e, MHD3D_ENOMEM, "autotune snapshot"); }
+        if (hipMemcpyAsync(snap[m], c->U[e->mixed ? 0 : e->parity], reg, hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
+            cleanup();
+            return ens_fail(e, MHD3D_EHIP, "autotune snapshot copy");
+        }
+    }
+    if (hipMalloc(&ds_snap, sizeof(DevScalars) * e->n) != hipSuccess) { cleanup(); return ens_fail(e, MHD3D_ENOMEM, "autotune snapshot"); }
+    (void)hipMemcpyAsync(ds_snap, e->ds, sizeof(DevScalars) * e->n, hipMemcpyDeviceToDevice, e->stream);
+    auto restore = [&]() -> int {
+        e->parity = p_save;
+        for (int m = 0; m < e->n; ++m) {
+            mhd3d_ctx* c = e->mem[m];
+            const size_t reg = (size_t)c->g.fs * c->nf * sizeof(double);
+            ENSCHK(e, hipMemcpyAsync(c->U[e->mixed ? 0 : e->parity], snap[m], reg, hipMemcpyDeviceToDevice, e->stream));
+            c->cur = e->mixed ? 0 : e->parity;
+        }
+        ENSCHK(e, hipMemcpyAsync(e->ds, ds_snap, sizeof(DevScalars) * e->n, hipMemcpyDeviceToDevice, e->stream));
+        ENSCHK(e, hipStreamSynchronize(e->stream));
+        e->dt_ready = ready_save;
+        return MHD3D_OK;
+    };
+    const int wv[2] = {1, 2};
+    const int sc[4] = {0, 1, 2, 3};
+    const int chs[4] = {0, 8, 16, 32};
+    int best_w = e->waves, best_s = e->sched, best_c = e->chunk;
+    double best = 1e300;
+    int ncand = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    const bool graph_save = e->use_graph;
+    int rc = MHD3D_OK;
+    for (int iw = 0; iw < 2 && !rc; ++iw)
+        for (int is = 0; is < 4 && !rc; ++is)
+            for (int ic = 0; ic < 4 && !rc; ++ic) {
+                e->waves = wv[iw];
+                e->sched = sc[is];
+                e->chunk = chs[ic];
+                rc = ens_build(e);
+                if (rc) break;
+                rc = mhd3d_ensemble_step(e, 1);   // warm-up (and the TSTEP pass if needed)
+                if (rc) break;
+                (void)hipEventRecord(t0, e->stream);
+                rc = mhd3d_ensemble_step(e, steps);
+                (void)hipEventRecord(t1, e->stream);
+                (void)hipEventSynchronize(t1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                const double per = ms / steps;
+                if (report) {
+                    report[4 * ncand + 0] = wv[iw];
+                    report[4 * ncand + 1] = sc[is];
+                    report[4 * ncand + 2] = chs[ic];
+                    report[4 * ncand + 3] = per;
+                }
+                ++ncand;
+                if (!rc && per < best) {
+                    best = per;
+                    best_w = wv[iw];
+                    best_s = sc[is];
+                    best_c = chs[ic];
+                }
+                int rr = restore();
+                if (!rc) rc = rr;
+            }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    e->use_graph = graph_save;
+    e->waves = best_w;
+    e->sched = best_s;
+    e->chunk = best_c;
+    int rb = ens_build(e);
+    cleanup();
+    if (ncand_out) *ncand_out = ncand;
+    return rc ? rc : rb;
 }

@@ -40,6 +40,20 @@ SIGNATURES = {
     "mhd3d_synchronize": (C.c_int, [_vp]),
     "mhd3d_last_error": (C.c_char_p, [_vp]),
     "mhd3d_device_bytes": (C.c_int64, [_vp]),
+    "mhd3d_ensemble_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "mhd3d_ensemble_destroy": (C.c_int, [_vp]),
+    "mhd3d_ensemble_last_error": (C.c_char_p, [_vp]),
+    "mhd3d_ensemble_size": (C.c_int, [_vp]),
+    "mhd3d_ensemble_cells": (C.c_int64, [_vp]),
+    "mhd3d_ensemble_set_state": (C.c_int, [_vp, C.c_int, _dpp]),
+    "mhd3d_ensemble_get_state": (C.c_int, [_vp, C.c_int, _dpp]),
+    "mhd3d_ensemble_step": (C.c_int, [_vp, C.c_int]),
+    "mhd3d_ensemble_clock": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), _dp, _dp, _dp]),
+    "mhd3d_ensemble_graph": (C.c_int, [_vp, C.c_int]),
+    "mhd3d_ensemble_set_variant": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "mhd3d_ensemble_get_variant": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mhd3d_ensemble_synchronize": (C.c_int, [_vp]),
+    "mhd3d_ensemble_autotune": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int)]),
 }
 
 
