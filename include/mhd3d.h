@@ -39,9 +39,13 @@ enum {
     MHD3D_EUNSUPPORTED = 2, /* flag combination this build does not implement   */
     MHD3D_EHIP = 3,         /* a HIP runtime call failed (see mhd3d_last_error) */
     MHD3D_ENOMEM = 4,       /* device allocation failed                         */
-    MHD3D_ENONFINITE = 5    /* state holds NaN/Inf after the step (the DT guard
+    MHD3D_ENONFINITE = 5,   /* state holds NaN/Inf after the step (the DT guard
                                of 3dmhd.cpp:292 cannot see it: minVal ignores NaN) */
+    MHD3D_EHALTED = 6       /* the DT guard stopped the run (mhd3d_set_dt_guard) */
 };
+
+/* DT guard threshold of the reference's driver: DT < 1.0e-8f (3dmhd.cpp:292). */
+#define MHD3D_REFERENCE_DT_MIN ((double)1.0e-8f)
 
 /* Arithmetic form of the stencil evaluation. */
 enum {
@@ -123,9 +127,17 @@ int mhd3d_get_state(mhd3d_ctx* ctx, double* const* fields);
 /* Advance `nsteps` RK3 steps: step() of 3dmhdsub.h:31-153, nsteps times.
  * fixed_dt == 0: each step computes DT from the CFL condition (TSTEP) and, on
  * return, *dt_inout holds the DT of the last step.  fixed_dt != 0: every step
- * uses *dt_inout.  Stops early (and returns MHD3D_ENONFINITE) when the state
- * stops being finite. */
+ * uses *dt_inout.  Returns MHD3D_ENONFINITE when the state is not finite
+ * after the steps, MHD3D_EHALTED when the DT guard stopped the run (the state
+ * is that after the step whose DT fell below the threshold; mhd3d_clock
+ * gives the steps done). */
 int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout);
+
+/* DT guard of the reference's driver (3dmhd.cpp:292-330): after any step
+ * with DT < dt_min the run halts -- later steps do nothing until the next
+ * set_state.  dt_min = 0 (the default) disables it; MHD3D_REFERENCE_DT_MIN is
+ * the reference's threshold.  Checked on the device, without host syncs. */
+int mhd3d_set_dt_guard(mhd3d_ctx* ctx, double dt_min);
 
 /* fluxes() of 3dmhdsub.h:194-696 on the current state, written to host
  * arrays in the field layout: FU,FV,FW,FR,FT and, with lMag, WW1,WW2,WW3
@@ -161,6 +173,17 @@ const char* mhd3d_last_error(const mhd3d_ctx* ctx);
 /* Bytes of device memory the context holds. */
 int64_t mhd3d_device_bytes(const mhd3d_ctx* ctx);
 
+/* Device-side synthetic stratified state (no counterpart in the reference,
+ * whose initial state comes from Static(), 3dmhdphsic.h:549-763): RO, TT from
+ * the z profiles RO_z[nz], TT_z[nz] (nz = nPz + 2, ghost planes included);
+ * RU, RV, RW = RO * amp * u on the planes between the walls; with lMag
+ * BX = b0 + b_amp*u, BY, BZ = b_amp*u on planes 1..nPz, BZ = 0 on the walls,
+ * ghost planes copies of the walls; u in [-1, 1) a counter-based hash of
+ * (seed, field, z, y, x).  For callers that must not ship fields from the
+ * host (the request loop of a server). */
+int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const double* RO_z, uint64_t seed,
+                          double amp, double b0, double b_amp);
+
 /* ------------------------------------------------------------------------
  * Ensembles: n independent domains (sizes, flags and parameters may differ;
  * one precision form for all) advanced together -- one persistent launch per
@@ -192,11 +215,60 @@ int mhd3d_ensemble_graph(mhd3d_ensemble* e, int on);
 int mhd3d_ensemble_set_variant(mhd3d_ensemble* e, int waves, int sched, int chunk);
 int mhd3d_ensemble_get_variant(const mhd3d_ensemble* e, int* waves, int* sched, int* chunk);
 int mhd3d_ensemble_synchronize(mhd3d_ensemble* e);
+/* the DT guard (mhd3d_set_dt_guard) for every member; a halted member stays
+ * as it is while the others advance */
+int mhd3d_ensemble_set_dt_guard(mhd3d_ensemble* e, double dt_min);
+int mhd3d_ensemble_halted(mhd3d_ensemble* e, int member, int* halted);
 /* Autotuner: times `steps` steps of every variant (waves x sched x chunk, 32
  * candidates) on the current states, restores states and clocks after each
  * trial and keeps the fastest.  report: NULL or room for 4*32 doubles
  * (waves, sched, chunk, ms per step); *ncand = number of trials. */
 int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* report, int* ncand);
+
+/* ------------------------------------------------------------------------
+ * Request server (no counterpart in the reference): runs arrive at any time
+ * and are admitted first-come-first-served while their device memory fits
+ * the HBM budget; their state is built on the device (as
+ * mhd3d_init_stratified) and kept there; each tick advances every admitted
+ * run by up to `quantum_steps` steps in one ensemble (continuous batching:
+ * runs join and leave between ticks).  A run's result is the same as the
+ * same run advanced alone.  Times are seconds since the server's creation.
+ * ------------------------------------------------------------------------ */
+typedef struct mhd3d_server mhd3d_server;
+
+enum {
+    MHD3D_REQ_FAILED = -1,   /* halted by the reference's DT guard (every
+                                run is admitted with MHD3D_REFERENCE_DT_MIN) */
+    MHD3D_REQ_QUEUED = 0,
+    MHD3D_REQ_RUNNING = 1,
+    MHD3D_REQ_DONE = 2,      /* state on the device until mhd3d_server_result */
+    MHD3D_REQ_RELEASED = 3
+};
+
+typedef struct mhd3d_request_info {
+    int32_t status;
+    int32_t steps_done;
+    double t_submit, t_admit, t_finish;   /* -1 while not reached */
+    double dt_last;
+} mhd3d_request_info;
+
+int mhd3d_server_create(int precision, int64_t hbm_budget_bytes, int quantum_steps,
+                        mhd3d_server** out);
+int mhd3d_server_destroy(mhd3d_server* s);
+const char* mhd3d_server_last_error(const mhd3d_server* s);
+double mhd3d_server_now(const mhd3d_server* s);
+int64_t mhd3d_server_bytes_in_use(const mhd3d_server* s);
+int64_t mhd3d_server_cell_steps(const mhd3d_server* s);
+/* desc and its arrays are copied; the initial state is that of
+ * mhd3d_init_stratified(TT_z, RO_z, seed, amp, b0, b_amp) */
+int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, const double* TT_z,
+                        const double* RO_z, uint64_t seed, double amp, double b0, double b_amp,
+                        int nsteps, int64_t* id);
+int mhd3d_server_tick(mhd3d_server* s, int* n_active, int* n_queued);
+int mhd3d_server_poll(mhd3d_server* s, int64_t id, mhd3d_request_info* info);
+/* copies the final state of a finished run (fields may be NULL) and releases
+ * its device memory */
+int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* fields);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,7 @@ struct GAcc {
 };
 
 struct CflPart;
+struct DevScalars;
 
 // Background profiles of the mixed form (index k * nz + z).
 // FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
@@ -118,6 +119,8 @@ struct StageArgs {
     double* __restrict__ Uout;
     double* __restrict__ Z;
     const double* __restrict__ dt;
+    // the domain's scalars (DT guard flag); null: no guard
+    const DevScalars* ds;
     // mixed form only: the FP32 copies of the stage values that stages 2 and
     // 3 take their stencils from (Uin32) and that stages 1 and 2 write
     // (Uout32); Uin (read by stage 1) and Uout (written by stage 3) are then
@@ -135,6 +138,9 @@ struct StageArgs {
     CflPart* cfl_parts;
     int cfl_base;
 };
+
+// Forward: whether the DT guard has stopped the domain (defined below).
+__device__ __forceinline__ bool domain_halted(const StageArgs& a);
 
 // Background subtracted from field f of plane z when the FP32 copy is written.
 __device__ __forceinline__ double bg_sub(const StageArgs& a, int f, int z)
@@ -277,6 +283,7 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z + 2;   // planes 2 .. nPz-1
+    if (domain_halted(a)) return;
     if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz - 1) return;
     GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
@@ -405,6 +412,7 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const bool top = blockIdx.z == 0;
+    if (domain_halted(a)) return;
     // every thread reaches the block reduction below; threads outside the
     // domain only contribute the identity
     CflAcc cfl;
@@ -513,8 +521,20 @@ struct DevScalars {
     double umach;
     double wmin[4];
     unsigned int nonfinite;
-    unsigned int pad;
+    unsigned int halted;   // the DT guard stopped this domain
+    double dt_min;         // DT guard threshold (0: no guard)
 };
+
+// The DT guard of the reference's driver (3dmhd.cpp:292): after a step whose
+// DT fell below dt_min the run halts -- here every later kernel of the
+// domain returns at once, leaving the state of that step.
+__device__ __forceinline__ void clock_and_guard(DevScalars* s)
+{
+    s->timc += s->dt;
+    s->nit += 1.0;
+    s->dt_last = s->dt;
+    if (s->dt_min > 0.0 && s->dt < s->dt_min) s->halted = 1u;
+}
 
 // Reduces the per-block partials in a fixed order (one block of 256 threads:
 // strided per-thread reduction, then the block reduction) and forms
@@ -559,17 +579,27 @@ __global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __rest
                                                            int nparts, DevScalars* s, Phys c,
                                                            int mag)
 {
+    if (s->halted) return;
     cfl_finalize_block(part, nparts, s, c, mag);
 }
+
+__device__ __forceinline__ bool domain_halted(const StageArgs& a) { return a.ds && a.ds->halted; }
 
 __global__ void set_dt_kernel(DevScalars* s, double dt) { s->dt = dt; }
 
 __global__ void clock_kernel(DevScalars* s)
 {
-    s->timc += s->dt;
-    s->nit += 1.0;
-    s->dt_last = s->dt;
+    if (s->halted) return;
+    clock_and_guard(s);
 }
+
+__global__ void set_guard_kernel(DevScalars* s, double dt_min)
+{
+    s->dt_min = dt_min;
+    s->halted = 0u;
+}
+
+__global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
 
 // Non-finite scan of a register (interior incl. walls): one thread per
 // (x, y) column, looping over z.
@@ -592,6 +622,50 @@ __global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,
 }
 
 __global__ void clear_flag_kernel(DevScalars* s) { s->nonfinite = 0u; }
+
+// Counter-based uniform deviate in [-1, 1) for (seed, field, z, y, x):
+// the splitmix64 finaliser of the packed key (x, y < 2^18, z < 2^20).
+__host__ __device__ inline double hash_uniform(unsigned long long seed, int f, int z, int y, int x)
+{
+    unsigned long long k = seed * 0x9E3779B97F4A7C15ull ^ ((unsigned long long)f << 56) ^
+                           ((unsigned long long)z << 36) ^ ((unsigned long long)y << 18) ^
+                           (unsigned long long)x;
+    k += 0x9E3779B97F4A7C15ull;
+    k = (k ^ (k >> 30)) * 0xBF58476D1CE4E5B9ull;
+    k = (k ^ (k >> 27)) * 0x94D049BB133111EBull;
+    k ^= k >> 31;
+    return (double)(k >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Synthetic stratified state on the device: RO, TT from the z profiles;
+// RU, RV, RW = RO * amp * u on the planes between the walls; with lMag
+// BX = b0 + b_amp*u, BY, BZ = b_amp*u on planes 1..nPz (BZ = 0 on the walls),
+// ghost planes copies of the walls.  u = hash_uniform(seed, field, z, y, x).
+__global__ void __launch_bounds__(256) init_stratified_kernel(Geom g, int nf, double* __restrict__ U,
+                                                              const double* __restrict__ Tz,
+                                                              const double* __restrict__ Rz,
+                                                              unsigned long long seed, double amp,
+                                                              double b0, double b_amp)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.nPx || y >= g.nPy) return;
+    const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
+    const double ro = Rz[z];
+    U[3 * g.fs + i] = ro;
+    U[4 * g.fs + i] = Tz[z];
+    const bool inner = z >= 2 && z <= g.nPz - 1;
+    for (int f = 0; f < 3; ++f)
+        U[f * g.fs + i] = inner ? ro * amp * hash_uniform(seed, f, z, y, x) : 0.0;
+    if (nf == NF_MAG) {
+        const int zz = z == 0 ? 1 : (z == g.nz - 1 ? g.nPz : z);
+        const bool wall = zz == 1 || zz == g.nPz;
+        U[5 * g.fs + i] = b0 + b_amp * hash_uniform(seed, 5, zz, y, x);
+        U[6 * g.fs + i] = b_amp * hash_uniform(seed, 6, zz, y, x);
+        U[7 * g.fs + i] = wall ? 0.0 : b_amp * hash_uniform(seed, 7, zz, y, x);
+    }
+}
 
 // FP32 copy of a register in the mixed form's deviation layout:
 // dst = (float)(src - background) for RO, TT, BX, BY, BZ, (float)src otherwise.

@@ -30,12 +30,15 @@ struct WallItem {
     int member, flags, x0, y0, top, part, pad0, pad1;
 };
 
-// Per-member closing data of a step.
+// Per-member closing data of a step: the member's partial records and
+// scalars, and the slot of the ensemble's contiguous copy of the scalars
+// (one readback for all members).
 struct FinishArgs {
     CflPart* parts;
     int nparts;
     int mag;
     DevScalars* ds;
+    DevScalars* mirror;
     Phys c;
 };
 
@@ -69,6 +72,7 @@ __global__ void __launch_bounds__(256, WAVES)
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const WorkItem w = items[it];
         const StageArgs& a = args[w.member];
+        if (domain_halted(a)) continue;   // uniform over the block
         CflAcc cfl;
         ens_item<T, STAGE, ONE_BARRIER, DEFER>(a, w, lds, cfl);
         if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, w.part);
@@ -85,6 +89,7 @@ __global__ void __launch_bounds__(256)
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const WallItem w = items[it];
         const StageArgs& a = args[w.member];
+        if (domain_halted(a)) continue;   // uniform over the block
         const int x = w.x0 + threadIdx.x, y = w.y0 + threadIdx.y;
         CflAcc cfl;
         if (x < a.g.nPx && y < a.g.nPy) {
@@ -101,24 +106,30 @@ __global__ void __launch_bounds__(256)
 }
 
 // Closes the step of every member (block m = member m): TIMC += DT,
-// nIt += 1, then the next DT from the member's TSTEP partials.
+// nIt += 1 and the DT guard, then the next DT from the member's TSTEP
+// partials.  A halted member keeps its scalars.
 __global__ void __launch_bounds__(256) ensemble_finish_kernel(const FinishArgs* __restrict__ f)
 {
     const FinishArgs& m = f[blockIdx.x];
-    if (threadIdx.x == 0) {
-        m.ds->timc += m.ds->dt;
-        m.ds->nit += 1.0;
-        m.ds->dt_last = m.ds->dt;
+    if (m.ds->halted) {
+        if (threadIdx.x == 0) *m.mirror = *m.ds;
+        return;
     }
+    __syncthreads();   // every thread has read the flag before thread 0 may set it
+    if (threadIdx.x == 0) clock_and_guard(m.ds);
     __syncthreads();
     cfl_finalize_block(m.parts, m.nparts, m.ds, m.c, m.mag);
+    if (threadIdx.x == 0) *m.mirror = *m.ds;   // thread 0 wrote the scalars
 }
 
 // Non-finite flags of all members, cleared before a call's steps.
 __global__ void ensemble_clear_kernel(const FinishArgs* __restrict__ f, int n)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < n) f[m].ds->nonfinite = 0u;
+    if (m < n) {
+        f[m].ds->nonfinite = 0u;
+        *f[m].mirror = *f[m].ds;
+    }
 }
 
 }  // namespace mhd

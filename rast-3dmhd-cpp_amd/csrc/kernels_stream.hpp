@@ -441,6 +441,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
                                                                   int ntx, int nty, int xcd_remap)
 {
     __shared__ __align__(16) char lds[StreamLds<T, MAG, ONE_BARRIER>::bytes];
+    if (domain_halted(a)) return;
     const int nb = gridDim.x;
     const int bid = blockIdx.x;
     const int logical = xcd_remap ? (bid % 8) * (nb / 8) + bid / 8 : bid;

@@ -310,6 +310,8 @@ static int download_field(mhd3d_ctx* ctx, double* dst, const double* src)
 // the wall planes, the one-sided forms of 3dmhdsub.h:638-690); HB is the
 // hydrostatic residual RO0*GRAV - D1z(P0).  The background stays fixed until
 // the next set_state.
+static int set_background_profiles(mhd3d_ctx* ctx, const std::vector<double>* mean);
+
 static int set_background(mhd3d_ctx* ctx, const double* const* fields)
 {
     const Geom& g = ctx->g;
@@ -329,6 +331,14 @@ static int set_background(mhd3d_ctx* ctx, const double* const* fields)
             mean[k][z] = sum / ((double)g.nPx * g.nPy);
         }
     }
+    return set_background_profiles(ctx, mean);
+}
+
+// Background from given z profiles mean[k][z] (k = RO, TT, BX, BY, BZ).
+static int set_background_profiles(mhd3d_ctx* ctx, const std::vector<double>* mean)
+{
+    const Geom& g = ctx->g;
+    const int nz = g.nz;
     const std::vector<double>& R0 = mean[0];
     const std::vector<double>& T0 = mean[1];
     std::vector<float> bf((size_t)NBGF * nz, 0.0f);
@@ -404,6 +414,55 @@ extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
         HIPCHK(ctx, hipMemcpyAsync(ctx->U[1 - ctx->cur], ctx->U[ctx->cur], (size_t)n * sizeof(double),
                                    hipMemcpyDeviceToDevice, ctx->stream));
     }
+    clear_halt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);   // a new state runs again
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}
+
+// Device-side synthetic stratified state (init_stratified_kernel); the mixed
+// form takes its background from the same profiles (B0 = (b0, 0, 0)).
+extern "C" int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const double* RO_z,
+                                     uint64_t seed, double amp, double b0, double b_amp)
+{
+    if (!ctx || !TT_z || !RO_z) return MHD3D_EINVAL;
+    const Geom& g = ctx->g;
+    if (g.nPx >= (1 << 18) || g.nPy >= (1 << 18)) return fail(ctx, MHD3D_EINVAL, "init_stratified: nPx, nPy < 262144");
+    ctx->dt_ready = false;
+    const int nz = g.nz;
+    double* prof = nullptr;
+    HIPCHK(ctx, hipMalloc(&prof, sizeof(double) * 2 * nz));
+    hipError_t e = hipMemcpyAsync(prof, TT_z, sizeof(double) * nz, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(prof + nz, RO_z, sizeof(double) * nz, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        dim3 blk(64, 4, 1);
+        dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, nz);
+        init_stratified_kernel<<<grd, blk, 0, ctx->stream>>>(g, ctx->nf, ctx->U[ctx->cur], prof, prof + nz,
+                                                             (unsigned long long)seed, amp, b0, b_amp);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(prof);
+    if (e != hipSuccess) return fail(ctx, MHD3D_EHIP, std::string("init_stratified: ") + hipGetErrorString(e));
+    const int64_t n = g.fs * ctx->nf;
+    if (ctx->mixed) {
+        std::vector<double> mean[5];
+        mean[0].assign(RO_z, RO_z + nz);
+        mean[1].assign(TT_z, TT_z + nz);
+        mean[2].assign(nz, ctx->nf == NF_MAG ? b0 : 0.0);
+        mean[3].assign(nz, 0.0);
+        mean[4].assign(nz, 0.0);
+        int rc = set_background_profiles(ctx, mean);
+        if (rc) return rc;
+        for (int k = 0; k < 2; ++k)
+            to_f32_dev_kernel<<<1024, 256, 0, ctx->stream>>>(ctx->U[0], ctx->U32[k], ctx->g, ctx->nf, ctx->bg64);
+        HIPCHK(ctx, hipGetLastError());
+    } else {
+        HIPCHK(ctx, hipMemcpyAsync(ctx->U[1 - ctx->cur], ctx->U[ctx->cur], (size_t)n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    clear_halt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);   // a new state runs again
+    HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
 }
@@ -433,6 +492,7 @@ static StageArgs stage_args(mhd3d_ctx* ctx, int k = 0)
     a.Uout = ctx->mixed ? ctx->U[0] : ctx->U[1 - ctx->cur];
     a.Z = ctx->Z;
     a.dt = &ctx->ds->dt;
+    a.ds = ctx->ds;
     a.Uin32 = nullptr;
     a.Uout32 = nullptr;
     a.bgf = ctx->bgf;
@@ -645,6 +705,7 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
     int rc = collect_timing(ctx);
     if (rc) return rc;
     if (dt_inout && nsteps > 0) *dt_inout = hs.dt_last;
+    if (hs.halted) return fail(ctx, MHD3D_EHALTED, "DT too small, halt (DT guard)");
     if (hs.nonfinite) return fail(ctx, MHD3D_ENONFINITE, "state is not finite after the step");
     return MHD3D_OK;
 }
@@ -790,15 +851,26 @@ extern "C" int mhd3d_synchronize(mhd3d_ctx* ctx)
     return MHD3D_OK;
 }
 
+extern "C" int mhd3d_set_dt_guard(mhd3d_ctx* ctx, double dt_min)
+{
+    if (!ctx || !(dt_min >= 0.0)) return MHD3D_EINVAL;
+    set_guard_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt_min);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MHD3D_OK;
+}
+
 // ===========================================================================
 // Ensembles: independent domains advanced together (kernels_ensemble.hpp).
 
 struct mhd3d_ensemble {
-    std::vector<mhd3d_ctx*> mem;
+    std::vector<mhd3d_ctx*> mem;       // members (their scalars: mem[m]->ds)
+    bool owns = true;                  // destroy the members with the ensemble
     int n = 0;
     bool mixed = false;
     hipStream_t stream = nullptr;
-    DevScalars* ds = nullptr;          // one per member
+    DevScalars* mirror = nullptr;      // contiguous copy of the members' scalars
+    int mirror_cap = 0;
     StageArgs* d_args = nullptr;       // [parity at step start][stage][member]
     WorkItem* d_items = nullptr;
     WallItem* d_walls = nullptr;
@@ -807,10 +879,10 @@ struct mhd3d_ensemble {
     int grid_int = 0, grid_wall = 0;
     int waves = 1, sched = 0, chunk = 0;   // kernel variant; chunk: planes per item (0 = auto)
     int parity = 0;                    // FP64: U buffer holding every member's state
-    bool dt_ready = false;
     bool use_graph = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int64_t cells = 0;
+    std::vector<DevScalars> last;      // members' scalars after the last step call
     std::string err;
 };
 
@@ -902,6 +974,13 @@ static void ens_drop_graphs(mhd3d_ensemble* e)
 static int ens_build(mhd3d_ensemble* e)
 {
     ens_drop_graphs(e);
+    if (e->n > e->mirror_cap) {
+        (void)hipFree(e->mirror);
+        e->mirror = nullptr;
+        e->mirror_cap = 0;
+        ENSCHK(e, hipMalloc(&e->mirror, sizeof(DevScalars) * e->n));
+        e->mirror_cap = e->n;
+    }
     std::vector<WorkItem> items;
     std::vector<WallItem> walls;
     std::vector<FinishArgs> fin(e->n);
@@ -942,7 +1021,8 @@ static int ens_build(mhd3d_ensemble* e)
         fin[m].parts = c->cfl_parts;
         fin[m].nparts = part;
         fin[m].mag = c->d.lMag ? 1 : 0;
-        fin[m].ds = e->ds + m;
+        fin[m].ds = c->ds;
+        fin[m].mirror = e->mirror + m;
         fin[m].c = c->c;
     }
     // largest items first (a z chunk of an MHD member costs about twice a hydro one)
@@ -961,7 +1041,8 @@ static int ens_build(mhd3d_ensemble* e)
                 c->cur = e->mixed ? 0 : (p0 ^ ((k - 1) & 1));
                 StageArgs a = stage_args(c, k);
                 c->cur = save;
-                a.dt = &e->ds[m].dt;
+                a.dt = &c->ds->dt;
+                a.ds = c->ds;
                 a.cfl_parts = k == 3 ? c->cfl_parts : nullptr;
                 a.cfl_base = 0;
                 args[((size_t)p0 * 3 + (k - 1)) * e->n + m] = a;
@@ -1012,10 +1093,6 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
     }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(MHD3D_EHIP, "hipStreamCreate failed");
-    if (hipMalloc(&e->ds, sizeof(DevScalars) * n) != hipSuccess)
-        return bail(MHD3D_ENOMEM, "hipMalloc(scalars) failed");
-    if (hipMemset(e->ds, 0, sizeof(DevScalars) * n) != hipSuccess)
-        return bail(MHD3D_EHIP, "hipMemset failed");
     e->waves = e->mixed ? 2 : 1;
     e->sched = e->mixed ? 2 : 0;
     const char* tv = std::getenv(e->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
@@ -1036,8 +1113,9 @@ extern "C" int mhd3d_ensemble_destroy(mhd3d_ensemble* e)
     if (!e) return MHD3D_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     ens_drop_graphs(e);
-    for (auto* c : e->mem) mhd3d_destroy(c);
-    (void)hipFree(e->ds);
+    if (e->owns)
+        for (auto* c : e->mem) mhd3d_destroy(c);
+    (void)hipFree(e->mirror);
     (void)hipFree(e->d_items);
     (void)hipFree(e->d_walls);
     (void)hipFree(e->d_fin);
@@ -1062,10 +1140,10 @@ extern "C" int mhd3d_ensemble_set_state(mhd3d_ensemble* e, int member, const dou
     c->cur = e->mixed ? 0 : e->parity;
     int rc = mhd3d_set_state(c, fields);
     if (rc) return ens_fail(e, rc, c->err);
-    // a fresh state needs its TSTEP; reset the member's clock
-    e->dt_ready = false;
+    // a fresh state needs its TSTEP (set_state cleared the member's
+    // dt_ready); reset the member's clock
     DevScalars z{};
-    ENSCHK(e, hipMemcpy(e->ds + member, &z, sizeof(z), hipMemcpyHostToDevice));
+    ENSCHK(e, hipMemcpy(c->ds, &z, sizeof(z), hipMemcpyHostToDevice));
     return MHD3D_OK;
 }
 
@@ -1086,6 +1164,7 @@ static int ens_cfl_all(mhd3d_ensemble* e)
 {
     for (int m = 0; m < e->n; ++m) {
         mhd3d_ctx* c = e->mem[m];
+        if (c->dt_ready) continue;
         const Geom& g = c->g;
         dim3 blk(64, 4, 1);
         const int zchunks = g.nPz < 8 ? g.nPz : 8;
@@ -1096,7 +1175,7 @@ static int ens_cfl_all(mhd3d_ensemble* e)
             cfl_kernel<true><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
         else
             cfl_kernel<false><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
-        cfl_finalize_kernel<<<1, 256, 0, e->stream>>>(c->cfl_parts, nparts, e->ds + m, c->c,
+        cfl_finalize_kernel<<<1, 256, 0, e->stream>>>(c->cfl_parts, nparts, c->ds, c->c,
                                                       c->d.lMag ? 1 : 0);
     }
     ENSCHK(e, hipGetLastError());
@@ -1116,10 +1195,9 @@ static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
 extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
 {
     if (!e || nsteps < 0) return MHD3D_EINVAL;
-    if (!e->dt_ready) {
-        int rc = ens_cfl_all(e);
+    {
+        int rc = ens_cfl_all(e);   // members without a TSTEP of their current state
         if (rc) return rc;
-        e->dt_ready = true;
     }
     ensemble_clear_kernel<<<(e->n + 255) / 256, 256, 0, e->stream>>>(e->d_fin, e->n);
     for (int s = 0; s < nsteps; ++s) {
@@ -1141,14 +1219,16 @@ extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
         if (!e->mixed) e->parity ^= 1;
     }
     ENSCHK(e, hipGetLastError());
-    std::vector<DevScalars> hs(e->n);
-    ENSCHK(e, hipMemcpyAsync(hs.data(), e->ds, sizeof(DevScalars) * e->n, hipMemcpyDeviceToHost, e->stream));
+    e->last.resize(e->n);
+    ENSCHK(e, hipMemcpyAsync(e->last.data(), e->mirror, sizeof(DevScalars) * e->n, hipMemcpyDeviceToHost, e->stream));
     ENSCHK(e, hipStreamSynchronize(e->stream));
     for (int m = 0; m < e->n; ++m) {
         e->mem[m]->cur = e->mixed ? 0 : e->parity;
-        if (hs[m].nonfinite)
-            return ens_fail(e, MHD3D_ENONFINITE, "member " + std::to_string(m) + ": state is not finite after the step");
+        e->mem[m]->dt_ready = true;
     }
+    for (int m = 0; m < e->n; ++m)
+        if (e->last[m].nonfinite)
+            return ens_fail(e, MHD3D_ENONFINITE, "member " + std::to_string(m) + ": state is not finite after the step");
     return MHD3D_OK;
 }
 
@@ -1157,7 +1237,7 @@ extern "C" int mhd3d_ensemble_clock(mhd3d_ensemble* e, int member, int64_t* nIt,
 {
     if (!e || member < 0 || member >= e->n) return MHD3D_EINVAL;
     DevScalars hs;
-    ENSCHK(e, hipMemcpyAsync(&hs, e->ds + member, sizeof(hs), hipMemcpyDeviceToHost, e->stream));
+    ENSCHK(e, hipMemcpyAsync(&hs, e->mem[member]->ds, sizeof(hs), hipMemcpyDeviceToHost, e->stream));
     ENSCHK(e, hipStreamSynchronize(e->stream));
     if (nIt) *nIt = (int64_t)hs.nit;
     if (timc) *timc = hs.timc;
@@ -1195,6 +1275,28 @@ extern "C" int mhd3d_ensemble_get_variant(const mhd3d_ensemble* e, int* waves, i
 
 extern "C" int64_t mhd3d_ensemble_cells(const mhd3d_ensemble* e) { return e ? e->cells : 0; }
 
+extern "C" int mhd3d_ensemble_set_dt_guard(mhd3d_ensemble* e, double dt_min)
+{
+    if (!e || !(dt_min >= 0.0)) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    for (auto* c : e->mem) {
+        int rc = mhd3d_set_dt_guard(c, dt_min);
+        if (rc) return ens_fail(e, rc, c->err);
+    }
+    return MHD3D_OK;
+}
+
+// Whether the DT guard stopped member `member` (after its last step call).
+extern "C" int mhd3d_ensemble_halted(mhd3d_ensemble* e, int member, int* halted)
+{
+    if (!e || !halted || member < 0 || member >= e->n) return MHD3D_EINVAL;
+    DevScalars hs;
+    ENSCHK(e, hipMemcpyAsync(&hs, e->mem[member]->ds, sizeof(hs), hipMemcpyDeviceToHost, e->stream));
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    *halted = hs.halted ? 1 : 0;
+    return MHD3D_OK;
+}
+
 extern "C" int mhd3d_ensemble_synchronize(mhd3d_ensemble* e)
 {
     if (!e) return MHD3D_EINVAL;
@@ -1215,7 +1317,8 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
     // snapshot: every member's state register, and the device scalars
     std::vector<double*> snap(e->n, nullptr);
     const int p_save = e->parity;
-    const bool ready_save = e->dt_ready;
+    std::vector<char> ready_save(e->n);
+    for (int m = 0; m < e->n; ++m) ready_save[m] = e->mem[m]->dt_ready;
     DevScalars* ds_snap = nullptr;
     auto cleanup = [&]() {
         for (double* p : snap) (void)hipFree(p);
@@ -1231,7 +1334,8 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
         }
     }
     if (hipMalloc(&ds_snap, sizeof(DevScalars) * e->n) != hipSuccess) { cleanup(); return ens_fail(e, MHD3D_ENOMEM, "autotune snapshot"); }
-    (void)hipMemcpyAsync(ds_snap, e->ds, sizeof(DevScalars) * e->n, hipMemcpyDeviceToDevice, e->stream);
+    for (int m = 0; m < e->n; ++m)
+        (void)hipMemcpyAsync(ds_snap + m, e->mem[m]->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, e->stream);
     auto restore = [&]() -> int {
         e->parity = p_save;
         for (int m = 0; m < e->n; ++m) {
@@ -1240,9 +1344,10 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
             ENSCHK(e, hipMemcpyAsync(c->U[e->mixed ? 0 : e->parity], snap[m], reg, hipMemcpyDeviceToDevice, e->stream));
             c->cur = e->mixed ? 0 : e->parity;
         }
-        ENSCHK(e, hipMemcpyAsync(e->ds, ds_snap, sizeof(DevScalars) * e->n, hipMemcpyDeviceToDevice, e->stream));
+        for (int m = 0; m < e->n; ++m)
+            ENSCHK(e, hipMemcpyAsync(e->mem[m]->ds, ds_snap + m, sizeof(DevScalars), hipMemcpyDeviceToDevice, e->stream));
         ENSCHK(e, hipStreamSynchronize(e->stream));
-        e->dt_ready = ready_save;
+        for (int m = 0; m < e->n; ++m) e->mem[m]->dt_ready = ready_save[m] != 0;
         return MHD3D_OK;
     };
     const int wv[2] = {1, 2};
@@ -1299,4 +1404,284 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
     cleanup();
     if (ncand_out) *ncand_out = ncand;
     return rc ? rc : rb;
+}
+
+// ===========================================================================
+// Request server: runs (grid, flags, parameters, step count) arrive at any
+// time; they are admitted first-come-first-served while their device memory
+// fits the HBM budget, their state is built and kept on the device, and every
+// tick advances all admitted runs together by one quantum of steps through an
+// ensemble over them (continuous batching: the ensemble is rebuilt whenever
+// runs join or leave).  A run advances exactly as it would alone.
+
+#include <chrono>
+#include <deque>
+#include <unordered_map>
+
+struct ServerReq {
+    int64_t id = 0;
+    mhd3d_desc d{};
+    std::vector<double> met;          // the descriptor's metric arrays
+    std::vector<double> Tz, Rz;       // initial z profiles
+    uint64_t seed = 0;
+    double amp = 0, b0 = 0, b_amp = 0;
+    int nsteps = 0, done = 0;
+    int status = MHD3D_REQ_QUEUED;
+    mhd3d_ctx* ctx = nullptr;
+    int64_t bytes = 0;
+    double t_submit = 0, t_admit = -1, t_finish = -1, dt_last = 0;
+};
+
+struct mhd3d_server {
+    int precision = 0;
+    int64_t budget = 0, used = 0;
+    int quantum = 10;
+    mhd3d_ensemble* ens = nullptr;    // view over the admitted runs (owns = false)
+    std::unordered_map<int64_t, ServerReq> reqs;
+    std::deque<int64_t> queue;
+    std::vector<int64_t> active;
+    bool dirty = false;
+    int64_t next_id = 1;
+    int64_t cell_steps = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::string err;
+};
+
+static std::string g_srv_create_error;
+
+static double srv_now(const mhd3d_server* s)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+// Device bytes a context for this descriptor allocates (mhd3d_create, plus
+// the TSTEP partial records of the fused and standalone passes).
+static int64_t ctx_bytes_estimate(const mhd3d_desc* d)
+{
+    const int nf = d->lMag ? NF_MAG : NF_HYDRO;
+    const int64_t pitch = (d->nPx + 7) / 8 * 8;
+    const int64_t plane = pitch * d->nPy;
+    const int64_t fs = (plane * (d->nPz + 2) + 31) / 32 * 32;
+    const int64_t reg = fs * nf * 8;
+    const bool mixed = d->precision == MHD3D_MIXED;
+    const int64_t nmet = 3 * (int64_t)(d->nPx + 2) + 3 * (int64_t)(d->nPy + 2) + 5 * (int64_t)(d->nPz + 2);
+    const int64_t tiles = (int64_t)((d->nPx + STX - 1) / STX) * ((d->nPy + STY - 1) / STY);
+    const int64_t parts = tiles * (d->nPz / 8 + 1) + (int64_t)((d->nPx + 63) / 64) * ((d->nPy + 3) / 4) * 10;
+    return (mixed ? 2 : 3) * reg + (mixed ? reg : 0) + nmet * 8 + (int64_t)sizeof(DevScalars) +
+           parts * (int64_t)sizeof(CflPart) + (mixed ? (int64_t)(NBGF * 4 + NBG64 * 8) * (d->nPz + 2) : 0);
+}
+
+extern "C" int mhd3d_server_create(int precision, int64_t hbm_budget_bytes, int quantum_steps,
+                                   mhd3d_server** out)
+{
+    if (!out || hbm_budget_bytes <= 0 || quantum_steps < 1 ||
+        (precision != MHD3D_FP64 && precision != MHD3D_MIXED))
+        return MHD3D_EINVAL;
+    mhd3d_server* s = new mhd3d_server();
+    s->precision = precision;
+    s->budget = hbm_budget_bytes;
+    s->quantum = quantum_steps;
+    s->t0 = std::chrono::steady_clock::now();
+    s->ens = new mhd3d_ensemble();
+    s->ens->owns = false;
+    s->ens->mixed = precision == MHD3D_MIXED;
+    s->ens->waves = s->ens->mixed ? 2 : 1;
+    s->ens->sched = s->ens->mixed ? 2 : 0;
+    if (hipStreamCreateWithFlags(&s->ens->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_srv_create_error = "hipStreamCreate failed";
+        mhd3d_server_destroy(s);
+        return MHD3D_EHIP;
+    }
+    *out = s;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_server_destroy(mhd3d_server* s)
+{
+    if (!s) return MHD3D_OK;
+    if (s->ens) mhd3d_ensemble_destroy(s->ens);   // a view: leaves the runs
+    for (auto& kv : s->reqs)
+        if (kv.second.ctx) mhd3d_destroy(kv.second.ctx);
+    delete s;
+    return MHD3D_OK;
+}
+
+extern "C" const char* mhd3d_server_last_error(const mhd3d_server* s)
+{
+    return s ? s->err.c_str() : g_srv_create_error.c_str();
+}
+
+extern "C" double mhd3d_server_now(const mhd3d_server* s) { return s ? srv_now(s) : 0.0; }
+
+extern "C" int64_t mhd3d_server_bytes_in_use(const mhd3d_server* s) { return s ? s->used : 0; }
+
+extern "C" int64_t mhd3d_server_cell_steps(const mhd3d_server* s) { return s ? s->cell_steps : 0; }
+
+extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, const double* TT_z,
+                                   const double* RO_z, uint64_t seed, double amp, double b0,
+                                   double b_amp, int nsteps, int64_t* id)
+{
+    if (!s || !desc || !TT_z || !RO_z || nsteps < 1 || !id) return MHD3D_EINVAL;
+    std::string why;
+    int rc = validate(desc, why);
+    if (rc) { s->err = why; return rc; }
+    if (desc->precision != s->precision) { s->err = "request precision differs from the server's"; return MHD3D_EINVAL; }
+    ServerReq r;
+    r.id = s->next_id++;
+    r.d = *desc;
+    const int nx = desc->nPx + 2, ny = desc->nPy + 2, nz = desc->nPz + 2;
+    const double* src[11] = {desc->dxxdx, desc->d2xxdx2, desc->DDX, desc->dyydy, desc->d2yydy2, desc->DDY,
+                             desc->dzzdz, desc->d2zzdz2, desc->DDZ, desc->RKAPA, desc->DKAPA};
+    const int len[11] = {nx, nx, nx, ny, ny, ny, nz, nz, nz, nz, nz};
+    size_t tot = 0;
+    for (int k = 0; k < 11; ++k) tot += len[k];
+    r.met.resize(tot);
+    size_t o = 0;
+    const double** dst[11] = {&r.d.dxxdx, &r.d.d2xxdx2, &r.d.DDX, &r.d.dyydy, &r.d.d2yydy2, &r.d.DDY,
+                              &r.d.dzzdz, &r.d.d2zzdz2, &r.d.DDZ, &r.d.RKAPA, &r.d.DKAPA};
+    std::vector<size_t> offs(11);
+    for (int k = 0; k < 11; ++k) {
+        std::memcpy(&r.met[o], src[k], sizeof(double) * len[k]);
+        offs[k] = o;
+        o += len[k];
+    }
+    r.Tz.assign(TT_z, TT_z + nz);
+    r.Rz.assign(RO_z, RO_z + nz);
+    r.seed = seed;
+    r.amp = amp;
+    r.b0 = b0;
+    r.b_amp = b_amp;
+    r.nsteps = nsteps;
+    r.bytes = ctx_bytes_estimate(desc);
+    if (r.bytes > s->budget) { s->err = "request alone exceeds the HBM budget"; return MHD3D_ENOMEM; }
+    r.t_submit = srv_now(s);
+    auto& slot = s->reqs[r.id];
+    slot = std::move(r);
+    for (int k = 0; k < 11; ++k) *dst[k] = nullptr;
+    // point the stored descriptor at the stored metric copies
+    const double** sd[11] = {&slot.d.dxxdx, &slot.d.d2xxdx2, &slot.d.DDX, &slot.d.dyydy, &slot.d.d2yydy2,
+                             &slot.d.DDY, &slot.d.dzzdz, &slot.d.d2zzdz2, &slot.d.DDZ, &slot.d.RKAPA,
+                             &slot.d.DKAPA};
+    for (int k = 0; k < 11; ++k) *sd[k] = slot.met.data() + offs[k];
+    s->queue.push_back(slot.id);
+    *id = slot.id;
+    return MHD3D_OK;
+}
+
+// Admits queued runs in arrival order while they fit the budget.
+static int srv_admit(mhd3d_server* s)
+{
+    while (!s->queue.empty()) {
+        ServerReq& r = s->reqs[s->queue.front()];
+        if (s->used + r.bytes > s->budget) break;
+        mhd3d_ctx* c = nullptr;
+        int rc = mhd3d_create(&r.d, &c);
+        if (rc) { s->err = g_create_error; return rc; }
+        rc = mhd3d_init_stratified(c, r.Tz.data(), r.Rz.data(), r.seed, r.amp, r.b0, r.b_amp);
+        if (!rc) rc = mhd3d_set_dt_guard(c, MHD3D_REFERENCE_DT_MIN);
+        if (rc) { s->err = c->err; mhd3d_destroy(c); return rc; }
+        r.ctx = c;
+        r.status = MHD3D_REQ_RUNNING;
+        r.t_admit = srv_now(s);
+        s->used += r.bytes;
+        s->active.push_back(r.id);
+        s->queue.pop_front();
+        s->dirty = true;
+    }
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_server_tick(mhd3d_server* s, int* n_active, int* n_queued)
+{
+    if (!s) return MHD3D_EINVAL;
+    int rc = srv_admit(s);
+    if (rc) return rc;
+    if (!s->active.empty()) {
+        mhd3d_ensemble* e = s->ens;
+        if (s->dirty) {
+            e->mem.clear();
+            e->cells = 0;
+            for (int64_t id : s->active) {
+                mhd3d_ctx* c = s->reqs[id].ctx;
+                c->cur = e->mixed ? 0 : e->parity;   // members share the ensemble's buffer parity
+                e->mem.push_back(c);
+                e->cells += (int64_t)c->g.nPx * c->g.nPy * c->g.nPz;
+            }
+            e->n = (int)e->mem.size();
+            rc = ens_build(e);
+            if (rc) { s->err = e->err; return rc; }
+            s->dirty = false;
+        }
+        int k = s->quantum;
+        for (int64_t id : s->active) {
+            const ServerReq& r = s->reqs[id];
+            if (r.nsteps - r.done < k) k = r.nsteps - r.done;
+        }
+        rc = mhd3d_ensemble_step(e, k);
+        const double now = srv_now(s);
+        if (rc == MHD3D_ENONFINITE) rc = MHD3D_OK;   // as the reference's driver, keep stepping
+        std::vector<int64_t> keep;
+        for (int m = 0; m < (int)s->active.size(); ++m) {
+            ServerReq& r = s->reqs[s->active[m]];
+            const DevScalars& hs = e->last[m];
+            r.dt_last = hs.dt_last;
+            const int now_done = (int)hs.nit;
+            s->cell_steps += (int64_t)(now_done - r.done) * r.ctx->g.nPx * r.ctx->g.nPy * r.ctx->g.nPz;
+            r.done = now_done;
+            if (hs.halted) {   // the reference's driver halts such a run (3dmhd.cpp:292-330)
+                r.status = MHD3D_REQ_FAILED;
+                r.t_finish = now;
+                continue;
+            }
+            if (r.done >= r.nsteps) {
+                r.status = MHD3D_REQ_DONE;
+                r.t_finish = now;
+            } else {
+                keep.push_back(r.id);
+            }
+        }
+        if (rc) { s->err = e->err; return rc; }
+        if (keep.size() != s->active.size()) s->dirty = true;
+        s->active.swap(keep);
+    }
+    if (n_active) *n_active = (int)s->active.size();
+    if (n_queued) *n_queued = (int)s->queue.size();
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_server_poll(mhd3d_server* s, int64_t id, mhd3d_request_info* info)
+{
+    if (!s || !info) return MHD3D_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { s->err = "unknown request"; return MHD3D_EINVAL; }
+    const ServerReq& r = it->second;
+    info->status = r.status;
+    info->steps_done = r.done;
+    info->t_submit = r.t_submit;
+    info->t_admit = r.t_admit;
+    info->t_finish = r.t_finish;
+    info->dt_last = r.dt_last;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* fields)
+{
+    if (!s) return MHD3D_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { s->err = "unknown request"; return MHD3D_EINVAL; }
+    ServerReq& r = it->second;
+    if (r.status != MHD3D_REQ_DONE && r.status != MHD3D_REQ_FAILED) { s->err = "request not finished"; return MHD3D_EINVAL; }
+    int rc = MHD3D_OK;
+    if (fields && r.status == MHD3D_REQ_DONE) {
+        r.ctx->cur = s->ens->mixed ? 0 : s->ens->parity;
+        rc = mhd3d_get_state(r.ctx, fields);
+        if (rc) s->err = r.ctx->err;
+    }
+    if (r.ctx) {
+        mhd3d_destroy(r.ctx);
+        r.ctx = nullptr;
+        s->used -= r.bytes;
+    }
+    r.status = r.status == MHD3D_REQ_DONE ? MHD3D_REQ_RELEASED : r.status;
+    return rc;
 }

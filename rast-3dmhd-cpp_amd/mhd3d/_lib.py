@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 LIB_PATH = os.path.join(PKG, "lib", "libmhd3d.so")
 
-OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENONFINITE = range(6)
+OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENONFINITE, EHALTED = range(7)
+REFERENCE_DT_MIN = float(np.float32(1.0e-8))   # the reference driver's DT guard (DT < 1.0e-8f)
 FP64, MIXED = 0, 1
 FIELDS = ("RU", "RV", "RW", "RO", "TT", "BX", "BY", "BZ")
 FLUXES = ("FU", "FV", "FW", "FR", "FT", "WW1", "WW2", "WW3")
@@ -40,6 +41,10 @@ SIGNATURES = {
     "mhd3d_synchronize": (C.c_int, [_vp]),
     "mhd3d_last_error": (C.c_char_p, [_vp]),
     "mhd3d_device_bytes": (C.c_int64, [_vp]),
+    "mhd3d_set_dt_guard": (C.c_int, [_vp, C.c_double]),
+    "mhd3d_ensemble_set_dt_guard": (C.c_int, [_vp, C.c_double]),
+    "mhd3d_ensemble_halted": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+    "mhd3d_init_stratified": (C.c_int, [_vp, _dp, _dp, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "mhd3d_ensemble_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
     "mhd3d_ensemble_destroy": (C.c_int, [_vp]),
     "mhd3d_ensemble_last_error": (C.c_char_p, [_vp]),
@@ -54,7 +59,25 @@ SIGNATURES = {
     "mhd3d_ensemble_get_variant": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mhd3d_ensemble_synchronize": (C.c_int, [_vp]),
     "mhd3d_ensemble_autotune": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int)]),
+    "mhd3d_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(_vp)]),
+    "mhd3d_server_destroy": (C.c_int, [_vp]),
+    "mhd3d_server_last_error": (C.c_char_p, [_vp]),
+    "mhd3d_server_now": (C.c_double, [_vp]),
+    "mhd3d_server_bytes_in_use": (C.c_int64, [_vp]),
+    "mhd3d_server_cell_steps": (C.c_int64, [_vp]),
+    "mhd3d_server_submit": (C.c_int, [_vp, _vp, _dp, _dp, C.c_uint64, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.POINTER(C.c_int64)]),
+    "mhd3d_server_tick": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mhd3d_server_poll": (C.c_int, [_vp, C.c_int64, _vp]),
+    "mhd3d_server_result": (C.c_int, [_vp, C.c_int64, _dpp]),
 }
+
+REQ_FAILED, REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_RELEASED = -1, 0, 1, 2, 3
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("steps_done", C.c_int32), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_finish", C.c_double), ("dt_last", C.c_double)]
 
 
 class CflInfo(C.Structure):

@@ -60,6 +60,17 @@ class Domain:
             arrs.append(a)
         self._check(L.lib().mhd3d_set_state(self.h, L._fieldptrs(arrs)))
 
+    def init_stratified(self, seed=12345, amp=1e-4, b0=1e-3, b_amp=1e-5):
+        """Build the synthetic stratified state on the device
+        (stratified_state_hashed is its host copy)."""
+        from .initial import stratified_profiles
+        T, rho = stratified_profiles(self.desc)
+        T = np.ascontiguousarray(T)
+        rho = np.ascontiguousarray(rho)
+        dp = C.POINTER(C.c_double)
+        self._check(L.lib().mhd3d_init_stratified(self.h, T.ctypes.data_as(dp), rho.ctypes.data_as(dp),
+                                                  int(seed), float(amp), float(b0), float(b_amp)))
+
     def state(self) -> dict:
         arrs = [np.empty(self.shape) for _ in range(self.nf)]
         self._check(L.lib().mhd3d_get_state(self.h, L._fieldptrs(arrs)))
@@ -107,6 +118,11 @@ class Domain:
         t = L.Timing()
         self._check(L.lib().mhd3d_timing_read(self.h, C.byref(t)))
         return {"stage_ms_total": t.stage_ms_total, "stage_launches": t.stage_launches}
+
+    def set_dt_guard(self, dt_min=L.REFERENCE_DT_MIN):
+        """The reference driver's DT guard: steps after one with DT < dt_min
+        do nothing and step() raises MhdError(EHALTED).  0 disables it."""
+        self._check(L.lib().mhd3d_set_dt_guard(self.h, float(dt_min)))
 
     def synchronize(self):
         self._check(L.lib().mhd3d_synchronize(self.h))

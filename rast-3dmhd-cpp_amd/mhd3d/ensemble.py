@@ -95,5 +95,13 @@ class Ensemble:
                        ms_per_step=float(rep[4 * k + 3])) for k in range(n.value)]
         return self.variant(), trials
 
+    def set_dt_guard(self, dt_min=L.REFERENCE_DT_MIN):
+        self._check(L.lib().mhd3d_ensemble_set_dt_guard(self.h, float(dt_min)))
+
+    def halted(self, i):
+        h = C.c_int()
+        self._check(L.lib().mhd3d_ensemble_halted(self.h, int(i), C.byref(h)))
+        return bool(h.value)
+
     def synchronize(self):
         self._check(L.lib().mhd3d_ensemble_synchronize(self.h))

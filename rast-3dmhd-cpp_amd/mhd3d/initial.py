@@ -66,19 +66,27 @@ def periodic_fill(a):
     return a
 
 
-def stratified_state(desc: Descriptor, seed=12345, amp=1e-4, b0=1e-3, b_amp=1e-5):
+def stratified_profiles(desc: Descriptor):
+    """z profiles (TT_z, RO_z) of the polytrope, ghost planes included, with
+    the wall temperatures BCON imposes."""
     cfg = desc.cfg
     par = cfg.par
     theta, grav = par[2], par[3]
     m_poly = grav / theta - 1.0
-    nz, ny, nx = desc.shape
     zee = np.array(desc.metrics.ZEE, dtype=np.float64)
     T = 1.0 + theta * zee
     rho = T ** m_poly
+    T[-1] = T[-2] = cfg.TB if cfg.izCon == 0 else T[-2]
+    T[0] = T[1] = 1.0
+    return T, rho
+
+
+def stratified_state(desc: Descriptor, seed=12345, amp=1e-4, b0=1e-3, b_amp=1e-5):
+    cfg = desc.cfg
+    nz, ny, nx = desc.shape
+    T, rho = stratified_profiles(desc)
     TT = np.broadcast_to(T[:, None, None], (nz, ny, nx)).copy()
     RO = np.broadcast_to(rho[:, None, None], (nz, ny, nx)).copy()
-    TT[-1] = TT[-2] = cfg.TB if cfg.izCon == 0 else TT[-2]
-    TT[0] = TT[1] = 1.0
     rng = np.random.default_rng(seed)
     inner = (slice(2, nz - 2), slice(1, ny - 1), slice(1, nx - 1))
     out = {}
@@ -101,6 +109,58 @@ def stratified_state(desc: Descriptor, seed=12345, amp=1e-4, b0=1e-3, b_amp=1e-5
         for B in (BX, BY, BZ):       # finite ghost planes: copies of the walls
             B[0] = B[1]
             B[-1] = B[-2]
+        out["BX"] = periodic_fill(BX)
+        out["BY"] = periodic_fill(BY)
+        out["BZ"] = periodic_fill(BZ)
+    return out
+
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def hash_uniform(seed, f, z, y, x):
+    """Counter-based uniform deviate in [-1, 1): the splitmix64 finaliser of
+    the packed key (the device initializer's generator, bit for bit)."""
+    z = np.asarray(z, dtype=np.uint64)
+    y = np.asarray(y, dtype=np.uint64)
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        k = (np.uint64(seed) * _GOLD) ^ (np.uint64(f) << np.uint64(56)) ^ (z << np.uint64(36)) \
+            ^ (y << np.uint64(18)) ^ x
+        k = k + _GOLD
+        k = (k ^ (k >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        k = (k ^ (k >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        k = k ^ (k >> np.uint64(31))
+    return (k >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
+def stratified_state_hashed(desc: Descriptor, seed=12345, amp=1e-4, b0=1e-3, b_amp=1e-5):
+    """Host copy of the state mhd3d_init_stratified builds on the device."""
+    cfg = desc.cfg
+    nz, ny, nx = desc.shape
+    T, rho = stratified_profiles(desc)
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny - 2), np.arange(nx - 2), indexing="ij")
+    out = {}
+    RO = np.zeros((nz, ny, nx))
+    RO[:, 1:-1, 1:-1] = rho[:, None, None]
+    TT = np.zeros((nz, ny, nx))
+    TT[:, 1:-1, 1:-1] = T[:, None, None]
+    inner = (zz >= 2) & (zz <= nz - 3)
+    for f, name in enumerate(("RU", "RV", "RW")):
+        a = np.zeros((nz, ny, nx))
+        a[:, 1:-1, 1:-1] = np.where(inner, RO[:, 1:-1, 1:-1] * amp * hash_uniform(seed, f, zz, yy, xx), 0.0)
+        out[name] = periodic_fill(a)
+    out["RO"] = periodic_fill(RO)
+    out["TT"] = periodic_fill(TT)
+    if cfg.lMag:
+        zw = np.where(zz == 0, 1, np.where(zz == nz - 1, nz - 2, zz))
+        wall = (zw == 1) | (zw == nz - 2)
+        BX = np.zeros((nz, ny, nx))
+        BY = np.zeros((nz, ny, nx))
+        BZ = np.zeros((nz, ny, nx))
+        BX[:, 1:-1, 1:-1] = b0 + b_amp * hash_uniform(seed, 5, zw, yy, xx)
+        BY[:, 1:-1, 1:-1] = b_amp * hash_uniform(seed, 6, zw, yy, xx)
+        BZ[:, 1:-1, 1:-1] = np.where(wall, 0.0, b_amp * hash_uniform(seed, 7, zw, yy, xx))
         out["BX"] = periodic_fill(BX)
         out["BY"] = periodic_fill(BY)
         out["BZ"] = periodic_fill(BZ)
