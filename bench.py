@@ -29,9 +29,12 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 def alg_bytes_per_cell_step(nf, precision=0):
     """Algorithmic HBM bytes per cell per RK3 step with the 2-register Wray
-    scheme: stage 1 reads U, writes U and Z; stage 2 reads U, Z, writes U, Z;
-    stage 3 reads U, Z, writes U -> 10 field sweeps of 8 bytes."""
-    return 10 * nf * 8
+    scheme.  FP64: stage 1 reads U, writes U and Z; stage 2 reads U, Z, writes
+    U, Z; stage 3 reads U, Z, writes U -> 10 field sweeps of 8 bytes.  Mixed:
+    stage 1 reads U (8) and writes the FP32 copy (4) and Z (8); stage 2 reads
+    the copy and Z, writes a copy and Z (24); stage 3 reads the copy and Z and
+    writes U (20) -> 64 bytes per field."""
+    return (64 if precision else 80) * nf
 
 
 def cpu_model():
@@ -76,6 +79,191 @@ def run_cpu_baseline(cfg, state, budget_s=12.0):
     }
 
 
+# HBM bytes per launch of the interior stage kernel (mean of the three
+# stages), from rocprofv3 FETCH_SIZE / WRITE_SIZE passes calibrated on a copy
+# kernel of known size (profiles/stage_kernel_mhd256x256x128_fp64.md).  Only
+# the profiled workload has a measured value.
+MEASURED_TRAFFIC = {("mhd", (256, 256, 128)): 1830.8e6}
+
+
+def measured_traffic(mag, grid):
+    return MEASURED_TRAFFIC.get(("mhd" if mag else "hydro", tuple(grid)))
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _relerr(g, c):
+    g = g[1:-1, 1:-1, 1:-1]
+    c = c[1:-1, 1:-1, 1:-1]
+    den = float(abs(c).max())
+    return float(abs(g - c).max()) / den if den > 0 else float(abs(g - c).max())
+
+
+def measure_mixed(size=(512, 512, 256), steps=20, err_steps=100):
+    """Mixed-precision MHD workload: FP64 and mixed forms from the same device
+    initial state; throughput over `steps` timed steps and the per-field /
+    integral error of the mixed form against the FP64 form after err_steps."""
+    from mhd3d.desc import Descriptor
+    from mhd3d.domain import Domain
+    from mhd3d.initial import stratified_config
+
+    out = {"workload": f"MHD+rotation {size[0]}x{size[1]}x{size[2]}, Ra 1e6, CFL dt",
+           "steps_timed": steps, "error_after_steps": err_steps}
+    cells = size[0] * size[1] * size[2]
+    doms = []
+    for prec in (0, 1):
+        cfg = stratified_config(*size, Ra=1e6, lMag=True, lRot=True, precision=prec)
+        dom = Domain(Descriptor(cfg))
+        dom.init_stratified(seed=4242)
+        dom.step(3)
+        dom.synchronize()
+        dom.timing(True)
+        t0 = time.perf_counter()
+        dom.step(steps)
+        dom.synchronize()
+        el = time.perf_counter() - t0
+        tm = dom.timing()
+        dom.timing(False)
+        nf = dom.nf
+        interior = size[0] * size[1] * (size[2] - 2)
+        avg = tm["stage_ms_total"] / max(1, tm["stage_launches"])
+        achieved = alg_bytes_per_cell_step(nf, prec) / 3.0 * interior / (avg * 1e-3) / 1e9
+        out["fp64" if prec == 0 else "mixed"] = {
+            "mcell_steps_per_s": round(cells * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 3),
+            "stage_kernel_ms": round(avg, 4), "stage_kernel_gbs": round(achieved, 1)}
+        doms.append(dom)
+        log(f"mixed workload: form {prec} {out['fp64' if prec == 0 else 'mixed']}")
+    out["speedup"] = round(out["mixed"]["mcell_steps_per_s"] / out["fp64"]["mcell_steps_per_s"], 3)
+    # error of the mixed form after err_steps from the same initial state
+    for dom in doms:
+        dom.init_stratified(seed=4242)
+        dom.step(err_steps)
+    s64 = doms[0].state()
+    g64 = doms[0].diagnostics()
+    smx = doms[1].state()
+    gmx = doms[1].diagnostics()
+    out["field_error"] = {k: float("%.3e" % _relerr(smx[k], s64[k])) for k in s64}
+    out["mass_rel_error"] = float("%.3e" % (abs(gmx["mass"] - g64["mass"]) / abs(g64["mass"])))
+    out["energy_rel_error"] = float("%.3e" % (abs(gmx["energy"] - g64["energy"]) / abs(g64["energy"])))
+    out["max_divB"] = {"fp64": float("%.6e" % g64["max_divB"]), "mixed": float("%.6e" % gmx["max_divB"])}
+    for dom in doms:
+        dom.close()
+    return out
+
+
+def measure_ensemble(n=64, steps=20):
+    """Ensemble workload: n runs of uneven grids / flags / parameters advanced
+    together (graph replay, autotuned variant), FP64 and mixed, against the
+    same runs stepped one after another in plain FP64."""
+    from mhd3d.desc import Descriptor
+    from mhd3d.domain import Domain
+    from mhd3d.ensemble import Ensemble
+    from mhd3d.workloads import ensemble_mix
+
+    specs = ensemble_mix(n)
+    cells = sum(s.cells for s in specs)
+    out = {"workload": f"{n} runs, nx=ny in {{32..128}}, nz in {{32..96}}, Ra 1e4-1e6, half lMag",
+           "cells": cells, "steps_timed": steps}
+    descs0 = []
+    for prec in (0, 1):
+        descs = [Descriptor(s.config(prec)) for s in specs]
+        if prec == 0:
+            descs0 = descs
+        ens = Ensemble(descs)
+        _init_ensemble(ens, descs, specs)
+        ens.step(2)
+        best, trials = ens.autotune(steps=3)
+        ens.graph(True)
+        ens.step(2)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        ens.synchronize()
+        el = time.perf_counter() - t0
+        out["fp64" if prec == 0 else "mixed"] = {
+            "mcell_steps_per_s": round(cells * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 3),
+            "variant": {"waves": best[0], "sched": best[1], "chunk": best[2]}}
+        log(f"ensemble: form {prec} {out['fp64' if prec == 0 else 'mixed']}")
+        ens.close()
+    tot = 0.0
+    for d, s in zip(descs0, specs):
+        dom = Domain(d)
+        dom.init_stratified(seed=s.seed)
+        dom.step(2)
+        dom.synchronize()
+        t0 = time.perf_counter()
+        dom.step(steps)
+        dom.synchronize()
+        tot += time.perf_counter() - t0
+        dom.close()
+    out["sequential_fp64"] = {"mcell_steps_per_s": round(cells * steps / tot / 1e6, 1),
+                              "ms_per_step": round(tot / steps * 1e3, 3)}
+    for k in ("fp64", "mixed"):
+        out[k]["vs_sequential_fp64"] = round(out[k]["mcell_steps_per_s"] / out["sequential_fp64"]["mcell_steps_per_s"], 3)
+    log(f"ensemble: sequential {out['sequential_fp64']}")
+    return out
+
+
+def _init_ensemble(ens, descs, specs):
+    """Initial state of every member: the synthetic stratified state of the
+    request server's initializer (its host copy)."""
+    from mhd3d.initial import stratified_state_hashed
+    for i, (d, s) in enumerate(zip(descs, specs)):
+        ens.set_state(i, stratified_state_hashed(d, seed=s.seed))
+
+
+def measure_served(n=40, rate=50.0, quantum=10):
+    """Served workload: a Poisson trace replayed against the request server
+    (FP64 and mixed) and the same runs one at a time in plain FP64."""
+    from mhd3d.server import sequential_trace, serve_trace
+    from mhd3d.workloads import served_trace
+
+    specs = served_trace(n, rate=rate)
+    out = {"workload": f"{n} runs, Poisson arrivals {rate:g}/s, grids 32^3..256x256x128, "
+                       f"steps 50-2000, random lMag/lRot", "quantum_steps": quantum,
+           "hbm_budget_gib": 64}
+    for prec in (0, 1):
+        r = serve_trace(specs, precision=prec, quantum_steps=quantum)
+        out["fp64" if prec == 0 else "mixed"] = {
+            "p50_s": round(r["p50_s"], 3), "p99_s": round(r["p99_s"], 3),
+            "mcell_steps_per_s": round(r["mcell_steps_per_s"], 1), "makespan_s": round(r["makespan_s"], 3),
+            "halted_by_dt_guard": r["failed"]}
+        log(f"served: form {prec} {out['fp64' if prec == 0 else 'mixed']}")
+    q = sequential_trace(specs, 0)
+    out["sequential_fp64"] = {"p50_s": round(q["p50_s"], 3), "p99_s": round(q["p99_s"], 3),
+                              "mcell_steps_per_s": round(q["mcell_steps_per_s"], 1),
+                              "halted_by_dt_guard": q["failed"]}
+    for k in ("fp64", "mixed"):
+        out[k]["vs_sequential_fp64"] = round(out[k]["mcell_steps_per_s"] / out["sequential_fp64"]["mcell_steps_per_s"], 3)
+    log(f"served: sequential {out['sequential_fp64']}")
+    return out
+
+
+def profile_run(steps):
+    """For rocprofv3: `steps` steps of the headline workload, then a plain
+    device copy of 1 GiB whose counters calibrate FETCH/WRITE_SIZE."""
+    from mhd3d.desc import Descriptor
+    from mhd3d.domain import Domain
+    from mhd3d.initial import stratified_config
+
+    cfg = stratified_config(256, 256, 128, Ra=1e5, lMag=True, lRot=True)
+    dom = Domain(Descriptor(cfg))
+    dom.init_stratified(seed=12345)
+    dom.step(steps)
+    dom.synchronize()
+    dom.close()
+    import ctypes as C
+    from mhd3d import _lib as L
+    ms = C.c_double()
+    rc = L.lib().mhd3d_calibration_copy(1 << 30, 3, C.byref(ms))
+    if rc != L.OK:
+        raise L.MhdError(rc, "calibration copy failed")
+    log(f"profile run: {steps} steps, 1 + 3 copies of {1 << 30} bytes, "
+        f"{ms.value:.4f} ms per copy ({2 * (1 << 30) / ms.value / 1e6:.0f} GB/s)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,12 +272,20 @@ def main():
     ap.add_argument("--size", default="256x256x128")
     ap.add_argument("--hydro", action="store_true", help="lMag off, lRot off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workloads", default="mixed,ensemble,served",
+                    help="extra workloads measured beside the headline ('' for none)")
+    ap.add_argument("--profile", type=int, default=0, metavar="STEPS",
+                    help="run STEPS steps and a calibration copy for rocprofv3, print nothing")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU z-slab halo design is documented but not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.profile:
+        profile_run(args.profile)
+        return
 
     from mhd3d.desc import Descriptor
     from mhd3d.domain import Domain
@@ -150,15 +346,42 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(mag, (nx, ny, nz)),
             "kernel": "interior stage kernel",
             "kernel_avg_ms": round(avg_ms, 5),
             "alg_bytes_per_launch": int(bytes_per_launch),
         },
     }
+    # the drop-in boundary hands over host arrays: time one upload and one
+    # download of the whole state (host layout, ghosts included)
+    t0 = time.perf_counter()
+    dom.set_state(state)
+    t_set = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dom.state()
+    t_get = time.perf_counter() - t0
+    out["boundary"] = {
+        "set_state_ms": round(t_set * 1e3, 2), "get_state_ms": round(t_get * 1e3, 2),
+        "pcie_inclusive_mcell_steps_per_s": round(cells * args.steps / (el + t_set + t_get) / 1e6, 1),
+        "note": f"{args.steps} steps between one set_state and one get_state"}
     dom.close()
+    log(f"headline: {value:.1f} Mcell-steps/s, stage kernel {avg_ms:.4f} ms, {achieved:.0f} GB/s")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = run_cpu_baseline(cfg, state)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.2f} Mcell-steps/s")
+    extra = {}
+    for w in [w for w in args.workloads.split(",") if w]:
+        fn = {"mixed": measure_mixed, "ensemble": measure_ensemble, "served": measure_served}.get(w)
+        if fn is None:
+            extra[w] = {"error": "unknown workload"}
+            continue
+        try:
+            extra[w] = fn()
+        except Exception as err:   # a failed side measurement must not lose the headline
+            extra[w] = {"error": f"{type(err).__name__}: {err}"}
+            log(f"{w}: {extra[w]['error']}")
+    if extra:
+        out["workloads"] = extra
     print(json.dumps(out), flush=True)
 
 

@@ -270,6 +270,12 @@ int mhd3d_server_poll(mhd3d_server* s, int64_t id, mhd3d_request_info* info);
  * its device memory */
 int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* fields);
 
+/* Measurement support (no counterpart in the reference): `reps` launches of
+ * a plain device copy of `bytes` bytes between two fresh buffers, to
+ * calibrate a profiler's HBM byte counters against a known traffic;
+ * *ms_per_copy gets the event-timed average. */
+int mhd3d_calibration_copy(int64_t bytes, int reps, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -667,6 +667,15 @@ __global__ void __launch_bounds__(256) init_stratified_kernel(Geom g, int nf, do
     }
 }
 
+// Plain streaming copy (HBM byte-counter calibration): dst[i] = src[i].
+__global__ void __launch_bounds__(256) calibration_copy_kernel(const double* __restrict__ src,
+                                                               double* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // FP32 copy of a register in the mixed form's deviation layout:
 // dst = (float)(src - background) for RO, TT, BX, BY, BZ, (float)src otherwise.
 __global__ void __launch_bounds__(256) to_f32_dev_kernel(const double* __restrict__ src,

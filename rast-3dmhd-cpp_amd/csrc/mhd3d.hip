@@ -1685,3 +1685,38 @@ extern "C" int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* f
     r.status = r.status == MHD3D_REQ_DONE ? MHD3D_REQ_RELEASED : r.status;
     return rc;
 }
+
+// Calibration of HBM byte counters: `reps` launches of a plain copy of
+// `bytes` bytes between two fresh device buffers (so a profiler's per-
+// dispatch FETCH_SIZE / WRITE_SIZE can be compared with a known traffic of
+// `bytes` read and `bytes` written); *ms_per_copy: event-timed average.
+extern "C" int mhd3d_calibration_copy(int64_t bytes, int reps, double* ms_per_copy)
+{
+    if (bytes < 8 || reps < 1) return MHD3D_EINVAL;
+    const int64_t n = bytes / 8;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MHD3D_OK;
+    if (hipMalloc(&a, n * 8) != hipSuccess || hipMalloc(&b, n * 8) != hipSuccess) rc = MHD3D_ENOMEM;
+    if (!rc && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = MHD3D_EHIP;
+    if (!rc && hipMemsetAsync(a, 0, n * 8, st) != hipSuccess) rc = MHD3D_EHIP;
+    if (!rc) {
+        calibration_copy_kernel<<<4096, 256, 0, st>>>(a, b, n);   // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) calibration_copy_kernel<<<4096, 256, 0, st>>>(a, b, n);
+        (void)hipEventRecord(e1, st);
+        if (hipStreamSynchronize(st) != hipSuccess) rc = MHD3D_EHIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms_per_copy) *ms_per_copy = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
