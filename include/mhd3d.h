@@ -270,6 +270,23 @@ int mhd3d_server_poll(mhd3d_server* s, int64_t id, mhd3d_request_info* info);
  * its device memory */
 int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* fields);
 
+/* nPx, nPy, nPz and the number of fields of a context. */
+int mhd3d_geometry(const mhd3d_ctx* ctx, int32_t out[4]);
+
+/* The reference's dump/restart files (3dmhdio.h:20-72).  .dat0: dump idx =
+ * records n_f*idx .. n_f*idx+n_f-1 of nPx*nPy*nPz doubles, fields RU, RV, RW,
+ * TT, RO [, BX, BY, BZ], planes 1..nPz x rows 1..nPy x columns 1..nPx, x
+ * fastest.  mhd3d_load_data keeps the context's z ghost planes (the
+ * reference's restart, 3dmhd.cpp:112-133, leaves them as they were) and
+ * fills x/y ghosts periodically.  .par: record idx = 96 doubles (par1:
+ * ipar[0..31] then par[0..63], with nIt at [14], DT at [62], TIMT/TIMI at
+ * [63], 0/TIMC at [64]; 3dmhd.cpp:271-289).  Other records of a file are
+ * kept; missing files are created. */
+int mhd3d_dump_data(mhd3d_ctx* ctx, const char* path, int idx);
+int mhd3d_load_data(mhd3d_ctx* ctx, const char* path, int idx);
+int mhd3d_dump_param(const char* path, int idx, const double* par1);
+int mhd3d_load_param(const char* path, int idx, double* par1);
+
 /* Measurement support (no counterpart in the reference): `reps` launches of
  * a plain device copy of `bytes` bytes between two fresh buffers, to
  * calibrate a profiler's HBM byte counters against a known traffic;

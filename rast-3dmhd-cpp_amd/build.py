@@ -1,16 +1,23 @@
-"""Builds libmhd3d.so (HIP, gfx950) in-tree: rast-3dmhd-cpp_amd/lib/libmhd3d.so."""
+"""Builds libmhd3d.so (HIP, gfx950) in-tree: rast-3dmhd-cpp_amd/lib/libmhd3d.so.
+
+Every source under csrc/ (.hip device + host code, .cpp host code) is compiled
+to an object in parallel, then the objects are linked into one library.
+"""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "mhd3d.hip")
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = [os.path.join(CSRC, f) for f in ("mhd3d.hip", "dumpio.cpp")]
 OUT_DIR = os.path.join(HERE, "lib")
+OBJ_DIR = os.path.join(HERE, "lib", "obj")
 OUT = os.path.join(OUT_DIR, "libmhd3d.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # parity: no contraction into FMA, same rounding as the reference's x86-64 build
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -18,10 +25,23 @@ FLAGS = [
 ]
 
 
+def _compile(src, extra, verbose):
+    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result", *extra, "-c", src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
 def build(verbose=False, extra=()):
-    os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, list(extra), verbose), SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, *extra, SRC, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

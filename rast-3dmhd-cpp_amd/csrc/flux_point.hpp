@@ -67,16 +67,57 @@ struct Ops {
     }
 };
 
-template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, typename A>
-__device__ __forceinline__ void flux_point(const A& a, const Phys& c, const PointMetric& m,
-                                           FluxOutT<T>& o)
+template <typename T>
+__device__ __forceinline__ Ops<T> make_ops(const Phys& c, const PointMetric& m)
 {
-    const Ops<T> op{T(m.dxx), T(m.d2xx), T(m.dyy), T(m.d2yy), T(m.dzz), T(m.d2zz),
-                    T(c.hx),  T(c.h2x),  T(c.hy),  T(c.h2y),  T(c.hz),  T(c.h2z)};
+    return Ops<T>{T(m.dxx), T(m.d2xx), T(m.dyy), T(m.d2yy), T(m.dzz), T(m.d2zz),
+                  T(c.hx),  T(c.h2x),  T(c.hy),  T(c.h2y),  T(c.hz),  T(c.h2z)};
+}
+
+// Contributions of the magnetic block to FU, FV, FW, FT, in the order the
+// reference adds them (3dmhdsub.h:592-690).  The ops a sink receives, in
+// sequence (U/V/W/T = FU/FV/FW/FT, + add, - subtract):
+//   U+ U- U- V+ V- W+ T+ T+ T+ T+ | U+ W- T+ T+ | V+ W- T+ T+ T+
+enum MagOp { MAG_UA, MAG_US, MAG_VA, MAG_VS, MAG_WA, MAG_WS, MAG_TA };
+constexpr int NMAGTERM = 19;
+constexpr MagOp kMagOps[NMAGTERM] = {MAG_UA, MAG_US, MAG_US, MAG_VA, MAG_VS, MAG_WA, MAG_TA,
+                                     MAG_TA, MAG_TA, MAG_TA, MAG_UA, MAG_WS, MAG_TA, MAG_TA,
+                                     MAG_VA, MAG_WS, MAG_TA, MAG_TA, MAG_TA};
+
+// Applies one contribution to the accumulators.
+template <typename T>
+__device__ __forceinline__ void mag_apply(MagOp op, T v, T& FU, T& FV, T& FW, T& FT)
+{
+    switch (op) {
+    case MAG_UA: FU += v; break;
+    case MAG_US: FU -= v; break;
+    case MAG_VA: FV += v; break;
+    case MAG_VS: FV -= v; break;
+    case MAG_WA: FW += v; break;
+    case MAG_WS: FW -= v; break;
+    default: FT += v; break;
+    }
+}
+
+// Sink that applies each contribution at once (the single-thread flux).
+template <typename T>
+struct InlineSink {
+    T &FU, &FV, &FW, &FT;
+    __device__ __forceinline__ void put(MagOp op, T v) { mag_apply(op, v, FU, FV, FW, FT); }
+};
+
+// Hydrodynamic part of fluxes(): continuity, pressure and gravity,
+// advection, conduction, viscous terms, viscous heating, pressure work and
+// Coriolis (3dmhdsub.h:204-545).
+template <bool ROT, int WALL, typename T, bool DEV, typename A>
+__device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const PointMetric& m, T& FU,
+                                           T& FV, T& FW, T& FR, T& FT)
+{
+    const Ops<T> op = make_ops<T>(c, m);
     const T ORE = T(c.ORE), OCV = T(c.OCV);
     const T ORE_C43 = T(c.ORE_C43);
     const T C13_ORE = T(c.C13_ORE);
-    T FU, FV, FW, FR, FT, tmp;
+    T tmp;
 
     // ---- continuity :205-227
     FR = op.d1x(a.ru(1, 0, 0), a.ru(-1, 0, 0)) * T(-1);
@@ -225,13 +266,22 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         FW -= a.rv(0, 0, 0) * T(c.OMX);
     }
 
-    T B1 = T(0), B2 = T(0), B3 = T(0);
-    if (MAG) {
-        // ---- induction, Lorentz force, Joule heating :547-695
+}
+
+// Magnetic part of fluxes(): induction (B1, B2, B3 = WW1..WW3), Lorentz
+// force and Joule heating (3dmhdsub.h:547-695); the contributions to FU, FV,
+// FW, FT go to the sink in the reference's order (kMagOps).
+template <int WALL, typename T, bool DEV, typename A, typename S>
+__device__ __forceinline__ void flux_mag(const A& a, const Phys& c, const PointMetric& m, S& sink,
+                                         T& B1, T& B2, T& B3)
+{
+    const Ops<T> op = make_ops<T>(c, m);
+    const T RI = a.ri(0, 0, 0);
+    {
         // The reference recomputes D1x/D1y/D1z of u, v, w here (:548-570);
         // so does this code, behind a compiler memory fence, instead of
-        // keeping the values above live through the hydro terms (the
-        // register budget of the fused kernel).  Same operands, same results.
+        // keeping the values of the hydro part live (the register budget of
+        // the fused kernel).  Same operands, same results.
         asm volatile("" ::: "memory");
         const T dxu = op.d1x(a.u(1, 0, 0), a.u(-1, 0, 0));
         const T dxv = op.d1x(a.v(1, 0, 0), a.v(-1, 0, 0));
@@ -283,16 +333,16 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         s2 = op.d1x(a.by(1, 0, 0), a.by(-1, 0, 0));
         s3 = op.d1x(a.bz(1, 0, 0), a.bz(-1, 0, 0));
         s4 = op.d1y(a.bx(0, 1, 0), a.bx(0, -1, 0));
-        FU += (BY * s4) * OBETA;
-        FU -= (BY * s2) * OBETA;
-        FU -= (BZ * s3) * OBETA;
-        FV += (BX * s2) * OBETA;
-        FV -= (BX * s4) * OBETA;
-        FW += (BX * s3) * OBETA;
-        FT += ((RI * s4) * s4) * KJ;
-        FT += ((RI * s2) * s2) * KJ;
-        FT += ((RI * s3) * s3) * KJ;
-        FT += ((RI * s2) * s4) * KJ2;
+        sink.put(MAG_UA, (BY * s4) * OBETA);
+        sink.put(MAG_US, (BY * s2) * OBETA);
+        sink.put(MAG_US, (BZ * s3) * OBETA);
+        sink.put(MAG_VA, (BX * s2) * OBETA);
+        sink.put(MAG_VS, (BX * s4) * OBETA);
+        sink.put(MAG_WA, (BX * s3) * OBETA);
+        sink.put(MAG_TA, ((RI * s4) * s4) * KJ);
+        sink.put(MAG_TA, ((RI * s2) * s2) * KJ);
+        sink.put(MAG_TA, ((RI * s3) * s3) * KJ);
+        sink.put(MAG_TA, ((RI * s2) * s4) * KJ2);
         // :609-612
         B1 -= U0 * s1;
         B1 -= V0 * s4;
@@ -312,10 +362,10 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         s1 = op.d1z(a.bx(0, 0, 1), a.bx(0, 0, -1));
         if constexpr (DEV) s1 += a.dzb(0);
         s2 = op.d1y(a.by(0, 1, 0), a.by(0, -1, 0));
-        FU += (BZ * s1) * OBETA;
-        FW -= (BX * s1) * OBETA;
-        FT += ((RI * s1) * s1) * KJ;
-        FT += ((RI * s1) * dxBZ) * KJ2;
+        sink.put(MAG_UA, (BZ * s1) * OBETA);
+        sink.put(MAG_WS, (BX * s1) * OBETA);
+        sink.put(MAG_TA, ((RI * s1) * s1) * KJ);
+        sink.put(MAG_TA, ((RI * s1) * dxBZ) * KJ2);
         B1 -= W0 * s1;
         B2 -= V0 * s2;
         // :638-658, one-sided D2z(BX) at the walls
@@ -340,11 +390,11 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         s3 = op.d1y(a.bz(0, 1, 0), a.bz(0, -1, 0));
         s1 = s2;
         s1 -= s3;
-        FV += (BZ * s1) * OBETA;
-        FW -= (BY * s1) * OBETA;
-        FT += ((RI * s2) * s2) * KJ;
-        FT += ((RI * s3) * s3) * KJ;
-        FT += ((RI * s2) * s3) * KJ2;
+        sink.put(MAG_VA, (BZ * s1) * OBETA);
+        sink.put(MAG_WS, (BY * s1) * OBETA);
+        sink.put(MAG_TA, ((RI * s2) * s2) * KJ);
+        sink.put(MAG_TA, ((RI * s3) * s3) * KJ);
+        sink.put(MAG_TA, ((RI * s2) * s3) * KJ2);
         B2 -= W0 * s2;
         B3 -= V0 * s3;
         // :672-690, one-sided D2z(BY) at the walls
@@ -370,6 +420,21 @@ __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const Poin
         s3 = op.d2z(a.bz(0, 0, 1), BZs, a.bz(0, 0, -1));
         if constexpr (DEV) s3 += a.d2zb(2);
         B3 += s3 * ORM;
+    }
+}
+
+// fluxes() of one point: the hydrodynamic part, then (lMag) the magnetic
+// part with its contributions applied in order.
+template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, typename A>
+__device__ __forceinline__ void flux_point(const A& a, const Phys& c, const PointMetric& m,
+                                           FluxOutT<T>& o)
+{
+    T FU, FV, FW, FR, FT;
+    flux_hydro<ROT, WALL, T, DEV>(a, c, m, FU, FV, FW, FR, FT);
+    T B1 = T(0), B2 = T(0), B3 = T(0);
+    if (MAG) {
+        InlineSink<T> sink{FU, FV, FW, FT};
+        flux_mag<WALL, T, DEV>(a, c, m, sink, B1, B2, B3);
     }
     o.FU = FU;
     o.FV = FV;

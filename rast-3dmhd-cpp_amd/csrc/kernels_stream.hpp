@@ -462,6 +462,9 @@ struct StreamPlan {
 
 // Work decomposition of one domain: 32 x 8 tiles times z chunks of at least
 // 8 planes (the last chunk takes the remainder), about target_blocks items.
+// The launch is fastest when its blocks fill the device's resident slots
+// once (no second round, no chunk-overlap planes beyond what that needs), so
+// the caller passes the resident block count of the kernel variant.
 inline StreamPlan stream_plan(const Geom& g, int target_blocks = 1024)
 {
     StreamPlan p;
@@ -506,10 +509,44 @@ static void launch_stream_w(const StageArgs& a, hipStream_t s, const StreamPlan&
     }
 }
 
-template <typename T, bool MAG, bool ROT, int STAGE>
-static void launch_stream_stage(const StageArgs& a, hipStream_t s, int waves, int sched)
+// Kernel function of a variant (stage 2 stands for all stages: they share
+// the resource footprint up to the stage-3 CFL partials).
+template <typename T, bool MAG, bool ROT>
+static const void* stream_fn(int waves, int sched)
 {
-    const StreamPlan p = stream_plan(a.g);
+    if (waves == 1) {
+        switch (sched & 3) {
+        case 0: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 1, false, false>;
+        case 1: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 1, true, false>;
+        case 2: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 1, false, true>;
+        default: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 1, true, true>;
+        }
+    }
+    switch (sched & 3) {
+    case 0: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 2, false, false>;
+    case 1: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 2, true, false>;
+    case 2: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 2, false, true>;
+    default: return (const void*)stage_stream_kernel<T, MAG, ROT, 2, 2, true, true>;
+    }
+}
+
+// Blocks of a variant resident on the whole device at once.
+inline int resident_blocks(const void* fn, int threads)
+{
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
+    return per_cu * ncu;
+}
+
+template <typename T, bool MAG, bool ROT, int STAGE>
+static void launch_stream_stage(const StageArgs& a, hipStream_t s, int waves, int sched,
+                                int target_blocks)
+{
+    const StreamPlan p = stream_plan(a.g, target_blocks);
     if (waves == 1)
         launch_stream_w<T, MAG, ROT, STAGE, 1>(a, s, p, sched);
     else

@@ -58,6 +58,8 @@ struct mhd3d_ctx {
     int tune_waves[2][2] = {{2, 1}, {2, 2}};
     int tune_sched[2][2] = {{2, 0}, {1, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
+    int target_blocks = 0;          // work items of one streaming launch (z chunking);
+                                    // 0: the resident blocks of the kernel variant
     int64_t bytes = 0;
     std::string err;
     // timing
@@ -227,9 +229,21 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
                 ctx->tune_sched[fm][ph] = sc;
             }
         }
+    const char* tb = std::getenv("MHD3D_TARGET_BLOCKS");
+    if (tb && std::atoi(tb) >= 1) ctx->target_blocks = std::atoi(tb);
     const char* fc = std::getenv("MHD3D_FUSE_CFL");
     if (fc && std::atoi(fc) == 0) ctx->fuse_cfl = false;
     if (ctx->mixed) ctx->kernel_mode = 0;   // the mixed form has only the streaming kernel
+    if (ctx->target_blocks == 0) {
+        const int fm = ctx->mixed ? 1 : 0, ph = desc->lMag ? 1 : 0;
+        const int w = ctx->tune_waves[fm][ph], sc = ctx->tune_sched[fm][ph];
+        const void* fn;
+        if (ctx->mixed)
+            fn = desc->lMag ? stream_fn<float, true, true>(w, sc) : stream_fn<float, false, true>(w, sc);
+        else
+            fn = desc->lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
+        ctx->target_blocks = resident_blocks(fn, STX * STY);
+    }
     *out = ctx;
     return MHD3D_OK;
 }
@@ -262,6 +276,16 @@ extern "C" const char* mhd3d_last_error(const mhd3d_ctx* ctx)
 }
 
 extern "C" int64_t mhd3d_device_bytes(const mhd3d_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+extern "C" int mhd3d_geometry(const mhd3d_ctx* ctx, int32_t out[4])
+{
+    if (!ctx || !out) return MHD3D_EINVAL;
+    out[0] = ctx->g.nPx;
+    out[1] = ctx->g.nPy;
+    out[2] = ctx->g.nPz;
+    out[3] = ctx->nf;
+    return MHD3D_OK;
+}
 
 // ---------------------------------------------------------------------------
 // host <-> device field transfer
@@ -526,13 +550,13 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
     const int fm = ctx->mixed ? 1 : 0, ph = MAG ? 1 : 0;
     if (ctx->mixed) {
         launch_stream_stage<float, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
-                                                    ctx->tune_sched[fm][ph]);
+                                                    ctx->tune_sched[fm][ph], ctx->target_blocks);
     } else if (ctx->kernel_mode == 1) {
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
         stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
     } else {
         launch_stream_stage<double, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
-                                                     ctx->tune_sched[fm][ph]);
+                                                     ctx->tune_sched[fm][ph], ctx->target_blocks);
     }
 }
 
@@ -573,7 +597,7 @@ static void cfl_part_layout(const mhd3d_ctx* ctx, int& nstream, int& nwall)
     if (ctx->kernel_mode == 1)
         nstream = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * (g.nPz - 2);
     else
-        nstream = stream_plan(g).nblocks;
+        nstream = stream_plan(g, ctx->target_blocks).nblocks;
     nwall = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * 2;
 }
 

@@ -106,7 +106,7 @@ def test_unsupported_flags_fail_loudly():
 
 
 @pytest.mark.parametrize("tune", ["1,0", "1,1", "1,2", "1,3", "2,0", "2,1", "2,2", "2,3"])
-@pytest.mark.parametrize("case", ["hydro", "mhd_rot"])
+@pytest.mark.parametrize("case", ["hydro", "mhd_rot", "mhd"])
 def test_schedule_variants_bit_identical(case, tune, monkeypatch):
     """Every streaming-kernel variant (register budget, barriers, deferred
     update) of the FP64 form is bit-identical to the oracle."""
