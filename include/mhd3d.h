@@ -287,6 +287,19 @@ int mhd3d_load_data(mhd3d_ctx* ctx, const char* path, int idx);
 int mhd3d_dump_param(const char* path, int idx, const double* par1);
 int mhd3d_load_param(const char* path, int idx, double* par1);
 
+/* The reference's initial state Static() (3dmhdphsic.h:549-763) of a run on
+ * nPEy x nPEz ranks, assembled into one domain of nPEz*(nPz_global/nPEz)
+ * planes: the polytrope of derivs() integrated by bsstep on the global z grid
+ * (nGrid = 0, PZP = 0), each plane as its rank takes it (the last z rank
+ * aligns with the top of the profile), TT times 1 + AMPT*(RAN2 - 1/2) drawn
+ * in rank 0's loop order (IDUM = -62659), RU = RV = RW = 0; x/y ghosts
+ * periodic.  par = the reference's par[64] (THETA = par[2], GRAV = par[3],
+ * zMax = par[18], AMPT = par[19]).  fields: RU, RV, RW, RO, TT host arrays
+ * of [nPEz*nRz+2][nPy+2][nPx+2].  lMag (tube() / AMPB layers) is not built:
+ * MHD3D_EUNSUPPORTED.  Host-only; needs no GPU. */
+int mhd3d_static_state(int nPx, int nPy, int nPz_global, int nPEy, int nPEz, const double* par,
+                       int lMag, double* const* fields);
+
 /* Measurement support (no counterpart in the reference): `reps` launches of
  * a plain device copy of `bytes` bytes between two fresh buffers, to
  * calibrate a profiler's HBM byte counters against a known traffic;

@@ -10,7 +10,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = [os.path.join(CSRC, f) for f in ("mhd3d.hip", "dumpio.cpp")]
+SOURCES = [os.path.join(CSRC, f) for f in ("mhd3d.hip", "dumpio.cpp", "static_state.cpp")]
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "lib", "obj")
 OUT = os.path.join(OUT_DIR, "libmhd3d.so")

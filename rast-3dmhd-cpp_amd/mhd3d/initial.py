@@ -165,3 +165,24 @@ def stratified_state_hashed(desc: Descriptor, seed=12345, amp=1e-4, b0=1e-3, b_a
         out["BY"] = periodic_fill(BY)
         out["BZ"] = periodic_fill(BZ)
     return out
+
+
+def reference_static_state(cfg, nPz_global, nPEy=1, nPEz=1):
+    """The reference's Static() initial state (mhd3d_static_state): the run of
+    nPEy x nPEz ranks on a grid of nPz_global planes, assembled into one
+    domain of nPEz*(nPz_global//nPEz) planes (cfg.nPz must be that)."""
+    import ctypes as C
+    from . import _lib as L
+
+    nRz = nPz_global // nPEz
+    if cfg.nPz != nPEz * nRz:
+        raise ValueError(f"cfg.nPz must be {nPEz * nRz} for {nPEz} z ranks of {nPz_global} planes")
+    shape = (cfg.nPz + 2, cfg.nPy + 2, cfg.nPx + 2)
+    arrs = [np.zeros(shape) for _ in range(5)]
+    par = np.ascontiguousarray(cfg.par, dtype=np.float64)
+    rc = L.lib().mhd3d_static_state(cfg.nPx, cfg.nPy, int(nPz_global), int(nPEy), int(nPEz),
+                                    par.ctypes.data_as(C.POINTER(C.c_double)), int(cfg.lMag),
+                                    L._fieldptrs(arrs))
+    if rc != L.OK:
+        raise L.MhdError(rc, "mhd3d_static_state failed")
+    return dict(zip(("RU", "RV", "RW", "RO", "TT"), arrs))

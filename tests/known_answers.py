@@ -53,15 +53,18 @@ def default_config(lRot=False):
                      lRot=lRot, izCon=1, iTCon=4, par=par)
 
 
-def default_state(cfg, oracle_mod):
-    """Static() of 3dmhdphsic.h:549-763 for the 4x6 rank run, as one domain.
+def default_state(cfg, oracle_mod, NP=NP, NPE_Y=NPE_Y, NPE_Z=NPE_Z):
+    """Static() of 3dmhdphsic.h:549-763 for an NPE_Y x NPE_Z rank run over NP
+    global planes (default: the 4x6 rank run), as one domain.
 
-    Plane g of the domain sits on z rank g//42; ranks 0..4 take T[g], the last
-    rank takes T[g+4] (3dmhdphsic.h:642-666 skips T[210..213]).  The RND
+    Plane g of the domain sits on z rank g//NRZ (NRZ = NP//NPE_Z); ranks
+    below the last take T[g], the last rank takes T[g + NP - NPE_Z*NRZ]
+    (3dmhdphsic.h:642-666 skips T[210..213] in the default run).  The RND
     perturbation follows the rank loop order of :709-744."""
     par = cfg.par
+    nrz = NP // NPE_Z
     T, R = oracle_mod.static_profile(NP, par[2], par[3], par[18])
-    nzg = NPE_Z * NRZ
+    nzg = NPE_Z * nrz
     shape = (nzg + 2, cfg.nPy + 2, cfg.nPx + 2)
     TT = np.zeros(shape)
     RO = np.zeros(shape)
@@ -72,7 +75,7 @@ def default_state(cfg, oracle_mod):
             k = NP - 1
         else:
             g = z - 1
-            k = g if g // NRZ < NPE_Z - 1 else g + (NP - nzg)
+            k = g if g // nrz < NPE_Z - 1 else g + (NP - nzg)
         TT[z, 1:-1, 1:-1] = T[k]
         RO[z, 1:-1, 1:-1] = R[k]
     rnd = oracle_mod.static_rnd(cfg.nPx, cfg.nPy, nzg, NPE_Y, NPE_Z, par[19])
