@@ -54,9 +54,10 @@ struct mhd3d_ctx {
     // streaming-kernel variant per [form][physics] (form 0 = FP64, 1 = mixed;
     // physics 0 = hydro, 1 = MHD): register budget in waves per SIMD, and
     // schedule bits (1: one barrier per plane, 2: deferred RK update).
-    // Defaults: the fastest variants measured on MI355X at 256x256x128.
+    // Defaults: the fastest variants measured on MI355X at 256x256x128
+    // (launches sized to the resident block count).
     int tune_waves[2][2] = {{2, 1}, {2, 2}};
-    int tune_sched[2][2] = {{2, 0}, {1, 2}};
+    int tune_sched[2][2] = {{2, 0}, {2, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
     int target_blocks = 0;          // work items of one streaming launch (z chunking);
                                     // 0: the resident blocks of the kernel variant
