@@ -30,6 +30,8 @@
 #pragma once
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "mhd3d_types.hpp"
 
 namespace mhd {
@@ -278,11 +280,13 @@ __device__ __forceinline__ void flux_mag(const A& a, const Phys& c, const PointM
     const Ops<T> op = make_ops<T>(c, m);
     const T RI = a.ri(0, 0, 0);
     {
-        // The reference recomputes D1x/D1y/D1z of u, v, w here (:548-570);
-        // so does this code, behind a compiler memory fence, instead of
-        // keeping the values of the hydro part live (the register budget of
-        // the fused kernel).  Same operands, same results.
-        asm volatile("" ::: "memory");
+        // The reference recomputes D1x/D1y/D1z of u, v, w here (:548-570).
+        // In FP32 (two waves per SIMD) a compiler memory fence forces the
+        // recomputation instead of keeping the hydro part's values live;
+        // the FP64 kernel runs one wave per SIMD and has the accumulation
+        // registers to keep them (measured faster without the fence).  Same
+        // operands, same results either way.
+        if constexpr (!std::is_same<T, double>::value) asm volatile("" ::: "memory");
         const T dxu = op.d1x(a.u(1, 0, 0), a.u(-1, 0, 0));
         const T dxv = op.d1x(a.v(1, 0, 0), a.v(-1, 0, 0));
         const T dxw = op.d1x(a.w(1, 0, 0), a.w(-1, 0, 0));

@@ -57,7 +57,7 @@ struct mhd3d_ctx {
     // Defaults: the fastest variants measured on MI355X at 256x256x128
     // (launches sized to the resident block count).
     int tune_waves[2][2] = {{2, 1}, {2, 2}};
-    int tune_sched[2][2] = {{2, 0}, {2, 2}};
+    int tune_sched[2][2] = {{2, 2}, {2, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
     int target_blocks = 0;          // work items of one streaming launch (z chunking);
                                     // 0: the resident blocks of the kernel variant
