@@ -82,7 +82,9 @@ typedef struct mhd3d_desc {
     double OMX, OMZ;
     double RM, ORM, OBETA;
     double TP, TB, TC;
-    double RLAX;        /* must be 0: relaxation (3dmhdsub.h:322-442) is not built  */
+    double RLAX;        /* must be 0: the relaxation of 3dmhdsub.h:322-442 is refused
+                           (the reference itself exits in its first fluxes() call
+                           of a run that starts at a TIMT multiple of 25, :427-439) */
     double TIMI;        /* initial simulation time (3dmhd.cpp:114)                   */
     /* AJACOBI (3dmhddata.h:20-22): lengths nPx+2, nPy+2, nPz+2 */
     const double *dxxdx, *d2xxdx2, *DDX;
@@ -90,6 +92,20 @@ typedef struct mhd3d_desc {
     const double *dzzdz, *d2zzdz2, *DDZ;
     /* CPEN (3dmhddata.h:32): length nPz+2 */
     const double *RKAPA, *DKAPA;
+    /* Off-default branches; read only when their flag asks for them.
+     * iD != 0 (density diffusion, 3dmhdsub.h:229-264): DH = par[55],
+     * DV = par[56] (3dmhd.cpp:449); iD == 2 also zMax = par[18] and ZEE. */
+    double DH, DV, zMax;
+    /* lRem wall temperatures that Static() leaves in TU, DZTU, DZTB
+     * (3dmhdphsic.h:590-592), used by BCON (3dmhdsub.h:783-858). */
+    double TU, DZTU, DZTB;
+    /* iTCon 2 heat loss (3dmhdsub.h:443-461) and the TP != 0 plume of iTCon
+     * 0/1 (:744-776): XP, YP, ZP = par[13..15], QFH = par[52], HH = par[54];
+     * ORZD = 1.0f/double(nPz-1) with the global nPz (:768). */
+    double XP, YP, ZP, QFH, HH, ORZD;
+    /* grid coordinates (3dmhddata.h:20-22): lengths nPx+2, nPy+2, nPz+2;
+     * needed by iD 2 (ZEE), iTCon 2 (all) and the plume (EXX, WYY). */
+    const double *EXX, *WYY, *ZEE;
 } mhd3d_desc;
 
 typedef struct mhd3d_ctx mhd3d_ctx;

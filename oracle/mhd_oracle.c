@@ -64,6 +64,9 @@ struct orc_ctx {
     /* AJACOBI + CPEN */
     double *dxxdx, *d2xxdx2, *DDX, *dyydy, *d2yydy2, *DDY;
     double *dzzdz, *d2zzdz2, *DDZ, *RKAPA, *DKAPA;
+    double *EXX, *WYY, *ZEE;        /* coordinates (off-default branches)   */
+    /* function-local statics of fluxes()/horizontal_mean (3dmhdsub.h:200,699) */
+    double *ROM, *TTM, *HRAD, *WWZ;
     /* CTIM / ITER */
     double DT, TIMT, TIMC, TIMI, UMACH;
     long long nIt;
@@ -144,6 +147,25 @@ static void f_d2_mul_az(orc_ctx* c, int op, int dim, rng_t r, double* t, const d
 {
     DIMSETUP(dim);
     LOOP_OP(op, r, t[i], D2EXPR(f) * g[i] * az[z] * sc);
+}
+
+static void f_d2_mul(orc_ctx* c, int op, int dim, rng_t r, double* t, const double* f,
+                     const double* g, double sc)
+{
+    DIMSETUP(dim);
+    LOOP_OP(op, r, t[i], D2EXPR(f) * g[i] * sc);
+}
+
+/* func_mul_extend / func_base_extend along z (3dmhdutils.h:173-187) */
+static void f_mul_ext(orc_ctx* c, int op, rng_t r, double* t, const double* a, const double* az,
+                      double sc)
+{
+    LOOP_OP(op, r, t[i], a[i] * az[z] * sc);
+}
+
+static void f_base_ext(orc_ctx* c, int op, rng_t r, double* t, const double* az, double sc)
+{
+    LOOP_OP(op, r, t[i], az[z] * sc);
 }
 
 static void f_mul_d1(orc_ctx* c, int op, int dim, rng_t r, double* t, const double* f,
@@ -228,6 +250,14 @@ orc_ctx* orc_create(const mhd3d_desc* d)
     c->DDZ = dup_arr(d->DDZ, c->nz);
     c->RKAPA = dup_arr(d->RKAPA, c->nz);
     c->DKAPA = dup_arr(d->DKAPA, c->nz);
+    c->EXX = dup_arr(d->EXX, c->nx);
+    c->WYY = dup_arr(d->WYY, c->ny);
+    c->ZEE = dup_arr(d->ZEE, c->nz);
+    c->ROM = dup_arr(NULL, c->nz);
+    c->TTM = dup_arr(NULL, c->nz);
+    c->HRAD = dup_arr(NULL, c->nz);
+    c->WWZ = dup_arr(NULL, c->nz);
+    c->d.EXX = c->d.WYY = c->d.ZEE = NULL;   /* the copies above are used */
     c->TIMI = d->TIMI;
     c->TIMT = d->TIMI;
     c->TIMC = 0.0;
@@ -241,7 +271,8 @@ void orc_destroy(orc_ctx* c)
                      c->FW, c->FR, c->FT, c->ZRU, c->ZRV, c->ZRW, c->ZRO, c->ZTT, c->WW1,
                      c->WW2, c->WW3, c->BX, c->BY, c->BZ, c->ZBX, c->ZBY, c->ZBZ, c->RI,
                      c->S1, c->S2, c->S3, c->S4, c->dxxdx, c->d2xxdx2, c->DDX, c->dyydy,
-                     c->d2yydy2, c->DDY, c->dzzdz, c->d2zzdz2, c->DDZ, c->RKAPA, c->DKAPA};
+                     c->d2yydy2, c->DDY, c->dzzdz, c->d2zzdz2, c->DDZ, c->RKAPA, c->DKAPA,
+                     c->EXX, c->WYY, c->ZEE, c->ROM, c->TTM, c->HRAD, c->WWZ};
     for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); ++k) free(all[k]);
     free(c);
 }
@@ -327,6 +358,29 @@ void orc_communicate(orc_ctx* c)
 }
 
 /* ------------------------------------------------------------------------ */
+/* horizontal_mean — 3dmhdsub.h:698-718 (nGrid 0, one y rank): the running    */
+/* sum over the plane interior, y outer, x inner, divided by float(nPx*nPy). */
+
+static void hmean(orc_ctx* c, double* varM, const double* var)
+{
+    const int nx = c->nx, ny = c->ny;
+    const double den = (float)(c->d.nPx * c->d.nPy);
+    for (int z = 0; z < c->nz; ++z) {
+        double w = 0;
+        for (int y = 1; y < ny - 1; y++)
+            for (int x = 1; x < nx - 1; x++) w += var[IX(z, y, x)];
+        varM[z] = w / den;
+    }
+}
+
+static double sq(double v) { return v * v; }   /* square() of 3dmhdutils.h:213 */
+
+/* -4.0f*log(2.0f): the reference's translation unit includes <math.h>
+ * (3dmhdphsic.h:11), whose C++ wrapper puts the float overload of log() in
+ * the global namespace, so log(2.0f) there is the float logarithm. */
+static double ln2f(void) { return (double)logf(2.0f); }
+
+/* ------------------------------------------------------------------------ */
 /* fluxes() — 3dmhdsub.h:194-696                                             */
 
 void orc_fluxes(orc_ctx* c)
@@ -360,6 +414,38 @@ void orc_fluxes(orc_ctx* c)
     }
     f_base(c, SUB, R, FR, WW1, 1);
 
+    /* density diffusion :229-264 */
+    if (d->iD != 0 || d->lRem) {
+        const size_t pl = (size_t)c->nx * c->ny;
+        if (!MUT(9)) {
+            memcpy(&RO[IX(0, 0, 0)], &RO[IX(2, 0, 0)], pl * sizeof(double));               /* :230-234 */
+            memcpy(&RO[IX(c->nz - 1, 0, 0)], &RO[IX(c->nz - 3, 0, 0)], pl * sizeof(double)); /* :235-239 */
+        }
+        hmean(c, c->ROM, RO);
+        f_base(c, SET, RE, WW1, RO, 1);
+        if (!MUT(10)) f_base_ext(c, SUB, RE, WW1, c->ROM, 1);
+    }
+    if (d->iD != 0) {
+        if (d->iD == 1) {
+            for (int z = 0; z < c->nz; z++) c->WWZ[z] = 1.0 / c->ROM[z];
+        } else {
+            const double SGM = 1.0;
+            const double CLN = -4.0 * ln2f() / SGM / SGM;
+            for (int z = 0; z < c->nz; z++)
+                c->WWZ[z] = exp(CLN * sq(c->ZEE[z])) + exp(CLN * sq(c->ZEE[z] - d->zMax));
+        }
+        if (d->DH != 0.0) {
+            f_d2(c, SET, DX, R, WW2, WW1, 1);
+            f_d2(c, SET, DY, R, WW3, WW1, 1);
+            f_mul_ext(c, ADD, R, FR, WW2, c->WWZ, d->DH * ORE);
+            f_mul_ext(c, ADD, R, FR, WW3, c->WWZ, d->DH * ORE);
+        }
+        if (d->DV != 0.0) {
+            f_d2(c, SET, DZ, R, WW2, WW1, 1);
+            f_mul_ext(c, ADD, R, FR, WW2, c->WWZ, d->DV * ORE);
+        }
+    }
+
     /* pressure, gravity :266-272 */
     f_base(c, SET, R, FW, RO, d->GRAV);
     f_mul2(c, SET, RE, WW1, RO, TT, 1);
@@ -383,16 +469,46 @@ void orc_fluxes(orc_ctx* c)
     f_mul_d1(c, SUB, DZ, R, FW, RW, WW);
 
     /* temperature advection and conduction :293-320 */
-    if (!shr) {
+    if (!shr || MUT(12)) {
         f_d1_mul(c, SET, DX, R, FT, TT, UU, -1);
         f_d1_mul(c, SUB, DY, R, FT, TT, VV, 1);
         f_d1_mul(c, SUB, DZ, R, FT, TT, WW, 1);
     }
+    if (!shr && d->lRem) {                                             /* :297-302 */
+        hmean(c, c->TTM, TT);
+        LOOP(RE, WW1[i] = TT[i] - c->TTM[z]);
+    }
     tmp = OCV / d->REPR;
-    f_d2_mul_az(c, ADD, DX, R, FT, TT, RI, c->RKAPA, tmp);
-    f_d2_mul_az(c, ADD, DY, R, FT, TT, RI, c->RKAPA, tmp);
-    f_d2_mul_az(c, ADD, DZ, R, FT, TT, RI, c->RKAPA, tmp);
-    f_d1_mul_az(c, ADD, DZ, R, FT, TT, RI, c->DKAPA, tmp);
+    if (d->lRem) {                                                     /* :304-313 */
+        f_d2_mul(c, ADD, DX, R, FT, WW1, RI, tmp);
+        f_d2_mul(c, ADD, DY, R, FT, WW1, RI, tmp);
+        f_d2_mul(c, ADD, DZ, R, FT, WW1, RI, tmp);
+        const double *TTM = c->TTM, *dzz = c->dzzdz;
+        for (int z = zl; z < zr; z++) {
+            c->HRAD[z] = (((TTM[z + 1] - 2.0 * TTM[z] + TTM[z - 1])) * d->h2z * dzz[z] * dzz[z] +
+                          (TTM[z + 1] - TTM[z - 1]) * d->hz * c->d2zzdz2[z]) * c->RKAPA[z];
+            if (!MUT(11)) c->HRAD[z] += (TTM[z + 1] - TTM[z - 1]) * d->hz * dzz[z] * c->DKAPA[z];
+        }
+        f_mul_ext(c, ADD, R, FT, RI, c->HRAD, tmp);
+    } else {                                                           /* :315-319 */
+        f_d2_mul_az(c, ADD, DX, R, FT, TT, RI, c->RKAPA, tmp);
+        f_d2_mul_az(c, ADD, DY, R, FT, TT, RI, c->RKAPA, tmp);
+        f_d2_mul_az(c, ADD, DZ, R, FT, TT, RI, c->RKAPA, tmp);
+        f_d1_mul_az(c, ADD, DZ, R, FT, TT, RI, c->DKAPA, tmp);
+    }
+    if (d->iTCon == 2) {                                               /* :443-461 */
+        const double HH = d->HH, CLN = -4.0 * ln2f() / HH / HH;
+        const double *EXX = c->EXX, *WYY = c->WYY, *ZEE = c->ZEE;
+        if (d->TC != 0.0) {
+            const double TIMT = c->TIMT;
+            LOOP(R, FT[i] -= d->TP * OCV * 0.5 * (1.0 + tanh(ln2f() / d->QFH * (TIMT - d->TC))) * (MUT(13) ? RO[i] : RI[i]) *
+                             exp(CLN * sq(EXX[x] - d->XP) / HH) * exp(CLN * sq(WYY[y] - d->YP) / HH) *
+                             exp(CLN * sq(ZEE[z] - d->ZP) / HH));
+        } else {
+            LOOP(R, FT[i] -= d->TP * OCV * RI[i] * exp(CLN * sq(EXX[x] - d->XP) / HH) *
+                             exp(CLN * sq(WYY[y] - d->YP) / HH) * exp(CLN * sq(ZEE[z] - d->ZP) / HH));
+        }
+    }
 
     /* viscous :462-472 */
     f_d2(c, ADD, DX, R, FU, UU, ORE * C43);
@@ -593,8 +709,8 @@ void orc_fluxes(orc_ctx* c)
 }
 
 /* ------------------------------------------------------------------------ */
-/* BCON() — 3dmhdsub.h:720-873 (supported: iTCon in {0,1,3,4} with TP=0 for  */
-/* iTCon 0/1, izCon in {0,1}, iBCon 0, lRem off)                             */
+/* BCON() — 3dmhdsub.h:720-873 (iTCon 0-4, the TP plume of iTCon 0/1, izCon  */
+/* 0/1, lRem and lShr walls, iBCon 0)                                         */
 
 void orc_bcon(orc_ctx* c)
 {
@@ -620,18 +736,42 @@ void orc_bcon(orc_ctx* c)
         }
     for (int y = yl; y < yr; y++)
         for (int x = xl; x < xr; x++) RW[IX(zl, y, x)] = 0.0;
-    if (d->iTCon == 0 || d->iTCon == 4) {                               /* :780-795 */
+    if (d->TP != 0.0 && (d->iTCon == 0 || d->iTCon == 1)) {             /* :750-776 */
+        const double TPR = d->TC != 0.0 ? d->TP + (1.0 - d->TP) * exp(-c->TIMT / d->TC) : d->TP;
+        const double HH = d->HH, CLN = -4.0 * ln2f() / HH / HH;
+        if (d->iTCon == 0) {
+            for (int y = yl; y < yr; y++)
+                for (int x = xl; x < xr; x++)
+                    TT[IX(zl, y, x)] = 1.0 - (1.0 - TPR) * exp(CLN * sq(c->EXX[x] - d->XP) / (MUT(14) ? HH : 1.0)) / HH *
+                                                 exp(CLN * sq(c->WYY[y] - d->YP)) / HH;
+        } else {
+            const double DZ = d->ORZD / c->dzzdz[zl];
+            for (int y = yl; y < yr; y++)
+                for (int x = xl; x < xr; x++)
+                    TT[IX(zl, y, x)] = C43 * TT[IX(zl + 1, y, x)] - C13 * TT[IX(zl + 2, y, x)] -
+                                       C23 * DZ * (d->THETA - (d->THETA - TPR) * exp(CLN * sq(c->EXX[x] - d->XP)) / HH *
+                                                                  exp(CLN * sq(c->WYY[y] - d->YP)) / HH);
+        }
+    } else if (d->iTCon == 0 || d->iTCon == 2 || d->iTCon == 4) {      /* :777-795 */
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) {
-                TT[IX(zl, y, x)] = 1.0;
-                if (d->iTCon == 4 && x < nx / 2) TT[IX(zl, y, x)] = 1.0 - d->TP;
+                if (d->lRem) {
+                    TT[IX(zl, y, x)] = d->TU;
+                } else {
+                    TT[IX(zl, y, x)] = 1.0;
+                    if (d->iTCon == 4 && x < nx / 2) TT[IX(zl, y, x)] = 1.0 - d->TP;
+                }
             }
     } else if (d->iTCon == 1) {                                          /* :796-809 */
         const double DZ = d->ORZF / c->dzzdz[zl];
         for (int y = yl; y < yr; y++)
-            for (int x = xl; x < xr; x++)
-                TT[IX(zl, y, x)] = C43 * TT[IX(zl + 1, y, x)] - C13 * TT[IX(zl + 2, y, x)] -
-                                   C23 * DZ * d->THETA;
+            for (int x = xl; x < xr; x++) {
+                if (d->lRem)
+                    TT[IX(zl, y, x)] = C43 * TT[IX(zl + 1, y, x)] - C13 * TT[IX(zl + 2, y, x)] + d->DZTU;
+                else
+                    TT[IX(zl, y, x)] = C43 * TT[IX(zl + 1, y, x)] - C13 * TT[IX(zl + 2, y, x)] -
+                                       C23 * DZ * d->THETA;
+            }
     }
     if (d->lMag && d->iBCon == 0)
         for (int y = yl; y < yr; y++)
@@ -655,6 +795,10 @@ void orc_bcon(orc_ctx* c)
     if (d->izCon == 0) {                                                 /* :842-847 */
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) TT[IX(zr - 1, y, x)] = d->TB;
+    } else if (d->lRem) {                                                /* :850-854 */
+        for (int y = yl; y < yr; y++)
+            for (int x = xl; x < xr; x++)
+                TT[IX(zr - 1, y, x)] = C43 * TT[IX(zr - 2, y, x)] - C13 * TT[IX(zr - 3, y, x)] + d->DZTB;
     } else {                                                             /* :848-861 */
         const double DZ = d->ORZF / c->dzzdz[zr - 1];
         for (int y = yl; y < yr; y++)
@@ -704,7 +848,10 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
         double DD = smin(DDY[y], DDZ[z]);
         if (xdim) DD = smin(DD, DDX[x]);
         WW1[i] = DD / sqrt(UU[i]);
-        WW2[i] = 0.5 * DD * DD * REPR * CV * RO[i] / RKAPA[z];
+        if (d->lRem)                                                    /* :78-82 */
+            WW2[i] = 0.5 * DD * DD * REPR * CV * RO[i] / ((MUT(15) ? 0.0 : 1.0) + RKAPA[z]);
+        else
+            WW2[i] = 0.5 * DD * DD * REPR * CV * RO[i] / RKAPA[z];
         WW3[i] = 0.375 * DD * DD * RE * RO[i];
         if (d->lMag) VV[i] = 0.5 * DD * DD * RM;
     });

@@ -9,7 +9,9 @@
  * files:
  *   3dmhdsub.h:31-153   step(): TSTEP/CFL, three Wray RK3 stages
  *   3dmhdsub.h:155-192  step_block_ZETA0 / ZETA / GAM
- *   3dmhdsub.h:194-696  fluxes() (default branches: iD=0, lRem=0, RLAX=0, iTCon!=2)
+ *   3dmhdsub.h:194-696  fluxes() incl. the off-default branches iD 1/2, lRem,
+ *                       lShr, iTCon 2 (RLAX is not restated: the product refuses it)
+ *   3dmhdsub.h:698-718  horizontal_mean (one y rank)
  *   3dmhdsub.h:720-873  BCON()
  *   3dmhdutils.h:45-239 operator templates and reductions
  *   3dmhdmpi.h:32-143   ghost refill, single domain (x, y periodic)

@@ -108,13 +108,37 @@ struct InlineSink {
     __device__ __forceinline__ void put(MagOp op, T v) { mag_apply(op, v, FU, FV, FW, FT); }
 };
 
+// Inputs of the off-default branches at one point (FP64 form only), formed
+// by the kernel from ExtArgs: the horizontal means of the neighbouring
+// planes, the plane's density-diffusion weight and mean-profile conduction,
+// the carried-over FT (lShr) and the heat-loss factors.
+struct ExtPt {
+    int flags = 0;
+    double rom[3] = {0, 0, 0};   // ROM at z-1, z, z+1
+    double ttm[3] = {0, 0, 0};   // TTM at z-1, z, z+1
+    double wwz = 0;              // density-diffusion weight of the plane (:244-252)
+    double cdh = 0, cdv = 0;     // DH*ORE, DV*ORE
+    double hrad = 0;             // HRAD of the plane (:322-325)
+    double ft_old = 0;           // lShr: FT before this call
+    double heat_amp = 0, hx = 0, hy = 0, hz = 0;   // iTCon 2 heat loss
+};
+
 // Hydrodynamic part of fluxes(): continuity, pressure and gravity,
 // advection, conduction, viscous terms, viscous heating, pressure work and
 // Coriolis (3dmhdsub.h:204-545).
-template <bool ROT, int WALL, typename T, bool DEV, typename A>
+//
+// EXT = true adds the off-default branches selected by e->flags (the FP64
+// point and wall kernels; the accessor gives RO and TT at any offset):
+// density diffusion of W = RO - ROM(z) into FR (iD), the lRem conduction of
+// TT - TTM(z), the carried-over FT of lShr (its advection, viscous-heating
+// and pressure-work terms are skipped, :293,477-538) and the heat loss of
+// iTCon 2, each at its place in the reference's statement order.
+template <bool ROT, int WALL, typename T, bool DEV, typename A, bool EXT = false>
 __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const PointMetric& m, T& FU,
-                                           T& FV, T& FW, T& FR, T& FT)
+                                           T& FV, T& FW, T& FR, T& FT, const ExtPt* e = nullptr)
 {
+    // lShr drops every FT term inside the reference's `#if !lShr` blocks
+    const bool heat_terms = !EXT || !(e->flags & EXT_SHR);
     const Ops<T> op = make_ops<T>(c, m);
     const T ORE = T(c.ORE), OCV = T(c.OCV);
     const T ORE_C43 = T(c.ORE_C43);
@@ -132,6 +156,35 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     else
         dzrw = op.d1z(a.rw(0, 0, 1), a.rw(0, 0, -1));
     FR -= dzrw;
+
+    // ---- density diffusion :229-264 (func_d2 of W = RO - ROM(z), then
+    // func_mul_extend: (d2 * wwz[z]) * (DH*ORE)).  The reference first copies
+    // the planes two in onto the z ghost planes of RO (:230-239), so on a
+    // wall W beyond the wall equals W two planes in.
+    if constexpr (EXT) {
+        if (e->flags & (EXT_DH | EXT_DV)) {
+            const double r0 = e->rom[1];
+            const double w0 = a.ro(0, 0, 0) - r0;
+            if (e->flags & EXT_DH) {
+                FR += (op.d2x(a.ro(1, 0, 0) - r0, w0, a.ro(-1, 0, 0) - r0) * e->wwz) * e->cdh;
+                FR += (op.d2y(a.ro(0, 1, 0) - r0, w0, a.ro(0, -1, 0) - r0) * e->wwz) * e->cdh;
+            }
+            if (e->flags & EXT_DV) {
+                double wp, wm;
+                if (WALL == 1) {
+                    wp = a.ro(0, 0, 1) - e->rom[2];
+                    wm = wp;
+                } else if (WALL == 2) {
+                    wm = a.ro(0, 0, -1) - e->rom[0];
+                    wp = wm;
+                } else {
+                    wp = a.ro(0, 0, 1) - e->rom[2];
+                    wm = a.ro(0, 0, -1) - e->rom[0];
+                }
+                FR += (op.d2z(wp, w0, wm) * e->wwz) * e->cdv;
+            }
+        }
+    }
 
     // ---- gravity and pressure :266-272
     if constexpr (DEV)
@@ -164,19 +217,36 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     const T dyT = op.d1y(a.tt(0, 1, 0), a.tt(0, -1, 0));
     T dzT = op.d1z(a.tt(0, 0, 1), a.tt(0, 0, -1));
     if constexpr (DEV) dzT += a.dzt0();
-    FT = (dxT * U0) * T(-1);
-    FT -= dyT * V0;
-    FT -= dzT * W0;
+    if (heat_terms) {
+        FT = (dxT * U0) * T(-1);
+        FT -= dyT * V0;
+        FT -= dzT * W0;
+    } else {
+        FT = e->ft_old;   // lShr: FT is never reset (the SET above is inside `#if !lShr`)
+    }
 
     // ---- conduction :315-319 (func_d2_mul_array_z: ((d2*g)*a[z])*s)
     tmp = T(c.OCV_REPR);
-    const T kap = T(m.kap);
-    FT += ((op.d2x(a.tt(1, 0, 0), T0, a.tt(-1, 0, 0)) * RI) * kap) * tmp;
-    FT += ((op.d2y(a.tt(0, 1, 0), T0, a.tt(0, -1, 0)) * RI) * kap) * tmp;
-    T d2zT = op.d2z(a.tt(0, 0, 1), T0, a.tt(0, 0, -1));
-    if constexpr (DEV) d2zT += a.d2zt0();
-    FT += ((d2zT * RI) * kap) * tmp;
-    FT += ((dzT * RI) * T(m.dkap)) * tmp;
+    if (EXT && (e->flags & EXT_REM)) {
+        // lRem :304-313: func_d2_mul of WW1 = TT - TTM(z) ((d2 * RO) * s, RO
+        // holding 1/RO), then func_mul_extend of HRAD: (RO * HRAD[z]) * s
+        const T m0 = e->ttm[1];
+        const T t0 = T0 - m0;
+        FT += (op.d2x(a.tt(1, 0, 0) - m0, t0, a.tt(-1, 0, 0) - m0) * RI) * tmp;
+        FT += (op.d2y(a.tt(0, 1, 0) - m0, t0, a.tt(0, -1, 0) - m0) * RI) * tmp;
+        FT += (op.d2z(a.tt(0, 0, 1) - e->ttm[2], t0, a.tt(0, 0, -1) - e->ttm[0]) * RI) * tmp;
+        FT += (RI * e->hrad) * tmp;
+    } else {
+        const T kap = T(m.kap);
+        FT += ((op.d2x(a.tt(1, 0, 0), T0, a.tt(-1, 0, 0)) * RI) * kap) * tmp;
+        FT += ((op.d2y(a.tt(0, 1, 0), T0, a.tt(0, -1, 0)) * RI) * kap) * tmp;
+        T d2zT = op.d2z(a.tt(0, 0, 1), T0, a.tt(0, 0, -1));
+        if constexpr (DEV) d2zT += a.d2zt0();
+        FT += ((d2zT * RI) * kap) * tmp;
+        FT += ((dzT * RI) * T(m.dkap)) * tmp;
+    }
+    // ---- embedded heat loss, iTCon 2 :443-461: FT -= (((amp*RO)*gx)*gy)*gz
+    if (EXT && (e->flags & EXT_HEAT)) FT -= (((e->heat_amp * RI) * e->hx) * e->hy) * e->hz;
 
     // ---- viscous :462-472
     FU += op.d2x(a.u(1, 0, 0), U0, a.u(-1, 0, 0)) * ORE_C43;
@@ -193,7 +263,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     const T dxu = op.d1x(a.u(1, 0, 0), a.u(-1, 0, 0));
     const T dxv = op.d1x(a.v(1, 0, 0), a.v(-1, 0, 0));
     const T dxw = op.d1x(a.w(1, 0, 0), a.w(-1, 0, 0));
-    {
+    if (heat_terms) {
         tmp = T(c.OCV_ORE);
         FT += ((RI * dxu) * dxu) * T(c.OCV_ORE_C43);
         FT += ((RI * dxv) * dxv) * tmp;
@@ -211,7 +281,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     }
     const T dyu = op.d1y(a.u(0, 1, 0), a.u(0, -1, 0));
     const T dyv = op.d1y(a.v(0, 1, 0), a.v(0, -1, 0));
-    {
+    if (heat_terms) {
         tmp = T(c.OCV_ORE);
         FT += ((RI * dyu) * dyu) * tmp;
         FT += ((RI * dyu) * dxv) * T(c.OCV_ORE_2);
@@ -221,7 +291,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     }
     // D1z(w) at the point and at x+-1, y+-1 (WW3 over FORRANGE_EXT_XY, :505)
     const T dzw = op.d1z(a.w(0, 0, 1), a.w(0, 0, -1));
-    {
+    if (heat_terms) {
         tmp = T(c.C43_OCV_ORE);
         FT += ((RI * dzw) * dzw) * tmp;
         FT += ((RI * dxu) * dzw) * (-tmp);
@@ -241,7 +311,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     const T dzu = op.d1z(a.u(0, 0, 1), a.u(0, 0, -1));
     const T dzv = op.d1z(a.v(0, 0, 1), a.v(0, 0, -1));
     const T dyw = op.d1y(a.w(0, 1, 0), a.w(0, -1, 0));
-    {
+    if (heat_terms) {
         tmp = T(c.OCV_ORE);
         FT += ((RI * dzu) * dzu) * tmp;
         FT += ((RI * dzv) * dzv) * tmp;
@@ -256,7 +326,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
         FW += op.d1x(dzu_xp, dzu_xm) * C13_ORE;
         FW += op.d1y(dzv_yp, dzv_ym) * C13_ORE;
     }
-    {
+    if (heat_terms) {
         FT += ((RI * dzu) * dxw) * T(c.OCV_ORE_2);
     }
 
@@ -429,12 +499,12 @@ __device__ __forceinline__ void flux_mag(const A& a, const Phys& c, const PointM
 
 // fluxes() of one point: the hydrodynamic part, then (lMag) the magnetic
 // part with its contributions applied in order.
-template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, typename A>
+template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, bool EXT = false, typename A>
 __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const PointMetric& m,
-                                           FluxOutT<T>& o)
+                                           FluxOutT<T>& o, const ExtPt* e = nullptr)
 {
     T FU, FV, FW, FR, FT;
-    flux_hydro<ROT, WALL, T, DEV>(a, c, m, FU, FV, FW, FR, FT);
+    flux_hydro<ROT, WALL, T, DEV, A, EXT>(a, c, m, FU, FV, FW, FR, FT, e);
     T B1 = T(0), B2 = T(0), B3 = T(0);
     if (MAG) {
         InlineSink<T> sink{FU, FV, FW, FT};

@@ -44,7 +44,20 @@ struct GAcc {
 };
 
 struct CflPart;
-struct DevScalars;
+
+// Scalars of one domain on the device (the CTIM/ITER globals of the
+// reference and the DT guard state).
+struct DevScalars {
+    double dt;          // DT of the current step
+    double timc;        // TIMC (3dmhdsub.h:151)
+    double dt_last;     // DT of the last completed step
+    double nit;         // nIt
+    double umach;
+    double wmin[4];
+    unsigned int nonfinite;
+    unsigned int halted;   // the DT guard stopped this domain
+    double dt_min;         // DT guard threshold (0: no guard)
+};
 
 // Background profiles of the mixed form (index k * nz + z).
 // FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
@@ -137,6 +150,8 @@ struct StageArgs {
     // writes one partial record per block at cfl_parts[cfl_base + block]
     CflPart* cfl_parts;
     int cfl_base;
+    // off-default branches (FP64 point and wall kernels only)
+    ExtArgs x;
 };
 
 // Forward: whether the DT guard has stopped the domain (defined below).
@@ -161,6 +176,44 @@ __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int
     pm.kap = m.RKAPA[zg];
     pm.dkap = m.DKAPA[zg];
     return pm;
+}
+
+// Off-default inputs of point (x, y, z) at field index i (ExtArgs -> ExtPt):
+// the horizontal means of planes z-1..z+1, the density-diffusion weight
+// wwz[z] (iD 1: 1.0f/ROM[z], iD 2: the host table, 3dmhdsub.h:243-252), the
+// lRem mean-profile term HRAD[z] from TTM (:322-325, same products), the
+// carried-over FT (lShr) and the heat-loss factors (iTCon 2).
+__device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int z, int64_t i, ExtPt& e)
+{
+    const ExtArgs& X = a.x;
+    e.flags = X.flags;
+    if (X.flags & (EXT_DH | EXT_DV)) {
+        e.rom[0] = X.means[z - 1];
+        e.rom[1] = X.means[z];
+        e.rom[2] = X.means[z + 1];
+        e.wwz = X.iD == 1 ? 1.0 / e.rom[1] : X.wwz2[z];
+        e.cdh = X.cdh;
+        e.cdv = X.cdv;
+    }
+    if (X.flags & EXT_REM) {
+        const double* t = X.means + a.g.nz;
+        const double tm = t[z - 1], t0 = t[z], tp = t[z + 1];
+        e.ttm[0] = tm;
+        e.ttm[1] = t0;
+        e.ttm[2] = tp;
+        const double dzz = a.m.dzzdz[z];
+        double h = (((tp - 2.0 * t0) + tm) * a.c.h2z * dzz * dzz + (tp - tm) * a.c.hz * a.m.d2zzdz2[z]) *
+                   a.m.RKAPA[z];
+        h += (tp - tm) * a.c.hz * dzz * a.m.DKAPA[z];
+        e.hrad = h;
+    }
+    if (X.flags & EXT_SHR) e.ft_old = X.ft[i];
+    if (X.flags & EXT_HEAT) {
+        e.heat_amp = X.heat_amp;
+        e.hx = X.heat_x[x + 1];
+        e.hy = X.heat_y[y + 1];
+        e.hz = X.heat_z[z];
+    }
 }
 
 template <typename T>
@@ -195,7 +248,7 @@ __device__ __forceinline__ void cfl_values(const Phys& c, const Metric& m, bool 
     double DD = ref_min(m.DDY[yg], m.DDZ[z]);
     if (c.xdim) DD = ref_min(DD, m.DDX[xg]);
     w[0] = DD / sqrt(UU);
-    w[1] = 0.5 * DD * DD * c.REPR * c.CV * RO / m.RKAPA[z];
+    w[1] = 0.5 * DD * DD * c.REPR * c.CV * RO / m.KCFL[z];
     w[2] = 0.375 * DD * DD * c.RE * RO;
     w[3] = mag ? 0.5 * DD * DD * c.RM : 1e99;
 }
@@ -277,7 +330,9 @@ __device__ __forceinline__ void cfl_block_reduce(CflAcc& a, CflPart* part, int b
 // Interior planes of one RK stage, one thread per point (baseline form).
 // Stage k: F = fluxes(Uin); Uout = Zin + GAM_k*DT*F; Z = Uout + ZETA_k*DT*F
 // (step_block_GAM / step_block_ZETA, 3dmhdsub.h:168-192; Zin = Uin for k = 1).
-template <bool MAG, bool ROT, int STAGE>
+// EXT: with the off-default branches of a.x (the only kernel that has them
+// between the walls).
+template <bool MAG, bool ROT, int STAGE, bool EXT = false>
 __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -287,12 +342,17 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
     if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz - 1) return;
     GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+    const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
     FluxOut o;
-    flux_point<MAG, ROT, 0, double>(acc, a.c, pm, o);
+    ExtPt e;
+    if constexpr (EXT) ext_point(a, x, y, z, i, e);
+    flux_point<MAG, ROT, 0, double, false, EXT>(acc, a.c, pm, o, &e);
+    if constexpr (EXT) {
+        if (a.x.flags & EXT_SHR) a.x.ft[i] = o.FT;
+    }
     const double DT = *a.dt;
     const double cg = a.c.GAM[STAGE - 1] * DT;
     const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
-    const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -305,6 +365,65 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// BCON() (3dmhdsub.h:720-873) of one wall column.  un[] holds the wall values
+// RU..TT[, BX..BZ] of the column (RO final); the *1, *2 arguments are the
+// values one and two planes inward; timt = TIMT of the step.  Covers lShr
+// (moving walls, :721-743,820-826), the iTCon 0/1 Gaussian plume with
+// TP != 0 (:744-776), iTCon 0/2/4, 1 and 3 and izCon 0/1 incl. the lRem wall
+// temperatures (:777-862), and BZ = 0 (iBCon 0).
+__device__ __forceinline__ void bcon_point(const Phys& c, const Metric& m, const ExtArgs& xa,
+                                           bool mag, int nPx, bool top, int x, int y, int z,
+                                           double timt, double ru1, double ru2, double rv1,
+                                           double rv2, double ro1, double ro2, double tt1,
+                                           double tt2, double* un)
+{
+    if (xa.lShr) {
+        const double RRT = 1.0, RRB = -1.0, RPP = 0.0;   // :722-726
+        un[0] = 0.0;
+        un[1] = (top ? RRT : RRB) * cos(RPP * timt) * un[3];
+    } else {
+        un[0] = un[3] * (c.C43 * ru1 / ro1 - c.C13 * ru2 / ro2);
+        un[1] = un[3] * (c.C43 * rv1 / ro1 - c.C13 * rv2 / ro2);
+    }
+    un[2] = 0.0;
+    if (top) {
+        if (xa.flags & EXT_PLUME) {
+            const double gx = xa.plume_x[x + 1], gy = xa.plume_y[y + 1], HH = xa.HH;
+            if (c.iTCon == 0) {
+                un[4] = 1.0 - (1.0 - xa.tpr) * gx / HH * gy / HH;
+            } else {
+                const double DZ = xa.ORZD / m.dzzdz[z];
+                un[4] = c.C43 * tt1 - c.C13 * tt2 - c.C23 * DZ * (c.THETA - (c.THETA - xa.tpr) * gx / HH * gy / HH);
+            }
+        } else if (c.iTCon == 0 || c.iTCon == 2 || c.iTCon == 4) {
+            if (xa.lRem) {
+                un[4] = xa.TU;
+            } else {
+                un[4] = 1.0;
+                if (c.iTCon == 4 && (x + 1) < (nPx + 2) / 2) un[4] = 1.0 - c.TP;
+            }
+        } else if (c.iTCon == 1) {
+            if (xa.lRem) {
+                un[4] = c.C43 * tt1 - c.C13 * tt2 + xa.DZTU;
+            } else {
+                const double DZ = c.ORZF / m.dzzdz[z];
+                un[4] = c.C43 * tt1 - c.C13 * tt2 - c.C23 * DZ * c.THETA;
+            }
+        }
+    } else {
+        if (c.izCon == 0) {
+            un[4] = c.TB;
+        } else if (xa.lRem) {
+            un[4] = c.C43 * tt1 - c.C13 * tt2 + xa.DZTB;
+        } else {
+            const double DZ = c.ORZF / m.dzzdz[z];
+            un[4] = c.C43 * tt1 - c.C13 * tt2 - c.C23 * DZ * c.THETA / m.RKAPA[z];
+        }
+    }
+    if (mag && c.iBCon == 0) un[7] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // One wall point of one RK stage: fluxes with the one-sided wall stencils, the
 // GAM update, BCON on the updated wall values, then the ZETA register; stage 3
 // adds the point to the CFL accumulator.  BCON reads the new values one and
@@ -314,22 +433,30 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
 // at stages 2-3; the update, BCON and the Z register in FP64; BCON reads the
 // new inward planes from the FP32 copy at stages 1-2 and from the FP64 state
 // at stage 3).
-template <typename T, bool MAG, bool ROT, int STAGE>
+template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false>
 __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, bool top, CflAcc& cfl)
 {
     constexpr bool F64 = std::is_same<T, double>::value;
+    static_assert(F64 || !EXT, "the off-default branches are FP64 only");
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
     const int z = top ? 1 : a.g.nPz;
     const int zi1 = top ? z + 1 : z - 1;   // first plane inward
     const int zi2 = top ? z + 2 : z - 2;   // second plane inward
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
+    const int64_t row = (int64_t)y * a.g.pitch + x;
+    const int64_t i = (int64_t)z * a.g.plane + row;
     FluxOutT<T> o;
     if constexpr (F64) {
         GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+        ExtPt e;
+        if constexpr (EXT) ext_point(a, x, y, z, i, e);
         if (top)
-            flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
+            flux_point<MAG, ROT, 1, double, false, EXT>(acc, a.c, pm, o, &e);
         else
-            flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
+            flux_point<MAG, ROT, 2, double, false, EXT>(acc, a.c, pm, o, &e);
+        if constexpr (EXT) {
+            if (a.x.flags & EXT_SHR) a.x.ft[i] = o.FT;
+        }
     } else if constexpr (STAGE == 1) {
         GAccDev<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
                             a.g.nz, a.bgf, a.bg64};
@@ -348,8 +475,6 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     const double DT = *a.dt;
     const double cg = a.c.GAM[STAGE - 1] * DT;
     const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
-    const int64_t row = (int64_t)y * a.g.pitch + x;
-    const int64_t i = (int64_t)z * a.g.plane + row;
     const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
     const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
     const int64_t fs = a.g.fs;
@@ -366,28 +491,10 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
         return double(a.Uout32[f * fs + j]) + bg_sub(a, f, zj);
     };
     const Phys& c = a.c;
-    const double ro1 = Un(3, i1, zi1), ro2 = Un(3, i2, zi2);
-    un[0] = un[3] * (c.C43 * Un(0, i1, zi1) / ro1 - c.C13 * Un(0, i2, zi2) / ro2);
-    un[1] = un[3] * (c.C43 * Un(1, i1, zi1) / ro1 - c.C13 * Un(1, i2, zi2) / ro2);
-    un[2] = 0.0;
-    if (top) {
-        if (c.iTCon == 0 || c.iTCon == 4) {
-            un[4] = 1.0;
-            if (c.iTCon == 4 && (x + 1) < (a.g.nPx + 2) / 2) un[4] = 1.0 - c.TP;
-        } else if (c.iTCon == 1) {
-            const double DZ = c.ORZF / a.m.dzzdz[z];
-            un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) - c.C23 * DZ * c.THETA;
-        }
-    } else {
-        if (c.izCon == 0) {
-            un[4] = c.TB;
-        } else {
-            const double DZ = c.ORZF / a.m.dzzdz[z];
-            un[4] = c.C43 * Un(4, i1, zi1) - c.C13 * Un(4, i2, zi2) -
-                    c.C23 * DZ * c.THETA / a.m.RKAPA[z];
-        }
-    }
-    if (MAG && c.iBCon == 0) un[7] = 0.0;
+    const double timt = (EXT && a.x.lShr) ? a.x.TIMI + a.ds->timc : 0.0;
+    bcon_point(c, a.m, a.x, MAG, a.g.nPx, top, x, y, z, timt, Un(0, i1, zi1), Un(0, i2, zi2),
+               Un(1, i1, zi1), Un(1, i2, zi2), Un(3, i1, zi1), Un(3, i2, zi2), Un(4, i1, zi1),
+               Un(4, i2, zi2), un);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         if (new64)
@@ -406,7 +513,7 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
 
 // Wall planes of one RK stage as a separate launch (used with the point
 // kernel; the streaming kernel handles the walls of its own columns).
-template <typename T, bool MAG, bool ROT, int STAGE>
+template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false>
 __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -416,7 +523,7 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
     // every thread reaches the block reduction below; threads outside the
     // domain only contribute the identity
     CflAcc cfl;
-    if (x < a.g.nPx && y < a.g.nPy) wall_point<T, MAG, ROT, STAGE>(a, x, y, top, cfl);
+    if (x < a.g.nPx && y < a.g.nPy) wall_point<T, MAG, ROT, STAGE, EXT>(a, x, y, top, cfl);
     if (STAGE == 3 && a.cfl_parts) {
         const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + b);
@@ -425,7 +532,9 @@ __global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
 
 // ---------------------------------------------------------------------------
 // Stand-alone fluxes() into F (all interior planes incl. walls).
-template <bool MAG, bool ROT>
+// (EXT: with the off-default branches; the carried-over FT of lShr is read,
+// not replaced.)
+template <bool MAG, bool ROT, bool EXT = false>
 __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __restrict__ F)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -434,14 +543,16 @@ __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __r
     if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz) return;
     GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
-    FluxOut o;
-    if (z == 1)
-        flux_point<MAG, ROT, 1, double>(acc, a.c, pm, o);
-    else if (z == a.g.nPz)
-        flux_point<MAG, ROT, 2, double>(acc, a.c, pm, o);
-    else
-        flux_point<MAG, ROT, 0, double>(acc, a.c, pm, o);
     const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
+    FluxOut o;
+    ExtPt e;
+    if constexpr (EXT) ext_point(a, x, y, z, i, e);
+    if (z == 1)
+        flux_point<MAG, ROT, 1, double, false, EXT>(acc, a.c, pm, o, &e);
+    else if (z == a.g.nPz)
+        flux_point<MAG, ROT, 2, double, false, EXT>(acc, a.c, pm, o, &e);
+    else
+        flux_point<MAG, ROT, 0, double, false, EXT>(acc, a.c, pm, o, &e);
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
 #pragma unroll
     for (int f = 0; f < NF; ++f) F[f * a.g.fs + i] = fsel(o, f);
@@ -450,7 +561,8 @@ __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __r
 // ---------------------------------------------------------------------------
 // BCON() on a register in place (3dmhdsub.h:720-873).
 template <bool MAG>
-__global__ void bcon_kernel(Geom g, Phys c, Metric m, double* __restrict__ U)
+__global__ void bcon_kernel(Geom g, Phys c, Metric m, ExtArgs xa, const DevScalars* ds,
+                            double* __restrict__ U)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -462,29 +574,17 @@ __global__ void bcon_kernel(Geom g, Phys c, Metric m, double* __restrict__ U)
     const int64_t i1 = (int64_t)(top ? z + 1 : z - 1) * g.plane + row;
     const int64_t i2 = (int64_t)(top ? z + 2 : z - 2) * g.plane + row;
     const int64_t fs = g.fs;
-    const double ro = U[3 * fs + i], ro1 = U[3 * fs + i1], ro2 = U[3 * fs + i2];
-    U[0 * fs + i] = ro * (c.C43 * U[0 * fs + i1] / ro1 - c.C13 * U[0 * fs + i2] / ro2);
-    U[1 * fs + i] = ro * (c.C43 * U[1 * fs + i1] / ro1 - c.C13 * U[1 * fs + i2] / ro2);
-    U[2 * fs + i] = 0.0;
-    if (top) {
-        if (c.iTCon == 0 || c.iTCon == 4) {
-            double t = 1.0;
-            if (c.iTCon == 4 && (x + 1) < (g.nPx + 2) / 2) t = 1.0 - c.TP;
-            U[4 * fs + i] = t;
-        } else if (c.iTCon == 1) {
-            const double DZ = c.ORZF / m.dzzdz[z];
-            U[4 * fs + i] = c.C43 * U[4 * fs + i1] - c.C13 * U[4 * fs + i2] - c.C23 * DZ * c.THETA;
-        }
-    } else {
-        if (c.izCon == 0) {
-            U[4 * fs + i] = c.TB;
-        } else {
-            const double DZ = c.ORZF / m.dzzdz[z];
-            U[4 * fs + i] = c.C43 * U[4 * fs + i1] - c.C13 * U[4 * fs + i2] -
-                            c.C23 * DZ * c.THETA / m.RKAPA[z];
-        }
-    }
-    if (MAG && c.iBCon == 0) U[7 * fs + i] = 0.0;
+    constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
+    double un[NF_MAG] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int f = 0; f < NF; ++f) un[f] = U[f * fs + i];
+    const double timt = xa.lShr ? xa.TIMI + ds->timc : 0.0;
+    bcon_point(c, m, xa, MAG, g.nPx, top, x, y, z, timt, U[i1], U[i2], U[fs + i1], U[fs + i2],
+               U[3 * fs + i1], U[3 * fs + i2], U[4 * fs + i1], U[4 * fs + i2], un);
+    U[0 * fs + i] = un[0];
+    U[1 * fs + i] = un[1];
+    U[2 * fs + i] = un[2];
+    U[4 * fs + i] = un[4];
+    if (MAG) U[7 * fs + i] = un[7];
 }
 
 // ---------------------------------------------------------------------------
@@ -511,19 +611,6 @@ __global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
     const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     cfl_block_reduce(acc, part, b);
 }
-
-// Device scalars of one context.
-struct DevScalars {
-    double dt;          // DT of the current step
-    double timc;        // TIMC (3dmhdsub.h:151)
-    double dt_last;     // DT of the last completed step
-    double nit;         // nIt
-    double umach;
-    double wmin[4];
-    unsigned int nonfinite;
-    unsigned int halted;   // the DT guard stopped this domain
-    double dt_min;         // DT guard threshold (0: no guard)
-};
 
 // The DT guard of the reference's driver (3dmhd.cpp:292): after a step whose
 // DT fell below dt_min the run halts -- here every later kernel of the
@@ -600,6 +687,31 @@ __global__ void set_guard_kernel(DevScalars* s, double dt_min)
 }
 
 __global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
+
+// horizontal_mean (3dmhdsub.h:698-718) of RO (blockIdx.y = 0) and TT (1) on
+// every plane of register U, into out[k * nz + z]: the sum over the plane's
+// interior in the reference's order -- y outer, x inner, one running double --
+// so it rounds exactly as there, divided by float(nPx*nPy).  A sum in that
+// order is one chain of dependent additions: a wave walks it with the same
+// addresses in every lane, one wave per plane, the planes in parallel.  For
+// RO the z ghost planes take the planes two in, which fluxes() copies there
+// before taking the mean (3dmhdsub.h:230-239).
+__global__ void __launch_bounds__(64) plane_mean_kernel(Geom g, const double* __restrict__ U,
+                                                        int want_ro, int want_tt,
+                                                        double* __restrict__ out)
+{
+    const int z = blockIdx.x, k = blockIdx.y;
+    if (k == 0 ? !want_ro : !want_tt) return;
+    int zs = z;
+    if (k == 0) zs = z == 0 ? 2 : (z == g.nz - 1 ? g.nz - 3 : z);
+    const double* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
+    double s = 0.0;
+    for (int y = 0; y < g.nPy; ++y) {
+        const double* __restrict__ r = p + (int64_t)y * g.pitch;
+        for (int x = 0; x < g.nPx; ++x) s += r[x];
+    }
+    if (threadIdx.x == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
+}
 
 // Non-finite scan of a register (interior incl. walls): one thread per
 // (x, y) column, looping over z.

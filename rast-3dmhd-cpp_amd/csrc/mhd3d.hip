@@ -59,6 +59,13 @@ struct mhd3d_ctx {
     int tune_waves[2][2] = {{2, 1}, {2, 2}};
     int tune_sched[2][2] = {{2, 2}, {2, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
+    // off-default branches (iD, lRem, lShr, iTCon 2, TP plume): FP64, evaluated
+    // by the point and wall kernels
+    ExtArgs xa{};
+    bool ext = false;
+    bool ext_timedep = false;       // heat loss or plume with TC != 0: factors of TIMT per step
+    double* ext_buf = nullptr;      // means [2 nz] | wwz2 [nz] | heat x, y, z | plume x, y
+    double* ft = nullptr;           // lShr: the FT register (field layout)
     int target_blocks = 0;          // work items of one streaming launch (z chunking);
                                     // 0: the resident blocks of the kernel variant
     int64_t bytes = 0;
@@ -92,6 +99,19 @@ static int fail(mhd3d_ctx* ctx, int code, const std::string& msg)
 
 // ---------------------------------------------------------------------------
 
+// The off-default branches a descriptor switches on (ExtArgs::flags).
+static int ext_flags(const mhd3d_desc* d)
+{
+    int f = 0;
+    if (d->iD != 0 && d->DH != 0.0) f |= EXT_DH;   // `#if DH != 0.0` (3dmhdsub.h:253)
+    if (d->iD != 0 && d->DV != 0.0) f |= EXT_DV;   // `#if DV != 0.0` (:259)
+    if (d->lRem) f |= EXT_REM;
+    if (d->lShr) f |= EXT_SHR;
+    if (d->iTCon == 2) f |= EXT_HEAT;
+    if ((d->iTCon == 0 || d->iTCon == 1) && d->TP != 0.0) f |= EXT_PLUME;
+    return f;
+}
+
 static int validate(const mhd3d_desc* d, std::string& why)
 {
     if (!d) { why = "null descriptor"; return MHD3D_EINVAL; }
@@ -103,14 +123,28 @@ static int validate(const mhd3d_desc* d, std::string& why)
         return MHD3D_EINVAL;
     }
     if (d->ixCon != 0 || d->iyCon != 0) { why = "fluxes: Non-periodic horizontal boundaries (3dmhdsub.h:208)"; return MHD3D_EUNSUPPORTED; }
-    if (d->lRem) { why = "lRem (Rempel relaxation) is not built"; return MHD3D_EUNSUPPORTED; }
-    if (d->lShr) { why = "lShr is not built on the GPU path"; return MHD3D_EUNSUPPORTED; }
-    if (d->iD != 0) { why = "density diffusion iD != 0 is not built"; return MHD3D_EUNSUPPORTED; }
-    if (d->RLAX != 0.0) { why = "RLAX relaxation is not built"; return MHD3D_EUNSUPPORTED; }
-    if (d->iTCon == 2) { why = "iTCon = 2 (embedded heat loss) is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->RLAX != 0.0) {
+        why = "RLAX relaxation is refused: the reference exits in it (\"yk not implement\", 3dmhdsub.h:427-439) "
+              "or rejects the wall combination (:344,416)";
+        return MHD3D_EUNSUPPORTED;
+    }
+    if (d->lShr && d->lRem) {
+        why = "lShr with lRem: the reference's lShr skips the TTM update lRem's conduction reads (3dmhdsub.h:293-303)";
+        return MHD3D_EUNSUPPORTED;
+    }
+    if (d->iD < 0 || d->iD > 2) { why = "iD must be 0, 1 or 2 (3dmhdsub.h:243-252)"; return MHD3D_EINVAL; }
     if (d->iTCon < 0 || d->iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
-    if ((d->iTCon == 0 || d->iTCon == 1) && d->TP != 0.0) { why = "iTCon 0/1 with TP != 0 (Gaussian plume wall) is not built"; return MHD3D_EUNSUPPORTED; }
     if (d->izCon != 0 && d->izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
+    const int ef = ext_flags(d);
+    if (ef && d->precision == MHD3D_MIXED) {
+        why = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) are built for the FP64 form only";
+        return MHD3D_EUNSUPPORTED;
+    }
+    if (((ef & (EXT_DH | EXT_DV)) && d->iD == 2 && !d->ZEE) ||
+        ((ef & EXT_HEAT) && (!d->EXX || !d->WYY || !d->ZEE)) || ((ef & EXT_PLUME) && (!d->EXX || !d->WYY))) {
+        why = "coordinate array (EXX, WYY or ZEE) missing";
+        return MHD3D_EINVAL;
+    }
     if (d->lMag && d->iBCon != 0) { why = "iBCon != 0 is not built"; return MHD3D_EUNSUPPORTED; }
     if (d->precision != MHD3D_FP64 && d->precision != MHD3D_MIXED) { why = "invalid precision"; return MHD3D_EINVAL; }
     return MHD3D_OK;
@@ -143,6 +177,107 @@ static Phys make_phys(const mhd3d_desc* d)
     return c;
 }
 
+// Factors of TIMT the off-default branches take for one step, formed on the
+// host with the reference's expressions: the heat-loss amplitude
+// TP*OCV*0.5f*(1+tanh(log(2.0f)/QFH*(TIMT-TC))) (TP*OCV when TC = 0,
+// 3dmhdsub.h:446-460) and the plume's TPR = TP + (1-TP)*exp(-TIMT/TC) (TP
+// when TC = 0, :752-757).  TIMT = TIMI + TIMC is constant within a step.
+static void step_factors(mhd3d_ctx* ctx, double TIMT)
+{
+    const mhd3d_desc& d = ctx->d;
+    ExtArgs& X = ctx->xa;
+    if (d.TC != 0.0) {
+        X.heat_amp = d.TP * d.OCV * 0.5f * (1.0f + std::tanh(std::log(2.0f) / d.QFH * (TIMT - d.TC)));
+        X.tpr = d.TP + (1.0f - d.TP) * std::exp(-TIMT / d.TC);
+    } else {
+        X.heat_amp = d.TP * d.OCV;
+        X.tpr = d.TP;
+    }
+}
+
+// Tables and registers of the off-default branches.  The factors that hold
+// exp() and log() of the coordinates are formed here on the host with the
+// reference's expressions (so with the same library function values): the
+// iD 2 weights (3dmhdsub.h:244-252), the iTCon 2 heat-loss Gaussians
+// exp(CLN*(EXX-XP)^2/HH) etc. (:443-461) and the plume Gaussians
+// exp(CLN*(EXX-XP)^2) (:758-772), CLN = -4.0f*log(2.0f)/HH/HH.  log(2.0f) is
+// the float logarithm: the reference's translation unit includes <math.h>
+// (3dmhdphsic.h:11) before these functions, which brings the float overload
+// into the global namespace.
+static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
+{
+    ExtArgs& X = ctx->xa;
+    X = ExtArgs{};
+    X.flags = ext_flags(d);
+    X.iD = d->iD;
+    X.lRem = d->lRem ? 1 : 0;
+    X.lShr = d->lShr ? 1 : 0;
+    X.TU = d->TU;
+    X.DZTU = d->DZTU;
+    X.DZTB = d->DZTB;
+    X.ORZD = d->ORZD;
+    X.HH = d->HH;
+    X.TIMI = d->TIMI;
+    X.cdh = d->DH * d->ORE;
+    X.cdv = d->DV * d->ORE;
+    ctx->ext = X.flags != 0;
+    if (!ctx->ext) return MHD3D_OK;
+    ctx->kernel_mode = 1;   // the branches are in the point kernel
+    ctx->ext_timedep = d->TC != 0.0 && (X.flags & (EXT_HEAT | EXT_PLUME));
+    const Geom& g = ctx->g;
+    const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;
+    // host image: means [2nz] | wwz2 [nz] | heat_x [nx] heat_y [ny] heat_z [nz] | plume_x [nx] plume_y [ny]
+    const size_t n = 2 * (size_t)nz + nz + nx + ny + nz + nx + ny;
+    std::vector<double> h(n, 0.0);
+    double* wwz2 = h.data() + 2 * nz;
+    double* hx = wwz2 + nz;
+    double* hy = hx + nx;
+    double* hz = hy + ny;
+    double* px = hz + nz;
+    double* py = px + nx;
+    auto sq = [](double v) { return v * v; };
+    if ((X.flags & (EXT_DH | EXT_DV)) && d->iD == 2) {
+        const double SGM = 1.0f;
+        const double CLN = -4.0f * std::log(2.0f) / SGM / SGM;
+        for (int z = 0; z < nz; ++z)
+            wwz2[z] = std::exp(CLN * sq(d->ZEE[z])) + std::exp(CLN * sq(d->ZEE[z] - d->zMax));
+    }
+    if (X.flags & EXT_HEAT) {
+        const double HH = d->HH;
+        const double CLN = -4.0f * std::log(2.0f) / HH / HH;
+        for (int x = 0; x < nx; ++x) hx[x] = std::exp(CLN * sq(d->EXX[x] - d->XP) / HH);
+        for (int y = 0; y < ny; ++y) hy[y] = std::exp(CLN * sq(d->WYY[y] - d->YP) / HH);
+        for (int z = 0; z < nz; ++z) hz[z] = std::exp(CLN * sq(d->ZEE[z] - d->ZP) / HH);
+    }
+    if (X.flags & EXT_PLUME) {
+        const double HH = d->HH;
+        const double CLN = -4.0f * std::log(2.0f) / HH / HH;
+        for (int x = 0; x < nx; ++x) px[x] = std::exp(CLN * sq(d->EXX[x] - d->XP));
+        for (int y = 0; y < ny; ++y) py[y] = std::exp(CLN * sq(d->WYY[y] - d->YP));
+    }
+    HIPCHK(ctx, hipMalloc(&ctx->ext_buf, n * sizeof(double)));
+    HIPCHK(ctx, hipMemcpy(ctx->ext_buf, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    ctx->bytes += (int64_t)(n * sizeof(double));
+    double* b = ctx->ext_buf;
+    X.means = b;
+    X.wwz2 = b + 2 * nz;
+    X.heat_x = X.wwz2 + nz;
+    X.heat_y = X.heat_x + nx;
+    X.heat_z = X.heat_y + ny;
+    X.plume_x = X.heat_z + nz;
+    X.plume_y = X.plume_x + nx;
+    if (X.flags & EXT_SHR) {
+        // FT is a static array of the reference, zero at program start
+        const size_t bytes = (size_t)g.fs * sizeof(double);
+        HIPCHK(ctx, hipMalloc(&ctx->ft, bytes));
+        HIPCHK(ctx, hipMemset(ctx->ft, 0, bytes));
+        ctx->bytes += (int64_t)bytes;
+        X.ft = ctx->ft;
+    }
+    step_factors(ctx, d->TIMI);
+    return MHD3D_OK;
+}
+
 extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
 {
     if (!out) return MHD3D_EINVAL;
@@ -169,8 +304,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(MHD3D_EHIP, "hipStreamCreate failed");
     const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;
-    // metric block: dxxdx d2xxdx2 DDX | dyydy d2yydy2 DDY | dzzdz d2zzdz2 DDZ RKAPA DKAPA
-    const size_t nmet = 3 * (size_t)nx + 3 * (size_t)ny + 5 * (size_t)nz;
+    // metric block: dxxdx d2xxdx2 DDX | dyydy d2yydy2 DDY | dzzdz d2zzdz2 DDZ RKAPA DKAPA KCFL
+    const size_t nmet = 3 * (size_t)nx + 3 * (size_t)ny + 6 * (size_t)nz;
     std::vector<double> hm(nmet);
     size_t o = 0;
     auto put = [&](const double* src, int n) { std::memcpy(&hm[o], src, n * sizeof(double)); o += n; };
@@ -178,6 +313,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     put(desc->dyydy, ny); put(desc->d2yydy2, ny); put(desc->DDY, ny);
     put(desc->dzzdz, nz); put(desc->d2zzdz2, nz); put(desc->DDZ, nz);
     put(desc->RKAPA, nz); put(desc->DKAPA, nz);
+    // TSTEP conduction denominator: RKAPA[z], or 1.0f + RKAPA[z] with lRem (3dmhdsub.h:78-82)
+    for (int z = 0; z < nz; ++z) hm[o++] = desc->lRem ? 1.0f + desc->RKAPA[z] : desc->RKAPA[z];
     if (hipMalloc(&ctx->dmet, nmet * sizeof(double)) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(metrics) failed");
     if (hipMemcpy(ctx->dmet, hm.data(), nmet * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(MHD3D_EHIP, "metric upload failed");
@@ -186,6 +323,9 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     ctx->m.dyydy = p; p += ny; ctx->m.d2yydy2 = p; p += ny; ctx->m.DDY = p; p += ny;
     ctx->m.dzzdz = p; p += nz; ctx->m.d2zzdz2 = p; p += nz; ctx->m.DDZ = p; p += nz;
     ctx->m.RKAPA = p; p += nz; ctx->m.DKAPA = p; p += nz;
+    ctx->m.KCFL = p; p += nz;
+    rc = setup_ext(ctx, desc);
+    if (rc != MHD3D_OK) return cleanup(rc, ctx->err);
     ctx->mixed = desc->precision == MHD3D_MIXED;
     const size_t reg = (size_t)g.fs * ctx->nf * sizeof(double);
     const size_t reg32 = (size_t)g.fs * ctx->nf * sizeof(float);
@@ -265,6 +405,8 @@ extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
     (void)hipFree(ctx->F);
     (void)hipFree(ctx->ds);
     (void)hipFree(ctx->cfl_parts);
+    (void)hipFree(ctx->ext_buf);
+    (void)hipFree(ctx->ft);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -530,6 +672,7 @@ static StageArgs stage_args(mhd3d_ctx* ctx, int k = 0)
     }
     a.cfl_parts = nullptr;
     a.cfl_base = 0;
+    a.x = ctx->xa;
     return a;
 }
 
@@ -554,7 +697,10 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
                                                     ctx->tune_sched[fm][ph], ctx->target_blocks);
     } else if (ctx->kernel_mode == 1) {
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
-        stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+        if (ctx->ext)
+            stage_point_kernel<MAG, ROT, STAGE, true><<<grd, blk, 0, ctx->stream>>>(a);
+        else
+            stage_point_kernel<MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
     } else {
         launch_stream_stage<double, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
                                                      ctx->tune_sched[fm][ph], ctx->target_blocks);
@@ -569,6 +715,8 @@ static void launch_wall_t(mhd3d_ctx* ctx, const StageArgs& a)
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
     if (ctx->mixed)
         stage_wall_kernel<float, MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+    else if (ctx->ext)
+        stage_wall_kernel<double, MAG, ROT, STAGE, true><<<grd, blk, 0, ctx->stream>>>(a);
     else
         stage_wall_kernel<double, MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
 }
@@ -613,9 +761,26 @@ static int ensure_cfl_parts(mhd3d_ctx* ctx, int n)
     return MHD3D_OK;
 }
 
+// Horizontal means of the current register's RO and TT, when a branch
+// reads them (iD: ROM, lRem: TTM).
+static int run_means(mhd3d_ctx* ctx, const double* U)
+{
+    const int f = ctx->xa.flags;
+    const int want_ro = (f & (EXT_DH | EXT_DV)) ? 1 : 0, want_tt = (f & EXT_REM) ? 1 : 0;
+    if (!want_ro && !want_tt) return MHD3D_OK;
+    plane_mean_kernel<<<dim3(ctx->g.nz, 2, 1), 64, 0, ctx->stream>>>(ctx->g, U, want_ro, want_tt,
+                                                                     ctx->ext_buf);
+    HIPCHK(ctx, hipGetLastError());
+    return MHD3D_OK;
+}
+
 static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
 {
     StageArgs a = stage_args(ctx, k);
+    if (ctx->ext) {
+        int rc = run_means(ctx, a.Uin);
+        if (rc) return rc;
+    }
     if (fused_cfl && k == 3 && ctx->kernel_mode == 0) {
         int ns, nw;
         cfl_part_layout(ctx, ns, nw);
@@ -693,6 +858,13 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
     // only when no fused value for the current state exists.
     const bool fuse = !fixed_dt && ctx->kernel_mode == 0 && ctx->fuse_cfl;
     for (int n = 0; n < nsteps; ++n) {
+        if (ctx->ext_timedep) {
+            // the heat-loss / plume factors of this step need TIMT on the host
+            DevScalars hs;
+            HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            step_factors(ctx, ctx->d.TIMI + hs.timc);
+        }
         if (!fixed_dt && !(fuse && ctx->dt_ready)) {
             int rc = run_cfl(ctx);
             if (rc) return rc;
@@ -742,6 +914,12 @@ extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
         return fail(ctx, MHD3D_EUNSUPPORTED,
                     "mhd3d_stage: the mixed form keeps stage values in FP32 copies; use mhd3d_step");
     ctx->dt_ready = false;
+    if (ctx->ext_timedep) {
+        DevScalars hs;
+        HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        step_factors(ctx, ctx->d.TIMI + hs.timc);
+    }
     set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt);
     int rc = run_stage(ctx, stage);
     if (rc) return rc;
@@ -759,11 +937,24 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
         ctx->bytes += (int64_t)reg;
     }
     HIPCHK(ctx, hipMemsetAsync(ctx->F, 0, reg, ctx->stream));
+    if (ctx->ext_timedep) {
+        DevScalars hs;
+        HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        step_factors(ctx, ctx->d.TIMI + hs.timc);
+    }
     StageArgs a = stage_args(ctx);
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz);
     const bool mag = ctx->d.lMag, rot = ctx->d.lRot;
-    if (mag && rot) flux_only_kernel<true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+    if (ctx->ext) {
+        int rc = run_means(ctx, a.Uin);
+        if (rc) return rc;
+        if (mag && rot) flux_only_kernel<true, true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+        else if (mag) flux_only_kernel<true, false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+        else if (rot) flux_only_kernel<false, true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+        else flux_only_kernel<false, false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
+    } else if (mag && rot) flux_only_kernel<true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
     else if (mag) flux_only_kernel<true, false><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
     else if (rot) flux_only_kernel<false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
     else flux_only_kernel<false, false><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
@@ -789,13 +980,19 @@ extern "C" int mhd3d_bcon(mhd3d_ctx* ctx)
 {
     if (!ctx) return MHD3D_EINVAL;
     ctx->dt_ready = false;
+    if (ctx->ext_timedep) {
+        DevScalars hs;
+        HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        step_factors(ctx, ctx->d.TIMI + hs.timc);
+    }
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
     if (ctx->d.lMag)
-        bcon_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur]);
+        bcon_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
     else
-        bcon_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur]);
+        bcon_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
@@ -1109,6 +1306,10 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
         if (!descs[m]) return bail(MHD3D_EINVAL, "null descriptor");
         if ((descs[m]->precision == MHD3D_MIXED) != e->mixed)
             return bail(MHD3D_EINVAL, "ensemble members must share one precision form");
+        if (ext_flags(descs[m]))
+            return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) +
+                                                ": the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) "
+                                                "run on single domains only");
         mhd3d_ctx* c = nullptr;
         int rc = mhd3d_create(descs[m], &c);
         if (rc) return bail(rc, "member " + std::to_string(m) + ": " + g_create_error);
@@ -1551,9 +1752,14 @@ extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, cons
     int rc = validate(desc, why);
     if (rc) { s->err = why; return rc; }
     if (desc->precision != s->precision) { s->err = "request precision differs from the server's"; return MHD3D_EINVAL; }
+    if (ext_flags(desc)) {
+        s->err = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) run on single domains only";
+        return MHD3D_EUNSUPPORTED;
+    }
     ServerReq r;
     r.id = s->next_id++;
     r.d = *desc;
+    r.d.EXX = r.d.WYY = r.d.ZEE = nullptr;   // not read without the branches above
     const int nx = desc->nPx + 2, ny = desc->nPy + 2, nz = desc->nPz + 2;
     const double* src[11] = {desc->dxxdx, desc->d2xxdx2, desc->DDX, desc->dyydy, desc->d2yydy2, desc->DDY,
                              desc->dzzdz, desc->d2zzdz2, desc->DDZ, desc->RKAPA, desc->DKAPA};

@@ -63,6 +63,43 @@ struct Metric {
     const double* __restrict__ DDZ;
     const double* __restrict__ RKAPA;
     const double* __restrict__ DKAPA;
+    // denominator of the TSTEP conduction limit WW2 per plane: RKAPA[z], or
+    // 1.0f + RKAPA[z] with lRem (3dmhdsub.h:78-82), formed on the host
+    const double* __restrict__ KCFL;
+};
+
+// Off-default branches of fluxes(), BCON() and TSTEP that the point kernel
+// and the wall kernel evaluate (bits of ExtArgs::flags).
+enum : int {
+    EXT_DH = 1,      // iD != 0, DH != 0: horizontal density diffusion (3dmhdsub.h:255-260)
+    EXT_DV = 2,      // iD != 0, DV != 0: vertical density diffusion (:261-264)
+    EXT_REM = 4,     // lRem: conduction of TT - TTM plus the mean-profile term (:297-326)
+    EXT_SHR = 8,     // lShr: FT carried over, heating/advection terms skipped (:293,477-538)
+    EXT_HEAT = 16,   // iTCon == 2: embedded Gaussian heat loss (:443-461)
+    EXT_PLUME = 32,  // TP != 0 with iTCon 0/1: Gaussian top-wall temperature (:750-776)
+};
+
+// Operands of the off-default branches (device pointers, host-formed scalars).
+struct ExtArgs {
+    int flags = 0;
+    int iD = 0;
+    int lRem = 0, lShr = 0;
+    double cdh = 0.0, cdv = 0.0;          // DH*ORE, DV*ORE
+    // horizontal means of the stage input, [0, nz): RO (with the ghost planes
+    // replaced by the planes two in, 3dmhdsub.h:230-239), [nz, 2nz): TT
+    const double* __restrict__ means = nullptr;
+    const double* __restrict__ wwz2 = nullptr;    // iD == 2 weights per plane (:246-252)
+    double* __restrict__ ft = nullptr;            // lShr: FT of the last fluxes() call (field layout)
+    const double* __restrict__ heat_x = nullptr;  // iTCon 2: exp(CLN*(EXX-XP)^2/HH), per ghosted x
+    const double* __restrict__ heat_y = nullptr;  //   ... per ghosted y
+    const double* __restrict__ heat_z = nullptr;  //   ... per plane
+    double heat_amp = 0.0;                        //   TP*OCV[*0.5f*(1+tanh(..))] of this step
+    const double* __restrict__ plume_x = nullptr; // TP plume: exp(CLN*(EXX-XP)^2), per ghosted x
+    const double* __restrict__ plume_y = nullptr; //   ... per ghosted y
+    double tpr = 0.0;                             //   TPR of this step (3dmhdsub.h:752-757)
+    double TU = 0.0, DZTU = 0.0, DZTB = 0.0;      // lRem wall temperatures (3dmhdphsic.h:590-592)
+    double ORZD = 0.0;                            // 1.0f / double(nPz - 1) (3dmhdsub.h:768)
+    double HH = 0.0, TIMI = 0.0;
 };
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }

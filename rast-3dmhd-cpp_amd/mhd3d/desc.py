@@ -60,6 +60,12 @@ class MhdDesc(C.Structure):
         ("dzzdz", C.POINTER(C.c_double)), ("d2zzdz2", C.POINTER(C.c_double)),
         ("DDZ", C.POINTER(C.c_double)),
         ("RKAPA", C.POINTER(C.c_double)), ("DKAPA", C.POINTER(C.c_double)),
+        ("DH", C.c_double), ("DV", C.c_double), ("zMax", C.c_double),
+        ("TU", C.c_double), ("DZTU", C.c_double), ("DZTB", C.c_double),
+        ("XP", C.c_double), ("YP", C.c_double), ("ZP", C.c_double),
+        ("QFH", C.c_double), ("HH", C.c_double), ("ORZD", C.c_double),
+        ("EXX", C.POINTER(C.c_double)), ("WYY", C.POINTER(C.c_double)),
+        ("ZEE", C.POINTER(C.c_double)),
     ]
 
 
@@ -174,6 +180,13 @@ class RunConfig:
     zgrid: tuple = ("uniform",)   # ("uniform",) or ("tanh", beta)
     precision: int = 0
     TIMI: float = 0.0
+    # lRem: the wall temperatures Static() leaves in TU, DZTU, DZTB
+    # (3dmhdphsic.h:590-592) and, since set() leaves RKAPA/DKAPA to Static()
+    # under lRem (3dmhd.cpp:477), the conductivity profile (RKAPA, DKAPA)
+    TU: float = 0.0
+    DZTU: float = 0.0
+    DZTB: float = 0.0
+    kappa: tuple | None = None
 
 
 class Descriptor:
@@ -194,7 +207,7 @@ class Descriptor:
             setattr(d, k, v)
         self._arrays = {}
         for name in ("dxxdx", "d2xxdx2", "DDX", "dyydy", "d2yydy2", "DDY",
-                     "dzzdz", "d2zzdz2", "DDZ", "RKAPA", "DKAPA"):
+                     "dzzdz", "d2zzdz2", "DDZ", "RKAPA", "DKAPA", "EXX", "WYY", "ZEE"):
             a = np.ascontiguousarray(getattr(m, name), dtype=np.float64)
             self._arrays[name] = a
             setattr(d, name, a.ctypes.data_as(C.POINTER(C.c_double)))
@@ -259,6 +272,12 @@ def set_constants(cfg: RunConfig):
     c["TC"] = TC
     c["RLAX"] = par[59]
     c["TIMI"] = float(cfg.TIMI)
+    c["DH"], c["DV"] = par[55], par[56]                        # :449
+    c["zMax"] = zMax
+    c["TU"], c["DZTU"], c["DZTB"] = float(cfg.TU), float(cfg.DZTU), float(cfg.DZTB)
+    c["XP"], c["YP"], c["ZP"] = par[13], par[14], par[15]      # :414-416
+    c["QFH"], c["HH"] = par[52], par[54]                       # :424,426
+    c["ORZD"] = 1.0 / float(nPz_g - 1)                         # 3dmhdsub.h:768
 
     if cfg.nPx > 1:                                             # :461-468
         EXX, dxxdx, d2xxdx2, DDX = _uniform_axis(cfg.nPx, cfg.nPx, xMax, ghosts_zero=True)
@@ -281,6 +300,9 @@ def set_constants(cfg: RunConfig):
                           for i in range(nz)])
         DKAPA = np.array([(RKAPST - 1.0) / 2.0 / SIGMA / math.cosh((ZEE[i] - PZP) / SIGMA) ** 2
                           for i in range(nz)])
+    if cfg.kappa is not None:
+        RKAPA = np.asarray(cfg.kappa[0], dtype=np.float64).copy()
+        DKAPA = np.asarray(cfg.kappa[1], dtype=np.float64).copy()
     m = Metrics(EXX, dxxdx, d2xxdx2, DDX, WYY, dyydy, d2yydy2, DDY,
                 ZEE, dzzdz, d2zzdz2, DDZ, RKAPA, DKAPA)
     return m, c

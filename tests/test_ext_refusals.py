@@ -1,0 +1,68 @@
+"""Descriptor checks of the off-default branches (CPU: mhd3d_create validates
+the descriptor before its first HIP call, so a refused descriptor returns
+without a device).
+
+Refused: RLAX != 0 (the reference's relaxation exits, 3dmhdsub.h:427-439, or
+rejects the wall combination, :344,416), lShr with lRem (lShr skips the TTM
+update lRem reads, :293-303), the branches in the mixed form, iD outside
+0..2 and missing coordinate arrays.
+"""
+import ctypes as C
+import dataclasses
+import os
+
+import pytest
+
+from mhd3d import _lib as L
+from mhd3d.desc import Descriptor
+from mhd3d.initial import stratified_config
+
+
+def _create(cfg, null_zee=False):
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("libmhd3d.so not built")
+    desc = Descriptor(cfg)
+    if null_zee:
+        desc.struct.ZEE = C.POINTER(C.c_double)()
+    h = C.c_void_p()
+    rc = L.lib().mhd3d_create(C.cast(desc.ptr(), C.c_void_p), C.byref(h))
+    msg = L.lib().mhd3d_last_error(None)
+    assert not h.value
+    return rc, (msg or b"").decode()
+
+
+def _base(**kw):
+    return dataclasses.replace(stratified_config(12, 10, 8), **kw)
+
+
+def test_rlax_refused():
+    cfg = _base()
+    par = cfg.par.copy()
+    par[59] = 0.1
+    rc, msg = _create(dataclasses.replace(cfg, par=par))
+    assert rc == L.EUNSUPPORTED and "RLAX" in msg
+
+
+def test_lshr_with_lrem_refused():
+    rc, msg = _create(_base(lShr=True, lRem=True))
+    assert rc == L.EUNSUPPORTED and "lShr" in msg
+
+
+def test_mixed_form_refuses_branches():
+    cfg = _base(precision=1, iD=1)
+    par = cfg.par.copy()
+    par[55] = 0.5
+    rc, msg = _create(dataclasses.replace(cfg, par=par))
+    assert rc == L.EUNSUPPORTED and "FP64" in msg
+    rc, _ = _create(_base(precision=1, lShr=True))
+    assert rc == L.EUNSUPPORTED
+
+
+def test_bad_iD_and_missing_coordinates():
+    rc, _ = _create(_base(iD=3))
+    assert rc == L.EINVAL
+    cfg = _base(iD=2)
+    par = cfg.par.copy()
+    par[56] = 0.5
+    rc, msg = _create(dataclasses.replace(cfg, par=par), null_zee=True)
+    assert rc == L.EINVAL and "ZEE" in msg

@@ -4,11 +4,13 @@ Each mutant is the oracle rebuilt with -DORC_MUTANT=k (see the switch list in
 oracle/mhd_oracle.c): a flipped sign, a transposed operand, a dropped term, a
 derivative taken along the wrong axis, a wrong wall-stencil plane, a wrong
 BCON coefficient, a dropped factor in the CFL.  Every one must make at least
-one check of test_oracle_math.py fail, otherwise those checks would not pin
-what they claim to pin.  Runs on the CPU only.
+one check of test_oracle_math.py (mutants 1-8) or test_oracle_ext.py (the
+off-default branches, 9-15) fail, otherwise those checks would not pin what
+they claim to pin.  Runs on the CPU only.
 """
 import pytest
 
+import test_oracle_ext as te
 import test_oracle_math as tm
 from oracle import oracle as om
 
@@ -21,6 +23,13 @@ MUTANTS = {
     6: "bottom-wall one-sided D2z(BX) reading the wrong plane",
     7: "izCon=1 bottom temperature with C23 in place of C13",
     8: "CFL fast speed without GAMMA",
+    9: "density diffusion without the RO ghost-plane copy",
+    10: "density diffusion without the plane-mean removal",
+    11: "lRem mean-profile term without DKAPA",
+    12: "lShr resetting FT with the advection term",
+    13: "heat loss times RO instead of 1/RO",
+    14: "plume Gaussian with the heat-loss exponent",
+    15: "lRem CFL limit with RKAPA in place of 1 + RKAPA",
 }
 
 
@@ -33,6 +42,19 @@ def _checks():
         yield lambda iz=iz, it=it: tm.test_bcon_variants(iz, it)
     yield lambda: tm.test_cfl_advective_limit(False)
     yield lambda: tm.test_cfl_advective_limit(True)
+    for dh, dv in [(0.7, 0.0), (0.0, 0.9), (0.7, 0.9)]:
+        yield lambda dh=dh, dv=dv: te.test_density_diffusion_iD1(dh, dv)
+    yield te.test_density_diffusion_iD2_weights_are_gaussians
+    yield te.test_lrem_conduction_splits_mean_and_fluctuation
+    yield te.test_lrem_cfl_takes_one_plus_rkapa
+    yield lambda: te.test_lrem_walls_keep_a_linear_profile(1)
+    yield lambda: te.test_lrem_walls_keep_a_linear_profile(4)
+    yield te.test_lshr_carries_ft_and_keeps_only_conduction
+    yield te.test_lshr_walls_move
+    yield te.test_heat_loss_gaussian_and_amplitude
+    yield te.test_heat_loss_without_time_factor
+    yield lambda: te.test_plume_wall(0)
+    yield lambda: te.test_plume_wall(1)
 
 
 def test_clean_oracle_passes_every_check():
