@@ -18,11 +18,15 @@ namespace mhd {
 // State accessor that reads the register straight from global memory and
 // forms 1/RO, u, v, w and P on the fly (3dmhdsub.h:267-276).  TI is the type
 // the register is stored in, T the type the fluxes are evaluated in.
+// nz_ghost_copy != 0 (= nz): RO on a z ghost plane reads the plane two in,
+// as fluxes() copies it there under iD != 0 or lRem before forming P and
+// 1/RO (3dmhdsub.h:230-239).
 template <typename T, typename TI = T>
 struct GAcc {
     const TI* __restrict__ U;
     int64_t fs, plane;
     int pitch, nPx, nPy, x, y, z;
+    int nz_ghost_copy = 0;
     __device__ __forceinline__ T ld(int f, int dx, int dy, int dz) const
     {
         const int xx = wrapi(x + dx, nPx), yy = wrapi(y + dy, nPy);
@@ -31,7 +35,15 @@ struct GAcc {
     __device__ __forceinline__ T ru(int a, int b, int c) const { return ld(0, a, b, c); }
     __device__ __forceinline__ T rv(int a, int b, int c) const { return ld(1, a, b, c); }
     __device__ __forceinline__ T rw(int a, int b, int c) const { return ld(2, a, b, c); }
-    __device__ __forceinline__ T ro(int a, int b, int c) const { return ld(3, a, b, c); }
+    __device__ __forceinline__ T ro(int a, int b, int c) const
+    {
+        if (nz_ghost_copy) {
+            const int zz = z + c;
+            if (zz == 0) c += 2;
+            else if (zz == nz_ghost_copy - 1) c -= 2;
+        }
+        return ld(3, a, b, c);
+    }
     __device__ __forceinline__ T tt(int a, int b, int c) const { return ld(4, a, b, c); }
     __device__ __forceinline__ T bx(int a, int b, int c) const { return ld(5, a, b, c); }
     __device__ __forceinline__ T by(int a, int b, int c) const { return ld(6, a, b, c); }
@@ -449,7 +461,10 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     if constexpr (F64) {
         GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
         ExtPt e;
-        if constexpr (EXT) ext_point(a, x, y, z, i, e);
+        if constexpr (EXT) {
+            ext_point(a, x, y, z, i, e);
+            if (a.x.flags & EXT_ROCOPY) acc.nz_ghost_copy = a.g.nz;
+        }
         if (top)
             flux_point<MAG, ROT, 1, double, false, EXT>(acc, a.c, pm, o, &e);
         else
@@ -546,7 +561,10 @@ __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __r
     const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
     FluxOut o;
     ExtPt e;
-    if constexpr (EXT) ext_point(a, x, y, z, i, e);
+    if constexpr (EXT) {
+        ext_point(a, x, y, z, i, e);
+        if (a.x.flags & EXT_ROCOPY) acc.nz_ghost_copy = a.g.nz;
+    }
     if (z == 1)
         flux_point<MAG, ROT, 1, double, false, EXT>(acc, a.c, pm, o, &e);
     else if (z == a.g.nPz)

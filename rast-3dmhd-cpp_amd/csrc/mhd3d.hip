@@ -109,6 +109,7 @@ static int ext_flags(const mhd3d_desc* d)
     if (d->lShr) f |= EXT_SHR;
     if (d->iTCon == 2) f |= EXT_HEAT;
     if ((d->iTCon == 0 || d->iTCon == 1) && d->TP != 0.0) f |= EXT_PLUME;
+    if (d->iD != 0 || d->lRem) f |= EXT_ROCOPY;
     return f;
 }
 

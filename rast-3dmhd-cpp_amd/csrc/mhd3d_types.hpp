@@ -77,6 +77,7 @@ enum : int {
     EXT_SHR = 8,     // lShr: FT carried over, heating/advection terms skipped (:293,477-538)
     EXT_HEAT = 16,   // iTCon == 2: embedded Gaussian heat loss (:443-461)
     EXT_PLUME = 32,  // TP != 0 with iTCon 0/1: Gaussian top-wall temperature (:750-776)
+    EXT_ROCOPY = 64, // iD != 0 or lRem: RO ghost planes read as the planes two in (:230-239)
 };
 
 // Operands of the off-default branches (device pointers, host-formed scalars).
