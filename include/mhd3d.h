@@ -310,11 +310,26 @@ int mhd3d_load_param(const char* path, int idx, double* par1);
  * aligns with the top of the profile), TT times 1 + AMPT*(RAN2 - 1/2) drawn
  * in rank 0's loop order (IDUM = -62659), RU = RV = RW = 0; x/y ghosts
  * periodic.  par = the reference's par[64] (THETA = par[2], GRAV = par[3],
- * zMax = par[18], AMPT = par[19]).  fields: RU, RV, RW, RO, TT host arrays
- * of [nPEz*nRz+2][nPy+2][nPx+2].  lMag (tube() / AMPB layers) is not built:
- * MHD3D_EUNSUPPORTED.  Host-only; needs no GPU. */
+ * zMax = par[18], AMPT = par[19]).  fields: RU, RV, RW, RO, TT (and BX, BY,
+ * BZ with lMag) host arrays of [nPEz*nRz+2][nPy+2][nPx+2].  lMag: the
+ * AMPB Gaussian BX layer on each rank's planes 2..nRz+1 with RO lowered by
+ * BX^2/TT*OBETA (AMPB = par[20], BFH = par[21], BZP = par[22], OBETA =
+ * 2.0f/par[7]; 3dmhdphsic.h:682-699), or B = 0 (tube() with nTube = 0).
+ * Host-only; needs no GPU. */
 int mhd3d_static_state(int nPx, int nPy, int nPz_global, int nPEy, int nPEz, const double* par,
                        int lMag, double* const* fields);
+
+/* Static() with the lShr and lRem switches as well (3dmhdphsic.h:565-616,
+ * 737-742): lShr starts from T = RO = 1 and puts the RAN2 perturbation into
+ * RV (RV + (1 + AMPT*(RAN2 - 1/2)) - 1); lRem takes M. Rempel's background
+ * (getbackground/ODEINT/RKQC/RK4/derivs1/kappa, :357-522, REPR = par[0]*par[1])
+ * and also returns the conductivity profile it sets, RKAPA and DKAPA
+ * ([nPEz*nRz+2], the rank assembly of RK*8.07f*THETA*REPR and its one-sided /
+ * central differences), and walls = {TU, DZTU, DZTB} for BCON; RKAPA, DKAPA
+ * and walls may be NULL without lRem.  lShr with lRem: MHD3D_EUNSUPPORTED. */
+int mhd3d_static_state_flags(int nPx, int nPy, int nPz_global, int nPEy, int nPEz,
+                             const double* par, int lMag, int lShr, int lRem,
+                             double* const* fields, double* RKAPA, double* DKAPA, double* walls);
 
 /* Measurement support (no counterpart in the reference): `reps` launches of
  * a plain device copy of `bytes` bytes between two fresh buffers, to

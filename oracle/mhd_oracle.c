@@ -1114,3 +1114,119 @@ int orc_static_profile(int nPz, double THETA, double GRAV, double zMax, double* 
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* M. Rempel's background of Static() under lRem (3dmhdphsic.h:357-522,      */
+/* 565-592): getbackground integrates derivs1 from x = 2 with (T, P) = (1, 1) */
+/* down to each plane's height by ODEINT (RKQC step doubling of RK4).        */
+
+static double rp_kappa(double z, double rho) { return rho + (1.25f - rho) * exp(-5.0f * z); }
+
+static void rp_derivs(double x, const double* yt, double* dyt)
+{
+    const double nad = 0.41f, os = 1.0f;
+    double krad = rp_kappa(x, yt[1] / yt[0]);
+    double nrad = 0.4f / krad;
+    if (nrad <= os * nad) {
+        dyt[0] = nrad;
+        dyt[1] = yt[1] / yt[0];
+    } else {
+        dyt[0] = nad;
+        dyt[1] = yt[1] / yt[0];
+    }
+}
+
+static void rp_rk4(const double* Y, const double* DYDX, double X, double H, double* YOUT)
+{
+    double YT[2], DYT[2], DYM[2];
+    double HH = H * 0.5f, H6 = H / 6.0f, XH = X + HH;
+    for (int i = 0; i < 2; i++) YT[i] = Y[i] + HH * DYDX[i];
+    rp_derivs(XH, YT, DYT);
+    for (int i = 0; i < 2; i++) YT[i] = Y[i] + HH * DYT[i];
+    rp_derivs(XH, YT, DYM);
+    for (int i = 0; i < 2; i++) {
+        YT[i] = Y[i] + H * DYM[i];
+        DYM[i] = DYT[i] + DYM[i];
+    }
+    rp_derivs(X + H, YT, DYT);
+    for (int i = 0; i < 2; i++) YOUT[i] = Y[i] + H6 * (DYDX[i] + DYT[i] + 2.0f * DYM[i]);
+}
+
+static int rp_rkqc(double* Y, double* DYDX, double* X, double HTRY, double EPS, const double* YSCAL,
+                   double* HDID, double* HNEXT)
+{
+    const double FCOR = 1.0f / 15.0f, SAFETY = 0.9f, ERRCON = 6.0e-4f;
+    const double PGROW = -0.20f, PSHRNK = -0.25f;
+    double YTEMP[2], YSAV[2], DYSAV[2];
+    double XSAV = *X, H = HTRY;
+    for (int i = 0; i < 2; i++) YSAV[i] = Y[i], DYSAV[i] = DYDX[i];
+    for (;;) {
+        double HH = 0.5f * H;
+        rp_rk4(YSAV, DYSAV, XSAV, HH, YTEMP);
+        *X = XSAV + HH;
+        rp_derivs(*X, YTEMP, DYDX);
+        rp_rk4(YTEMP, DYDX, *X, HH, Y);
+        *X = XSAV + H;
+        if (*X == XSAV) return -1;
+        rp_rk4(YSAV, DYSAV, XSAV, H, YTEMP);
+        double ERRMAX = 0.0f;
+        for (int i = 0; i < 2; i++) {
+            YTEMP[i] = Y[i] - YTEMP[i];
+            ERRMAX = smax(ERRMAX, fabs(YTEMP[i] / YSCAL[i]));
+        }
+        ERRMAX = ERRMAX / EPS;
+        if (ERRMAX > 1.0) {
+            H = SAFETY * H * pow(ERRMAX, PSHRNK);
+        } else {
+            *HDID = H;
+            *HNEXT = ERRMAX > ERRCON ? SAFETY * H * pow(ERRMAX, PGROW) : 4.0f * H;
+            break;
+        }
+    }
+    for (int i = 0; i < 2; i++) Y[i] += YTEMP[i] * FCOR;
+    return 0;
+}
+
+static int rp_background(double x2, double* t, double* rho)
+{
+    const double TINY = 1.0e-30f, EPS = 1e-8, H1 = 1e-4, HMIN = 1e-50;
+    const double X1 = 2.0f;
+    double Y[2] = {1, 1}, DYDX[2], YSCAL[2];
+    double X = X1, H = (x2 > X1) ? H1 : -H1;
+    for (int n = 1; n <= 1000; ++n) {
+        rp_derivs(X, Y, DYDX);
+        for (int i = 0; i < 2; ++i) YSCAL[i] = fabs(Y[i]) + fabs(H * DYDX[i]) + TINY;
+        if ((X + H - x2) * (X + H - X1) > 0) H = x2 - X;
+        double HDID = 0, HNEXT = 0;
+        if (rp_rkqc(Y, DYDX, &X, H, EPS, YSCAL, &HDID, &HNEXT)) return -1;
+        if ((X - x2) * (x2 - X1) >= 0) {
+            *t = Y[0];
+            *rho = Y[1] / Y[0];
+            return 0;
+        }
+        if (fabs(HNEXT) < HMIN) return -1;
+        H = HNEXT;
+    }
+    return -1;
+}
+
+int orc_rempel_profile(int nPz, double THETA, double REPR, double zMax, double* T, double* R,
+                       double* RK, double* DRK, double walls[3])
+{
+    if (nPz < 3) return -1;
+    const double hz = (double)(0.5f * (float)(nPz - 1));
+    for (int k = 0; k < nPz; k++) {
+        double SZ = (double)((float)k / (float)(nPz - 1)) * zMax;     /* mkGrid, nGrid 0 */
+        if (rp_background(SZ, &T[k], &R[k])) return -1;
+        RK[k] = rp_kappa(SZ, R[k]);
+        RK[k] *= 8.07f * THETA * REPR;
+    }
+    const double SDZZDZ = 1.0f / zMax;
+    DRK[0] = (-3.0f * RK[0] + 4.0f * RK[1] - RK[2]) * hz * SDZZDZ;
+    for (int k = 1; k < nPz - 1; ++k) DRK[k] = (RK[k + 1] - RK[k - 1]) * hz * SDZZDZ;
+    DRK[nPz - 1] = (3.0f * RK[nPz - 1] - 4.0f * RK[nPz - 2] + RK[nPz - 3]) * hz * SDZZDZ;
+    walls[0] = T[0];
+    walls[1] = T[0] - 4.0f / 3.0f * T[1] + 1.0f / 3.0f * T[2];
+    walls[2] = T[nPz - 1] - 4.0f / 3.0f * T[nPz - 2] + 1.0f / 3.0f * T[nPz - 3];
+    return 0;
+}

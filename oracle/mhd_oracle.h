@@ -15,7 +15,8 @@
  *   3dmhdsub.h:720-873  BCON()
  *   3dmhdutils.h:45-239 operator templates and reductions
  *   3dmhdmpi.h:32-143   ghost refill, single domain (x, y periodic)
- *   3dmhdphsic.h:230-355,523-547,549-610  bsstep/mmid/rzextr/derivs, RAN2, Static
+ *   3dmhdphsic.h:230-355,523-547,549-763  bsstep/mmid/rzextr/derivs, RAN2, Static
+ *   3dmhdphsic.h:357-522  the lRem background (ODEINT/RKQC/RK4/derivs1/kappa)
  * The arrays are the reference's [nz][ny][nx] with a 1-cell ghost layer.
  * Domain = one z slab holding both walls: the one-sided wall stencils of both
  * walls apply (the reference's nPEz>=2 behaviour, 3dmhdsub.h:211-226,572-584).
@@ -66,6 +67,11 @@ int orc_static_profile(int nPz, double THETA, double GRAV, double zMax,
  * out = 1 + AMPT*(RAN2-0.5), drawn in the reference's rank order. */
 void orc_static_rnd(int nPx, int nPy, int nRz_total, int nPEy, int nPEz,
                     double AMPT, double* out);
+/* M. Rempel's background of Static() under lRem (3dmhdphsic.h:357-522,
+ * 565-592) on the uniform global grid: T[k], R[k], RK[k] = kappa*8.07f*THETA*
+ * REPR, DRK[k] and walls = {TU, DZTU, DZTB}.  Returns 0 or -1. */
+int orc_rempel_profile(int nPz, double THETA, double REPR, double zMax, double* T, double* R,
+                       double* RK, double* DRK, double walls[3]);
 
 #ifdef __cplusplus
 }

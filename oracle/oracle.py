@@ -62,6 +62,8 @@ def load(path):
     L.orc_static_profile.restype = C.c_int
     L.orc_static_profile.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
     L.orc_static_rnd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp]
+    L.orc_rempel_profile.restype = C.c_int
+    L.orc_rempel_profile.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp, dp]
     return L
 
 
@@ -151,6 +153,15 @@ def static_profile(nPz, THETA, GRAV, zMax):
     if lib().orc_static_profile(nPz, THETA, GRAV, zMax, _dp(T), _dp(R)) != 0:
         raise RuntimeError("bsstep failed")
     return T, R
+
+
+def rempel_profile(nPz, THETA, REPR, zMax):
+    """lRem background of Static(): T, R, RK, DRK and (TU, DZTU, DZTB)."""
+    T, R, RK, DRK = (np.zeros(nPz) for _ in range(4))
+    walls = np.zeros(3)
+    if lib().orc_rempel_profile(nPz, THETA, REPR, zMax, _dp(T), _dp(R), _dp(RK), _dp(DRK), _dp(walls)) != 0:
+        raise RuntimeError("ODEINT failed")
+    return T, R, RK, DRK, walls
 
 
 def static_rnd(nPx, nPy, nzg, nPEy, nPEz, AMPT):

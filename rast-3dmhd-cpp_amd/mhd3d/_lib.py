@@ -44,6 +44,8 @@ SIGNATURES = {
     "mhd3d_set_dt_guard": (C.c_int, [_vp, C.c_double]),
     "mhd3d_geometry": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "mhd3d_static_state": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dpp]),
+    "mhd3d_static_state_flags": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int,
+                                           C.c_int, C.c_int, _dpp, _dp, _dp, _dp]),
     "mhd3d_dump_data": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "mhd3d_load_data": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "mhd3d_dump_param": (C.c_int, [C.c_char_p, C.c_int, _dp]),

@@ -167,10 +167,13 @@ def stratified_state_hashed(desc: Descriptor, seed=12345, amp=1e-4, b0=1e-3, b_a
     return out
 
 
-def reference_static_state(cfg, nPz_global, nPEy=1, nPEz=1):
-    """The reference's Static() initial state (mhd3d_static_state): the run of
-    nPEy x nPEz ranks on a grid of nPz_global planes, assembled into one
-    domain of nPEz*(nPz_global//nPEz) planes (cfg.nPz must be that)."""
+def reference_static_state(cfg, nPz_global, nPEy=1, nPEz=1, with_profiles=False):
+    """The reference's Static() initial state (mhd3d_static_state_flags): the
+    run of nPEy x nPEz ranks on a grid of nPz_global planes, assembled into one
+    domain of nPEz*(nPz_global//nPEz) planes (cfg.nPz must be that), with the
+    lMag, lShr and lRem switches of cfg.  with_profiles: also return what
+    Static() sets besides the arrays under lRem, {"RKAPA", "DKAPA", "TU",
+    "DZTU", "DZTB"} (the descriptor's kappa and wall temperatures)."""
     import ctypes as C
     from . import _lib as L
 
@@ -178,11 +181,21 @@ def reference_static_state(cfg, nPz_global, nPEy=1, nPEz=1):
     if cfg.nPz != nPEz * nRz:
         raise ValueError(f"cfg.nPz must be {nPEz * nRz} for {nPEz} z ranks of {nPz_global} planes")
     shape = (cfg.nPz + 2, cfg.nPy + 2, cfg.nPx + 2)
-    arrs = [np.zeros(shape) for _ in range(5)]
+    nf = 8 if cfg.lMag else 5
+    arrs = [np.zeros(shape) for _ in range(nf)]
     par = np.ascontiguousarray(cfg.par, dtype=np.float64)
-    rc = L.lib().mhd3d_static_state(cfg.nPx, cfg.nPy, int(nPz_global), int(nPEy), int(nPEz),
-                                    par.ctypes.data_as(C.POINTER(C.c_double)), int(cfg.lMag),
-                                    L._fieldptrs(arrs))
+    rk = np.zeros(shape[0])
+    drk = np.zeros(shape[0])
+    walls = np.zeros(3)
+    dp = C.POINTER(C.c_double)
+    rc = L.lib().mhd3d_static_state_flags(cfg.nPx, cfg.nPy, int(nPz_global), int(nPEy), int(nPEz),
+                                          par.ctypes.data_as(dp), int(cfg.lMag), int(cfg.lShr),
+                                          int(cfg.lRem), L._fieldptrs(arrs), rk.ctypes.data_as(dp),
+                                          drk.ctypes.data_as(dp), walls.ctypes.data_as(dp))
     if rc != L.OK:
-        raise L.MhdError(rc, "mhd3d_static_state failed")
-    return dict(zip(("RU", "RV", "RW", "RO", "TT"), arrs))
+        raise L.MhdError(rc, "mhd3d_static_state_flags failed")
+    st = dict(zip(L.FIELDS[:nf], arrs))
+    if not with_profiles:
+        return st
+    prof = {"RKAPA": rk, "DKAPA": drk, "TU": walls[0], "DZTU": walls[1], "DZTB": walls[2]}
+    return st, prof

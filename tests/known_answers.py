@@ -53,21 +53,33 @@ def default_config(lRot=False):
                      lRot=lRot, izCon=1, iTCon=4, par=par)
 
 
-def default_state(cfg, oracle_mod, NP=NP, NPE_Y=NPE_Y, NPE_Z=NPE_Z):
+def default_state(cfg, oracle_mod, NP=NP, NPE_Y=NPE_Y, NPE_Z=NPE_Z, with_profiles=False):
     """Static() of 3dmhdphsic.h:549-763 for an NPE_Y x NPE_Z rank run over NP
     global planes (default: the 4x6 rank run), as one domain.
 
     Plane g of the domain sits on z rank g//NRZ (NRZ = NP//NPE_Z); ranks
     below the last take T[g], the last rank takes T[g + NP - NPE_Z*NRZ]
     (3dmhdphsic.h:642-666 skips T[210..213] in the default run).  The RND
-    perturbation follows the rank loop order of :709-744."""
+    perturbation follows the rank loop order of :709-744.  cfg.lShr: T = R = 1
+    and the perturbation goes to RV (:737-742); cfg.lRem: the oracle's Rempel
+    background, with the RKAPA/DKAPA assembly and TU/DZTU/DZTB returned when
+    with_profiles; cfg.lMag: the AMPB layer (:682-699) or B = 0."""
+    import math
+
     par = cfg.par
     nrz = NP // NPE_Z
-    T, R = oracle_mod.static_profile(NP, par[2], par[3], par[18])
+    prof = None
+    if cfg.lShr:
+        T, R = np.ones(NP), np.ones(NP)
+    elif cfg.lRem:
+        T, R, RK, DRK, walls = oracle_mod.rempel_profile(NP, par[2], par[0] * par[1], par[18])
+    else:
+        T, R = oracle_mod.static_profile(NP, par[2], par[3], par[18])
     nzg = NPE_Z * nrz
     shape = (nzg + 2, cfg.nPy + 2, cfg.nPx + 2)
     TT = np.zeros(shape)
     RO = np.zeros(shape)
+    kmap = []
     for z in range(nzg + 2):
         if z == 0:
             k = 0
@@ -76,9 +88,37 @@ def default_state(cfg, oracle_mod, NP=NP, NPE_Y=NPE_Y, NPE_Z=NPE_Z):
         else:
             g = z - 1
             k = g if g // nrz < NPE_Z - 1 else g + (NP - nzg)
+        kmap.append(k)
         TT[z, 1:-1, 1:-1] = T[k]
         RO[z, 1:-1, 1:-1] = R[k]
+    if cfg.lRem:
+        prof = {"RKAPA": np.array([RK[k] for k in kmap]), "DKAPA": np.array([DRK[k] for k in kmap]),
+                "TU": walls[0], "DZTU": walls[1], "DZTB": walls[2]}
+    out = {"RU": np.zeros(shape), "RV": np.zeros(shape), "RW": np.zeros(shape)}
+    if cfg.lMag:
+        B = [np.zeros(shape) for _ in range(3)]
+        AMPB, BFH, BZP = par[20], par[21], par[22]
+        if AMPB != 0.0:
+            f32 = np.float32
+            OBETA = 2.0 / par[7]
+            ORZ = float(f32(1.0) / f32(NP - 1))
+            CLN = float(f32(-4.0) * f32(math.log(2.0))) / (BFH * BFH)
+            for z in range(1, nzg + 2):
+                local = nrz + 1 if z == nzg + 1 else (z - 1) % nrz + 1
+                if local < 2:            # each rank's first plane keeps B = 0
+                    continue
+                zee = 1.0 * par[18] if z == nzg + 1 else (float(kmap[z]) * ORZ) * par[18]
+                d = zee - BZP
+                B[0][z] = AMPB * math.exp(CLN * (d * d))
+                inner = (z, slice(1, -1), slice(1, -1))
+                RO[inner] -= (B[0][inner] * B[0][inner] / TT[inner]) * OBETA
+        out.update(BX=B[0], BY=B[1], BZ=B[2])
     rnd = oracle_mod.static_rnd(cfg.nPx, cfg.nPy, nzg, NPE_Y, NPE_Z, par[19])
-    TT[1:-1, 1:-1, 1:-1] *= rnd[1:-1, 1:-1, 1:-1]
-    zero = np.zeros(shape)
-    return {"RU": zero, "RV": zero.copy(), "RW": zero.copy(), "RO": RO, "TT": TT}
+    if cfg.lShr:
+        inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+        out["RV"][inner] = (out["RV"][inner] + rnd[inner]) - 1.0
+    else:
+        TT[1:-1, 1:-1, 1:-1] *= rnd[1:-1, 1:-1, 1:-1]
+    out["RO"] = RO
+    out["TT"] = TT
+    return (out, prof) if with_profiles else out

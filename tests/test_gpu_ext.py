@@ -164,3 +164,35 @@ def test_ensemble_refuses_branches():
     h = C.c_void_p()
     rc = L.lib().mhd3d_ensemble_create(arr, 1, C.byref(h))
     assert rc == L.EUNSUPPORTED and not h.value
+
+
+@pytest.mark.parametrize("flag", ["lRem", "lShr", "lMag"])
+def test_from_reference_static(flag):
+    """A 2 x 3 rank run's Static() (product host code) with lRem, lShr or lMag
+    (AMPB layer), advanced 5 CFL steps on the GPU, against the oracle from
+    the oracle's Static(): bit for bit."""
+    from known_answers import default_config, default_state
+    from mhd3d.initial import reference_static_state
+    print(f"ext static {flag}", flush=True)
+    cfg = dataclasses.replace(default_config(), nPx=16, nPy=12, nPz=18, nPz_global=20, nPy_global=12,
+                              izCon=1, iTCon=1)
+    par = cfg.par.copy()
+    par[18] = 1.0
+    if flag == "lMag":
+        par[6], par[7] = 300.0, 1.0                 # RM, BETA
+        par[20], par[21], par[22] = 0.05, 0.3, 0.5  # AMPB, BFH, BZP
+    cfg = dataclasses.replace(cfg, par=par, **{flag: True})
+    got, prof = reference_static_state(cfg, 20, 2, 3, with_profiles=True)
+    ref, rprof = default_state(cfg, om, NP=20, NPE_Y=2, NPE_Z=3, with_profiles=True)
+    if flag == "lRem":
+        cfg = dataclasses.replace(cfg, TU=prof["TU"], DZTU=prof["DZTU"], DZTB=prof["DZTB"],
+                                  kappa=(prof["RKAPA"], prof["DKAPA"]))
+    desc = Descriptor(cfg)
+    dom = Domain(desc)
+    dom.set_state(got)
+    orc = om.Oracle(desc)
+    orc.set_state(ref)
+    orc.communicate()
+    dts = [orc.step() for _ in range(5)]
+    assert dom.step(5) == dts[-1]
+    _same(dom, orc, flag)
