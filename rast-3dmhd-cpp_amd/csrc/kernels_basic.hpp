@@ -706,14 +706,26 @@ __global__ void set_guard_kernel(DevScalars* s, double dt_min)
 
 __global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
 
+// Value of lane j (0 <= j < 64) of a double held by every lane of the wave.
+__device__ __forceinline__ double lane_value(double v, int j)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
 // horizontal_mean (3dmhdsub.h:698-718) of RO (blockIdx.y = 0) and TT (1) on
 // every plane of register U, into out[k * nz + z]: the sum over the plane's
 // interior in the reference's order -- y outer, x inner, one running double --
 // so it rounds exactly as there, divided by float(nPx*nPy).  A sum in that
-// order is one chain of dependent additions: a wave walks it with the same
-// addresses in every lane, one wave per plane, the planes in parallel.  For
-// RO the z ghost planes take the planes two in, which fluxes() copies there
-// before taking the mean (3dmhdsub.h:230-239).
+// order is one chain of dependent additions, walked by one wave per plane
+// (the planes in parallel): the wave loads 64 consecutive values of a row at
+// once (the next 64 already in flight) and adds them in x order from the
+// lanes that hold them.  (Values read with the same address in every lane go
+// to scalar registers, too few to keep enough loads in flight.)  For RO the z
+// ghost planes take the planes two in, which fluxes() copies there before
+// taking the mean (3dmhdsub.h:230-239).
 __global__ void __launch_bounds__(64) plane_mean_kernel(Geom g, const double* __restrict__ U,
                                                         int want_ro, int want_tt,
                                                         double* __restrict__ out)
@@ -723,12 +735,28 @@ __global__ void __launch_bounds__(64) plane_mean_kernel(Geom g, const double* __
     int zs = z;
     if (k == 0) zs = z == 0 ? 2 : (z == g.nz - 1 ? g.nz - 3 : z);
     const double* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
+    const int lane = threadIdx.x;
+    const int per_row = (g.nPx + 63) / 64;
+    const int nchunk = g.nPy * per_row;
+    auto load = [&](int c) -> double {
+        const int y = c / per_row, xx = (c - y * per_row) * 64 + lane;
+        return xx < g.nPx ? p[(int64_t)y * g.pitch + xx] : 0.0;
+    };
     double s = 0.0;
-    for (int y = 0; y < g.nPy; ++y) {
-        const double* __restrict__ r = p + (int64_t)y * g.pitch;
-        for (int x = 0; x < g.nPx; ++x) s += r[x];
+    double cur = load(0);
+    for (int c = 0; c < nchunk; ++c) {
+        const double nxt = c + 1 < nchunk ? load(c + 1) : 0.0;
+        const int y = c / per_row;
+        const int n = min(64, g.nPx - (c - y * per_row) * 64);
+        if (n == 64) {
+#pragma unroll
+            for (int j = 0; j < 64; ++j) s += lane_value(cur, j);
+        } else {
+            for (int j = 0; j < n; ++j) s += lane_value(cur, j);
+        }
+        cur = nxt;
     }
-    if (threadIdx.x == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
+    if (lane == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
 }
 
 // Non-finite scan of a register (interior incl. walls): one thread per

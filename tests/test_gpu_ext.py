@@ -178,6 +178,7 @@ def test_from_reference_static(flag):
                               izCon=1, iTCon=1)
     par = cfg.par.copy()
     par[18] = 1.0
+    par[11] = 0.0     # TP: with iTCon 1 a nonzero TP selects the plume wall (HH = 0 here)
     if flag == "lMag":
         par[6], par[7] = 300.0, 1.0                 # RM, BETA
         par[20], par[21], par[22] = 0.05, 0.3, 0.5  # AMPB, BFH, BZP
