@@ -18,10 +18,10 @@ namespace mhd {
 // State accessor that reads the register straight from global memory and
 // forms 1/RO, u, v, w and P on the fly (3dmhdsub.h:267-276).  TI is the type
 // the register is stored in, T the type the fluxes are evaluated in.
-// nz_ghost_copy != 0 (= nz): RO on a z ghost plane reads the plane two in,
-// as fluxes() copies it there under iD != 0 or lRem before forming P and
-// 1/RO (3dmhdsub.h:230-239).
-template <typename T, typename TI = T>
+// GHOST_COPY (the off-default kernels): with nz_ghost_copy != 0 (= nz), RO
+// on a z ghost plane reads the plane two in, as fluxes() copies it there
+// under iD != 0 or lRem before forming P and 1/RO (3dmhdsub.h:230-239).
+template <typename T, typename TI = T, bool GHOST_COPY = false>
 struct GAcc {
     const TI* __restrict__ U;
     int64_t fs, plane;
@@ -37,7 +37,7 @@ struct GAcc {
     __device__ __forceinline__ T rw(int a, int b, int c) const { return ld(2, a, b, c); }
     __device__ __forceinline__ T ro(int a, int b, int c) const
     {
-        if (nz_ghost_copy) {
+        if (GHOST_COPY && nz_ghost_copy) {
             const int zz = z + c;
             if (zz == 0) c += 2;
             else if (zz == nz_ghost_copy - 1) c -= 2;
@@ -378,58 +378,61 @@ __global__ void __launch_bounds__(256) stage_point_kernel(StageArgs a)
 
 // ---------------------------------------------------------------------------
 // BCON() (3dmhdsub.h:720-873) of one wall column.  un[] holds the wall values
-// RU..TT[, BX..BZ] of the column (RO final); the *1, *2 arguments are the
-// values one and two planes inward; timt = TIMT of the step.  Covers lShr
-// (moving walls, :721-743,820-826), the iTCon 0/1 Gaussian plume with
-// TP != 0 (:744-776), iTCon 0/2/4, 1 and 3 and izCon 0/1 incl. the lRem wall
-// temperatures (:777-862), and BZ = 0 (iBCon 0).
+// RU..TT[, BX..BZ] of the column (RO final); inward(f, k) gives field f one
+// (k = 1) or two (k = 2) planes inward, read only by the branches that need
+// it; timt = TIMT of the step.  Covers lShr (moving walls, :721-743,820-826),
+// the iTCon 0/1 Gaussian plume with TP != 0 (:744-776), iTCon 0/2/4, 1 and 3
+// and izCon 0/1 incl. the lRem wall temperatures (:777-862), and BZ = 0
+// (iBCon 0).  EXT = false: the default branches only (the off-default ones
+// compiled out of the default kernels).
+template <bool EXT, typename In>
 __device__ __forceinline__ void bcon_point(const Phys& c, const Metric& m, const ExtArgs& xa,
                                            bool mag, int nPx, bool top, int x, int y, int z,
-                                           double timt, double ru1, double ru2, double rv1,
-                                           double rv2, double ro1, double ro2, double tt1,
-                                           double tt2, double* un)
+                                           double timt, const In& inward, double* un)
 {
-    if (xa.lShr) {
+    if (EXT && xa.lShr) {
         const double RRT = 1.0, RRB = -1.0, RPP = 0.0;   // :722-726
         un[0] = 0.0;
         un[1] = (top ? RRT : RRB) * cos(RPP * timt) * un[3];
     } else {
-        un[0] = un[3] * (c.C43 * ru1 / ro1 - c.C13 * ru2 / ro2);
-        un[1] = un[3] * (c.C43 * rv1 / ro1 - c.C13 * rv2 / ro2);
+        const double ro1 = inward(3, 1), ro2 = inward(3, 2);
+        un[0] = un[3] * (c.C43 * inward(0, 1) / ro1 - c.C13 * inward(0, 2) / ro2);
+        un[1] = un[3] * (c.C43 * inward(1, 1) / ro1 - c.C13 * inward(1, 2) / ro2);
     }
     un[2] = 0.0;
     if (top) {
-        if (xa.flags & EXT_PLUME) {
+        if (EXT && (xa.flags & EXT_PLUME)) {
             const double gx = xa.plume_x[x + 1], gy = xa.plume_y[y + 1], HH = xa.HH;
             if (c.iTCon == 0) {
                 un[4] = 1.0 - (1.0 - xa.tpr) * gx / HH * gy / HH;
             } else {
                 const double DZ = xa.ORZD / m.dzzdz[z];
-                un[4] = c.C43 * tt1 - c.C13 * tt2 - c.C23 * DZ * (c.THETA - (c.THETA - xa.tpr) * gx / HH * gy / HH);
+                un[4] = c.C43 * inward(4, 1) - c.C13 * inward(4, 2) -
+                        c.C23 * DZ * (c.THETA - (c.THETA - xa.tpr) * gx / HH * gy / HH);
             }
         } else if (c.iTCon == 0 || c.iTCon == 2 || c.iTCon == 4) {
-            if (xa.lRem) {
+            if (EXT && xa.lRem) {
                 un[4] = xa.TU;
             } else {
                 un[4] = 1.0;
                 if (c.iTCon == 4 && (x + 1) < (nPx + 2) / 2) un[4] = 1.0 - c.TP;
             }
         } else if (c.iTCon == 1) {
-            if (xa.lRem) {
-                un[4] = c.C43 * tt1 - c.C13 * tt2 + xa.DZTU;
+            if (EXT && xa.lRem) {
+                un[4] = c.C43 * inward(4, 1) - c.C13 * inward(4, 2) + xa.DZTU;
             } else {
                 const double DZ = c.ORZF / m.dzzdz[z];
-                un[4] = c.C43 * tt1 - c.C13 * tt2 - c.C23 * DZ * c.THETA;
+                un[4] = c.C43 * inward(4, 1) - c.C13 * inward(4, 2) - c.C23 * DZ * c.THETA;
             }
         }
     } else {
         if (c.izCon == 0) {
             un[4] = c.TB;
-        } else if (xa.lRem) {
-            un[4] = c.C43 * tt1 - c.C13 * tt2 + xa.DZTB;
+        } else if (EXT && xa.lRem) {
+            un[4] = c.C43 * inward(4, 1) - c.C13 * inward(4, 2) + xa.DZTB;
         } else {
             const double DZ = c.ORZF / m.dzzdz[z];
-            un[4] = c.C43 * tt1 - c.C13 * tt2 - c.C23 * DZ * c.THETA / m.RKAPA[z];
+            un[4] = c.C43 * inward(4, 1) - c.C13 * inward(4, 2) - c.C23 * DZ * c.THETA / m.RKAPA[z];
         }
     }
     if (mag && c.iBCon == 0) un[7] = 0.0;
@@ -455,14 +458,12 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     const int zi1 = top ? z + 1 : z - 1;   // first plane inward
     const int zi2 = top ? z + 2 : z - 2;   // second plane inward
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
-    const int64_t row = (int64_t)y * a.g.pitch + x;
-    const int64_t i = (int64_t)z * a.g.plane + row;
     FluxOutT<T> o;
     if constexpr (F64) {
-        GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+        GAcc<double, double, EXT> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
         ExtPt e;
         if constexpr (EXT) {
-            ext_point(a, x, y, z, i, e);
+            ext_point(a, x, y, z, (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x, e);
             if (a.x.flags & EXT_ROCOPY) acc.nz_ghost_copy = a.g.nz;
         }
         if (top)
@@ -470,7 +471,7 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
         else
             flux_point<MAG, ROT, 2, double, false, EXT>(acc, a.c, pm, o, &e);
         if constexpr (EXT) {
-            if (a.x.flags & EXT_SHR) a.x.ft[i] = o.FT;
+            if (a.x.flags & EXT_SHR) a.x.ft[(int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x] = o.FT;
         }
     } else if constexpr (STAGE == 1) {
         GAccDev<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
@@ -490,6 +491,8 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     const double DT = *a.dt;
     const double cg = a.c.GAM[STAGE - 1] * DT;
     const double cz = STAGE < 3 ? a.c.ZETA[STAGE - 1] * DT : 0.0;
+    const int64_t row = (int64_t)y * a.g.pitch + x;
+    const int64_t i = (int64_t)z * a.g.plane + row;
     const int64_t i1 = (int64_t)zi1 * a.g.plane + row;
     const int64_t i2 = (int64_t)zi2 * a.g.plane + row;
     const int64_t fs = a.g.fs;
@@ -507,9 +510,8 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     };
     const Phys& c = a.c;
     const double timt = (EXT && a.x.lShr) ? a.x.TIMI + a.ds->timc : 0.0;
-    bcon_point(c, a.m, a.x, MAG, a.g.nPx, top, x, y, z, timt, Un(0, i1, zi1), Un(0, i2, zi2),
-               Un(1, i1, zi1), Un(1, i2, zi2), Un(3, i1, zi1), Un(3, i2, zi2), Un(4, i1, zi1),
-               Un(4, i2, zi2), un);
+    auto inward = [&](int f, int k) { return k == 1 ? Un(f, i1, zi1) : Un(f, i2, zi2); };
+    bcon_point<EXT>(c, a.m, a.x, MAG, a.g.nPx, top, x, y, z, timt, inward, un);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         if (new64)
@@ -528,8 +530,12 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
 
 // Wall planes of one RK stage as a separate launch (used with the point
 // kernel; the streaming kernel handles the walls of its own columns).
+// Latency-bound (two planes of one-sided stencils from global memory): built
+// for three waves per SIMD (FP64) or two (the mixed form's deviation
+// stencils), so the register allocator keeps the occupancy; the off-default
+// variant needs the registers of one wave.
 template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false>
-__global__ void __launch_bounds__(256) stage_wall_kernel(StageArgs a)
+__global__ void __launch_bounds__(256, EXT ? 1 : (sizeof(T) == 4 ? 2 : 3)) stage_wall_kernel(StageArgs a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -556,7 +562,7 @@ __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __r
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z + 1;   // planes 1 .. nPz
     if (x >= a.g.nPx || y >= a.g.nPy || z > a.g.nPz) return;
-    GAcc<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+    GAcc<double, double, EXT> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
     const int64_t i = (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x;
     FluxOut o;
@@ -578,7 +584,7 @@ __global__ void __launch_bounds__(256) flux_only_kernel(StageArgs a, double* __r
 
 // ---------------------------------------------------------------------------
 // BCON() on a register in place (3dmhdsub.h:720-873).
-template <bool MAG>
+template <bool MAG, bool EXT>
 __global__ void bcon_kernel(Geom g, Phys c, Metric m, ExtArgs xa, const DevScalars* ds,
                             double* __restrict__ U)
 {
@@ -595,9 +601,9 @@ __global__ void bcon_kernel(Geom g, Phys c, Metric m, ExtArgs xa, const DevScala
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
     double un[NF_MAG] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int f = 0; f < NF; ++f) un[f] = U[f * fs + i];
-    const double timt = xa.lShr ? xa.TIMI + ds->timc : 0.0;
-    bcon_point(c, m, xa, MAG, g.nPx, top, x, y, z, timt, U[i1], U[i2], U[fs + i1], U[fs + i2],
-               U[3 * fs + i1], U[3 * fs + i2], U[4 * fs + i1], U[4 * fs + i2], un);
+    const double timt = (EXT && xa.lShr) ? xa.TIMI + ds->timc : 0.0;
+    auto inward = [&](int f, int k) { return U[f * fs + (k == 1 ? i1 : i2)]; };
+    bcon_point<EXT>(c, m, xa, MAG, g.nPx, top, x, y, z, timt, inward, un);
     U[0 * fs + i] = un[0];
     U[1 * fs + i] = un[1];
     U[2 * fs + i] = un[2];

@@ -297,9 +297,9 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     };
     // DEFER: the update of plane z runs one iteration later, after the next
     // plane is published, with its Z operand loaded right after the flux
-    // evaluation -- the operand's load latency then overlaps the next
-    // plane's LDS work, and nothing but the 8 tendencies and the operand is
-    // carried through it.  Otherwise the operand of plane z+1 is prefetched
+    // evaluation (FP64: right before it) -- the operand's load latency then
+    // overlaps the next plane's LDS work, and nothing but the 8 tendencies
+    // and the operand is carried through it.  Otherwise the operand of plane z+1 is prefetched
     // one iteration ahead and plane z is updated at once.
     FluxOutT<T> fpend;
     double zpend[NF];
@@ -366,6 +366,16 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             zp = -1;
         }
         const int z = L - 1;
+        // FP64 with DEFER: the Z operand of plane z is loaded before its flux
+        // evaluation, whose duration then covers the load latency (measured
+        // faster in FP64; in FP32 the 2-wave register budget prefers the load
+        // after the flux)
+        constexpr bool ZEARLY = DEFER && ZLOAD && F64;
+        double zearly[NF];
+        if (ZEARLY && z >= zb) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) zearly[f] = zsrc[f * fs + (int64_t)z * g.plane + go];
+        }
         if (!DEFER && ZLOAD && z >= zb) {
             // this iteration updates plane z; fetch the operand of plane z+1
 #pragma unroll
@@ -420,7 +430,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
                 zp = z;
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
-                    zpend[f] = ZLOAD ? zsrc[f * fs + (int64_t)z * g.plane + go] : uc[f];
+                    zpend[f] = ZEARLY ? zearly[f] : (ZLOAD ? zsrc[f * fs + (int64_t)z * g.plane + go] : uc[f]);
             } else {
                 update(z, o, ZLOAD ? zcur : uc);
             }

@@ -990,10 +990,14 @@ extern "C" int mhd3d_bcon(mhd3d_ctx* ctx)
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
-    if (ctx->d.lMag)
-        bcon_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
-    else
-        bcon_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
+    const bool mag = ctx->d.lMag;
+    if (ctx->ext) {
+        if (mag) bcon_kernel<true, true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
+        else bcon_kernel<false, true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
+    } else {
+        if (mag) bcon_kernel<true, false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
+        else bcon_kernel<false, false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
+    }
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
