@@ -127,9 +127,9 @@ struct ExtPt {
 // advection, conduction, viscous terms, viscous heating, pressure work and
 // Coriolis (3dmhdsub.h:204-545).
 //
-// EXT = true adds the off-default branches selected by e->flags (the FP64
-// point and wall kernels; the accessor gives RO and TT at any offset):
-// density diffusion of W = RO - ROM(z) into FR (iD), the lRem conduction of
+// EXT = true adds the off-default branches selected by e->flags (FP64):
+// density diffusion of W = RO - ROM(z) into FR (iD; only with an accessor
+// that gives RO at the neighbours, A::kNeighbourRO), the lRem conduction of
 // TT - TTM(z), the carried-over FT of lShr (its advection, viscous-heating
 // and pressure-work terms are skipped, :293,477-538) and the heat loss of
 // iTCon 2, each at its place in the reference's statement order.
@@ -161,7 +161,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     // func_mul_extend: (d2 * wwz[z]) * (DH*ORE)).  The reference first copies
     // the planes two in onto the z ghost planes of RO (:230-239), so on a
     // wall W beyond the wall equals W two planes in.
-    if constexpr (EXT) {
+    if constexpr (EXT && A::kNeighbourRO) {
         if (e->flags & (EXT_DH | EXT_DV)) {
             const double r0 = e->rom[1];
             const double w0 = a.ro(0, 0, 0) - r0;

@@ -23,6 +23,7 @@ namespace mhd {
 // under iD != 0 or lRem before forming P and 1/RO (3dmhdsub.h:230-239).
 template <typename T, typename TI = T, bool GHOST_COPY = false>
 struct GAcc {
+    static constexpr bool kNeighbourRO = true;   // ro() at any offset
     const TI* __restrict__ U;
     int64_t fs, plane;
     int pitch, nPx, nPy, x, y, z;
@@ -91,6 +92,7 @@ __host__ __device__ __forceinline__ int bg64_slot(int f)
 // values flux_point<..., DEV = true> asks for.
 template <typename TI>
 struct GAccDev {
+    static constexpr bool kNeighbourRO = true;
     const TI* __restrict__ U;
     int64_t fs, plane;
     int pitch, nPx, nPy, x, y, z, nz;

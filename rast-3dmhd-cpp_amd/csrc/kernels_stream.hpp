@@ -122,6 +122,7 @@ __device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const
 // z-1, z, z+1).
 template <typename T, bool MAG>
 struct SAcc {
+    static constexpr bool kNeighbourRO = false;   // ro() is the centre value only
     static constexpr int NW = 5;               // RU RV RW TT P
     const T* st;                               // centre plane z, at this thread's centre
     const T* um;                               // u/v/w [BX BY BZ] ring, plane z-1, at centre
@@ -193,11 +194,12 @@ struct StreamLds {
 // one-sided stencils from global memory would raise this kernel's register
 // budget).  lds: StreamLds::bytes of shared memory.  Stage 3 accumulates the
 // CFL terms of every point it updates into cfl.
-template <typename T, bool MAG, bool ROT, int STAGE, bool ONE_BARRIER, bool DEFER>
+template <typename T, bool MAG, bool ROT, int STAGE, bool ONE_BARRIER, bool DEFER, bool EXT = false>
 __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int tile_y, int zb,
                                            int ze, char* lds, CflAcc& cfl)
 {
     constexpr bool F64 = std::is_same<T, double>::value;
+    static_assert(F64 || !EXT, "the off-default branches are FP64 only");
     constexpr int NR = MAG ? 6 : 3;   // ring fields: u v w [BX BY BZ]
     constexpr int NS = 5;             // centre-plane fields: RU RV RW TT P
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
@@ -410,7 +412,12 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             }
             const PointMetric pm = point_metric(a.m, gx + 1, gy + 1, z);
             FluxOutT<T> o;
-            flux_point<MAG, ROT, 0, T, !F64>(acc, a.c, pm, o);
+            ExtPt e;
+            if constexpr (EXT) ext_point(a, gx, gy, z, (int64_t)z * g.plane + go, e);
+            flux_point<MAG, ROT, 0, T, !F64, EXT>(acc, a.c, pm, o, &e);
+            if constexpr (EXT) {
+                if (valid && (a.x.flags & EXT_SHR)) a.x.ft[(int64_t)z * g.plane + go] = o.FT;
+            }
             // FP64 stage 1: Z = U, the centre values of plane z
             double uc[NF];
             if constexpr (!ZLOAD) {
@@ -446,7 +453,8 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
 // xcd_remap, consecutive tiles go to the same XCD (blocks are dealt to the 8
 // XCDs round-robin) so halo re-reads hit its L2.  Chunk k covers the planes
 // 2 + k*zchunk .. (the last chunk runs to nPz-1).
-template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER>
+template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER,
+          bool EXT = false>
 __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int nch,
                                                                   int ntx, int nty, int xcd_remap)
 {
@@ -462,7 +470,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     const int zb = 2 + chunk * zchunk;
     const int ze = chunk == nch - 1 ? a.g.nPz : zb + zchunk;
     CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
-    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER>(a, tile_x, tile_y, zb, ze, lds, cfl);
+    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER, EXT>(a, tile_x, tile_y, zb, ze, lds, cfl);
     if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
 }
 
@@ -550,6 +558,23 @@ inline int resident_blocks(const void* fn, int threads)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
         ncu = 256;
     return per_cu * ncu;
+}
+
+// The off-default branches that need no neighbouring RO (lRem, lShr, iTCon 2,
+// the plume walls) run in one FP64 variant: one wave per SIMD (room for
+// their per-point inputs), two barriers, deferred update.
+template <bool MAG, bool ROT>
+static const void* stream_ext_fn()
+{
+    return (const void*)stage_stream_kernel<double, MAG, ROT, 2, 1, false, true, true>;
+}
+
+template <bool MAG, bool ROT, int STAGE>
+static void launch_stream_ext(const StageArgs& a, hipStream_t s, int target_blocks)
+{
+    const StreamPlan p = stream_plan(a.g, target_blocks);
+    stage_stream_kernel<double, MAG, ROT, STAGE, 1, false, true, true><<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(
+        a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
 }
 
 template <typename T, bool MAG, bool ROT, int STAGE>

@@ -223,7 +223,9 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
     X.cdv = d->DV * d->ORE;
     ctx->ext = X.flags != 0;
     if (!ctx->ext) return MHD3D_OK;
-    ctx->kernel_mode = 1;   // the branches are in the point kernel
+    // density diffusion needs RO at the neighbours: the point kernel; the
+    // other branches run in the streaming kernel's off-default variant
+    if (X.flags & (EXT_DH | EXT_DV)) ctx->kernel_mode = 1;
     ctx->ext_timedep = d->TC != 0.0 && (X.flags & (EXT_HEAT | EXT_PLUME));
     const Geom& g = ctx->g;
     const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;
@@ -382,6 +384,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
         const void* fn;
         if (ctx->mixed)
             fn = desc->lMag ? stream_fn<float, true, true>(w, sc) : stream_fn<float, false, true>(w, sc);
+        else if (ctx->ext)
+            fn = desc->lMag ? stream_ext_fn<true, true>() : stream_ext_fn<false, true>();
         else
             fn = desc->lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
         ctx->target_blocks = resident_blocks(fn, STX * STY);
@@ -696,6 +700,8 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
     if (ctx->mixed) {
         launch_stream_stage<float, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
                                                     ctx->tune_sched[fm][ph], ctx->target_blocks);
+    } else if (ctx->ext && ctx->kernel_mode == 0) {
+        launch_stream_ext<MAG, ROT, STAGE>(a, ctx->stream, ctx->target_blocks);
     } else if (ctx->kernel_mode == 1) {
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
         if (ctx->ext)
