@@ -128,8 +128,8 @@ struct ExtPt {
 // Coriolis (3dmhdsub.h:204-545).
 //
 // EXT = true adds the off-default branches selected by e->flags (FP64):
-// density diffusion of W = RO - ROM(z) into FR (iD; only with an accessor
-// that gives RO at the neighbours, A::kNeighbourRO), the lRem conduction of
+// density diffusion of W = RO - ROM(z) into FR (iD; compiled only for an
+// accessor that gives RO at the neighbours, A::kNeighbourRO), the lRem conduction of
 // TT - TTM(z), the carried-over FT of lShr (its advection, viscous-heating
 // and pressure-work terms are skipped, :293,477-538) and the heat loss of
 // iTCon 2, each at its place in the reference's statement order.

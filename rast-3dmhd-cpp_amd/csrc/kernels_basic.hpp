@@ -714,55 +714,89 @@ __global__ void set_guard_kernel(DevScalars* s, double dt_min)
 
 __global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
 
-// Value of lane j (0 <= j < 64) of a double held by every lane of the wave.
-__device__ __forceinline__ double lane_value(double v, int j)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, j);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
 // horizontal_mean (3dmhdsub.h:698-718) of RO (blockIdx.y = 0) and TT (1) on
 // every plane of register U, into out[k * nz + z]: the sum over the plane's
 // interior in the reference's order -- y outer, x inner, one running double --
 // so it rounds exactly as there, divided by float(nPx*nPy).  A sum in that
 // order is one chain of dependent additions, walked by one wave per plane
-// (the planes in parallel): the wave loads 64 consecutive values of a row at
-// once (the next 64 already in flight) and adds them in x order from the
-// lanes that hold them.  (Values read with the same address in every lane go
-// to scalar registers, too few to keep enough loads in flight.)  For RO the z
-// ghost planes take the planes two in, which fluxes() copies there before
-// taking the mean (3dmhdsub.h:230-239).
-__global__ void __launch_bounds__(64) plane_mean_kernel(Geom g, const double* __restrict__ U,
-                                                        int want_ro, int want_tt,
-                                                        double* __restrict__ out)
+// (the planes in parallel).  The wave streams the plane's nPx*nPy values in
+// that order as chunks of 64 (lane l loads value 64c + l; a chunk may span
+// two rows), PM_G chunks per group with the next group's loads in flight,
+// stages each group in LDS and adds it from uniform-address LDS reads,
+// which reach every lane at once: the additions are the only serial work.
+// For RO the z ghost planes take the planes two in, which fluxes() copies
+// there before taking the mean (3dmhdsub.h:230-239).
+constexpr int PM_G = 8;
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt,
+                  double* __restrict__ out)
 {
+    __shared__ __align__(16) double buf[2][PM_G * 64];
     const int z = blockIdx.x, k = blockIdx.y;
     if (k == 0 ? !want_ro : !want_tt) return;
     int zs = z;
     if (k == 0) zs = z == 0 ? 2 : (z == g.nz - 1 ? g.nz - 3 : z);
     const double* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
     const int lane = threadIdx.x;
-    const int per_row = (g.nPx + 63) / 64;
-    const int nchunk = g.nPy * per_row;
-    auto load = [&](int c) -> double {
-        const int y = c / per_row, xx = (c - y * per_row) * 64 + lane;
-        return xx < g.nPx ? p[(int64_t)y * g.pitch + xx] : 0.0;
-    };
-    double s = 0.0;
-    double cur = load(0);
-    for (int c = 0; c < nchunk; ++c) {
-        const double nxt = c + 1 < nchunk ? load(c + 1) : 0.0;
-        const int y = c / per_row;
-        const int n = min(64, g.nPx - (c - y * per_row) * 64);
-        if (n == 64) {
-#pragma unroll
-            for (int j = 0; j < 64; ++j) s += lane_value(cur, j);
-        } else {
-            for (int j = 0; j < n; ++j) s += lane_value(cur, j);
+    const int nPx = g.nPx;
+    const int64_t N = (int64_t)nPx * g.nPy;
+    // (y, x) of this lane's next value f, advanced by 64 values per chunk
+    int64_t f = lane;
+    int y = lane / nPx, x = lane - (lane / nPx) * nPx;
+    const int qy = 64 / nPx, qx = 64 - qy * nPx;
+    auto next = [&]() -> double {
+        const double v = f < N ? p[(int64_t)y * g.pitch + x] : 0.0;
+        f += 64;
+        x += qx;
+        y += qy;
+        if (x >= nPx) {
+            x -= nPx;
+            ++y;
         }
-        cur = nxt;
+        return v;
+    };
+    double pre[PM_G];
+#pragma unroll
+    for (int d = 0; d < PM_G; ++d) pre[d] = next();
+    double s = 0.0;
+    const int64_t ngroups = (N + PM_G * 64 - 1) / (PM_G * 64);
+    for (int64_t gi = 0; gi < ngroups; ++gi) {
+        double* b = buf[gi & 1];
+#pragma unroll
+        for (int d = 0; d < PM_G; ++d) b[d * 64 + lane] = pre[d];
+#pragma unroll
+        for (int d = 0; d < PM_G; ++d) pre[d] = next();
+        // group gi+2 writes this buffer after the next barrier, which every
+        // read below precedes
+        __syncthreads();
+        const int64_t base = gi * (PM_G * 64);
+        if (base + PM_G * 64 <= N) {
+            // batches of 16 values: the reads of batch j+1 are issued before
+            // the additions of batch j, so LDS latency stays hidden
+            const double2* b2 = reinterpret_cast<const double2*>(b);
+            constexpr int NB = PM_G * 64 / 16;
+            double2 r[2][8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) r[0][i] = b2[i];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j + 1 < NB) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) r[(j + 1) & 1][i] = b2[(j + 1) * 8 + i];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    s += r[j & 1][i].x;
+                    s += r[j & 1][i].y;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+            const int n = (int)(N - base);
+            for (int j = 0; j < n; ++j) s += b[j];
+        }
     }
     if (lane == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
 }

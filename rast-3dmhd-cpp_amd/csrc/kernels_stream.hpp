@@ -118,11 +118,12 @@ __device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const
 }
 
 // LDS-backed accessor for an interior plane.  Centre-plane fields are
-// [RU, RV, RW, TT, P]; ws[f][k] is the own-column window (k = 0, 1, 2 for
-// z-1, z, z+1).
-template <typename T, bool MAG>
+// [RU, RV, RW, TT, P] and, with RO_N, RO; ws[f][k] is the own-column window
+// (k = 0, 1, 2 for z-1, z, z+1).  RO_N: ro() at the neighbours too (the
+// centre plane's RO from LDS, the own column's z-1 / z+1 from rz).
+template <typename T, bool MAG, bool RO_N = false>
 struct SAcc {
-    static constexpr bool kNeighbourRO = false;   // ro() is the centre value only
+    static constexpr bool kNeighbourRO = RO_N;
     static constexpr int NW = 5;               // RU RV RW TT P
     const T* st;                               // centre plane z, at this thread's centre
     const T* um;                               // u/v/w [BX BY BZ] ring, plane z-1, at centre
@@ -130,6 +131,7 @@ struct SAcc {
     const T* up;                               // plane z+1
     T ws[NW][3];
     T ro0, ri0;
+    T rz[3];
 
     __device__ __forceinline__ T star(int f, int dx, int dy, int dz) const
     {
@@ -153,7 +155,14 @@ struct SAcc {
     __device__ __forceinline__ T bx(int a, int b, int c) const { return ring(3, a, b, c); }
     __device__ __forceinline__ T by(int a, int b, int c) const { return ring(4, a, b, c); }
     __device__ __forceinline__ T bz(int a, int b, int c) const { return ring(5, a, b, c); }
-    __device__ __forceinline__ T ro(int, int, int) const { return ro0; }
+    __device__ __forceinline__ T ro(int dx, int dy, int dz) const
+    {
+        if constexpr (RO_N) {
+            if (dz != 0) return rz[dz + 1];
+            if (dx != 0 || dy != 0) return st[5 * SHP + dy * SHW + dx];
+        }
+        return ro0;
+    }
     __device__ __forceinline__ T ri(int, int, int) const { return ri0; }
     // deviation form (mixed): background terms of the centre plane
     T hb_ = T(0), dzt0_ = T(0), d2zt0_ = T(0), t0_ = T(0), rdev_ = T(0);
@@ -179,11 +188,11 @@ struct SAcc {
 // 256 VGPRs per wave; 1 lets the FP64 MHD variant use AGPRs instead of
 // spilling).
 // LDS of one work item: NRING planes of u, v, w [BX, BY, BZ] and NSTAR
-// centre planes of RU, RV, RW, TT, P.
-template <typename T, bool MAG, bool ONE_BARRIER>
+// centre planes of RU, RV, RW, TT, P [RO].
+template <typename T, bool MAG, bool ONE_BARRIER, bool RO_N = false>
 struct StreamLds {
     static constexpr int NR = MAG ? 6 : 3;
-    static constexpr int NS = 5;
+    static constexpr int NS = RO_N ? 6 : 5;
     static constexpr int NRING = ONE_BARRIER ? 4 : 3;
     static constexpr int NSTAR = ONE_BARRIER ? 3 : 2;
     static constexpr int bytes = (int)sizeof(T) * (NRING * NR + NSTAR * NS) * SHP;
@@ -201,7 +210,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     constexpr bool F64 = std::is_same<T, double>::value;
     static_assert(F64 || !EXT, "the off-default branches are FP64 only");
     constexpr int NR = MAG ? 6 : 3;   // ring fields: u v w [BX BY BZ]
-    constexpr int NS = 5;             // centre-plane fields: RU RV RW TT P
+    constexpr int NS = EXT ? 6 : 5;   // centre-plane fields: RU RV RW TT P [RO]
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
     // ring depths: NRING planes of u, v, w (BX, BY, BZ) and NSTAR centre
     // planes, deep enough that one barrier per plane suffices (a thread can be
@@ -250,7 +259,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         U = a.Uin32;
     const double* __restrict__ zsrc = (STAGE == 1) ? a.Uin : a.Z;
     const int64_t fs = g.fs;
-    SAcc<T, MAG> acc;
+    SAcc<T, MAG, EXT> acc;
     T row[3] = {T(0), T(0), T(0)};
     T rinv[3] = {T(0), T(0), T(0)};
     T rdv[3] = {T(0), T(0), T(0)};
@@ -324,6 +333,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             ss[2 * SHP + pc] = q.rw;
             ss[3 * SHP + pc] = q.tt;
             ss[4 * SHP + pc] = q.p;
+            if (EXT) ss[5 * SHP + pc] = q.ro;
             if (MAG) {
                 su[3 * SHP + pc] = q.bx;
                 su[4 * SHP + pc] = q.by;
@@ -331,7 +341,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             }
             const T nw[5] = {q.ru, q.rv, q.rw, q.tt, q.p};
 #pragma unroll
-            for (int k = 0; k < SAcc<T, MAG>::NW; ++k) {
+            for (int k = 0; k < SAcc<T, MAG, EXT>::NW; ++k) {
                 acc.ws[k][0] = acc.ws[k][1];
                 acc.ws[k][1] = acc.ws[k][2];
                 acc.ws[k][2] = nw[k];
@@ -350,6 +360,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
                 ss[2 * SHP + ph] = h.rw;
                 ss[3 * SHP + ph] = h.tt;
                 ss[4 * SHP + ph] = h.p;
+                if (EXT) ss[5 * SHP + ph] = h.ro;
                 if (MAG) {
                     su[3 * SHP + ph] = h.bx;
                     su[4 * SHP + ph] = h.by;
@@ -394,6 +405,11 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             acc.up = &s_uvw[(z + 1) % NRING][0][pc];
             acc.ro0 = row[1];
             acc.ri0 = rinv[1];
+            if constexpr (EXT) {
+                acc.rz[0] = row[0];
+                acc.rz[1] = row[1];
+                acc.rz[2] = row[2];
+            }
             if constexpr (!F64) {
                 const int nz = g.nz;
                 acc.rdev_ = rdv[1];
@@ -458,7 +474,7 @@ template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER
 __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int nch,
                                                                   int ntx, int nty, int xcd_remap)
 {
-    __shared__ __align__(16) char lds[StreamLds<T, MAG, ONE_BARRIER>::bytes];
+    __shared__ __align__(16) char lds[StreamLds<T, MAG, ONE_BARRIER, EXT>::bytes];
     if (domain_halted(a)) return;
     const int nb = gridDim.x;
     const int bid = blockIdx.x;
@@ -560,9 +576,9 @@ inline int resident_blocks(const void* fn, int threads)
     return per_cu * ncu;
 }
 
-// The off-default branches that need no neighbouring RO (lRem, lShr, iTCon 2,
-// the plume walls) run in one FP64 variant: one wave per SIMD (room for
-// their per-point inputs), two barriers, deferred update.
+// The off-default branches (iD, lRem, lShr, iTCon 2, the plume walls) run in
+// one FP64 variant: RO of the centre plane in LDS as well, one wave per SIMD
+// (room for the branches' per-point inputs), two barriers, deferred update.
 template <bool MAG, bool ROT>
 static const void* stream_ext_fn()
 {

@@ -223,9 +223,6 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
     X.cdv = d->DV * d->ORE;
     ctx->ext = X.flags != 0;
     if (!ctx->ext) return MHD3D_OK;
-    // density diffusion needs RO at the neighbours: the point kernel; the
-    // other branches run in the streaming kernel's off-default variant
-    if (X.flags & (EXT_DH | EXT_DV)) ctx->kernel_mode = 1;
     ctx->ext_timedep = d->TC != 0.0 && (X.flags & (EXT_HEAT | EXT_PLUME));
     const Geom& g = ctx->g;
     const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;

@@ -105,6 +105,32 @@ def test_ten_steps_cfl(case):
     assert n == n_c == 10 and timc == timc_c and timt == timt_c
 
 
+@pytest.mark.parametrize("case,n", [("iD1", (128, 16, 8)), ("lRem1", (128, 16, 8)),
+                                    ("iD2_mhd", (136, 12, 8)), ("lRem0", (72, 10, 8))])
+def test_plane_means_wide_planes(case, n):
+    """Planes of 2048 and 1632 values (whole groups of 512 summed by the
+    pipelined path, chunks of 64 that span two rows) and of 720 values (one
+    whole group and a rest): five CFL steps bit for bit."""
+    print(f"ext wide planes {case} {n}", flush=True)
+    desc, dom, orc = _pair(n, case, seed=11)
+    dts_c = [orc.step() for _ in range(5)]
+    assert dom.step(5) == dts_c[-1], case
+    _same(dom, orc, (case, n))
+
+
+@pytest.mark.parametrize("case", ["iD1", "iD2_mhd", "lRem1", "lShr", "heatT"])
+def test_point_kernel_override(case, monkeypatch):
+    """MHD3D_KERNEL=point (read when a context is created) runs the one-thread-
+    per-point stage kernel instead of the z-streaming one: the same ten CFL
+    steps bit for bit."""
+    print(f"ext point kernel {case}", flush=True)
+    monkeypatch.setenv("MHD3D_KERNEL", "point")
+    desc, dom, orc = _pair((24, 20, 12), case, seed=13)
+    dts_c = [orc.step() for _ in range(10)]
+    assert dom.step(10) == dts_c[-1], case
+    _same(dom, orc, case)
+
+
 @pytest.mark.parametrize("case", ["iD2_mhd", "lRem1", "lShr_mhd", "heatT", "plume0"])
 def test_stages_and_fluxes(case):
     print(f"ext stages {case}", flush=True)
