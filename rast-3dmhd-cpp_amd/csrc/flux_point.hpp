@@ -28,7 +28,14 @@
 // bfull(k) the full centre field and dzb(k), d2zb(k) the background's D1z and
 // D2z (one-sided at the walls for BX and BY, as the stencils below).
 #pragma once
+// FP64 arithmetic follows the reference's operation order, so nothing is
+// contracted into FMA; the translation unit of the FP32 (mixed-form) kernels
+// alone allows contraction within an expression.
+#ifdef MHD3D_FP32_FORM_TU
+#pragma clang fp contract(on)
+#else
 #pragma clang fp contract(off)
+#endif
 
 #include <type_traits>
 

@@ -6,7 +6,14 @@
 // reading the new values two planes inward that the interior stage kernel has
 // already written.
 #pragma once
+// FP64 arithmetic follows the reference's operation order, so nothing is
+// contracted into FMA; the translation unit of the FP32 (mixed-form) kernels
+// alone allows contraction within an expression.
+#ifdef MHD3D_FP32_FORM_TU
+#pragma clang fp contract(on)
+#else
 #pragma clang fp contract(off)
+#endif
 
 #include <type_traits>
 
@@ -688,7 +695,7 @@ __device__ __forceinline__ void cfl_finalize_block(const CflPart* __restrict__ p
     }
 }
 
-__global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __restrict__ part,
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __restrict__ part,
                                                            int nparts, DevScalars* s, Phys c,
                                                            int mag)
 {
@@ -698,21 +705,21 @@ __global__ void __launch_bounds__(256) cfl_finalize_kernel(const CflPart* __rest
 
 __device__ __forceinline__ bool domain_halted(const StageArgs& a) { return a.ds && a.ds->halted; }
 
-__global__ void set_dt_kernel(DevScalars* s, double dt) { s->dt = dt; }
+MHD_TU_LOCAL __global__ void set_dt_kernel(DevScalars* s, double dt) { s->dt = dt; }
 
-__global__ void clock_kernel(DevScalars* s)
+MHD_TU_LOCAL __global__ void clock_kernel(DevScalars* s)
 {
     if (s->halted) return;
     clock_and_guard(s);
 }
 
-__global__ void set_guard_kernel(DevScalars* s, double dt_min)
+MHD_TU_LOCAL __global__ void set_guard_kernel(DevScalars* s, double dt_min)
 {
     s->dt_min = dt_min;
     s->halted = 0u;
 }
 
-__global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
+MHD_TU_LOCAL __global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
 
 // horizontal_mean (3dmhdsub.h:698-718) of RO (blockIdx.y = 0) and TT (1) on
 // every plane of register U, into out[k * nz + z]: the sum over the plane's
@@ -728,7 +735,7 @@ __global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
 // there before taking the mean (3dmhdsub.h:230-239).
 constexpr int PM_G = 8;
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+MHD_TU_LOCAL __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt,
                   double* __restrict__ out)
 {
@@ -803,7 +810,7 @@ plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt
 
 // Non-finite scan of a register (interior incl. walls): one thread per
 // (x, y) column, looping over z.
-__global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,
                                                         const double* __restrict__ U,
                                                         DevScalars* s)
 {
@@ -821,7 +828,7 @@ __global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,
     if (bad) atomicOr(&s->nonfinite, 1u);
 }
 
-__global__ void clear_flag_kernel(DevScalars* s) { s->nonfinite = 0u; }
+MHD_TU_LOCAL __global__ void clear_flag_kernel(DevScalars* s) { s->nonfinite = 0u; }
 
 // Counter-based uniform deviate in [-1, 1) for (seed, field, z, y, x):
 // the splitmix64 finaliser of the packed key (x, y < 2^18, z < 2^20).
@@ -841,7 +848,7 @@ __host__ __device__ inline double hash_uniform(unsigned long long seed, int f, i
 // RU, RV, RW = RO * amp * u on the planes between the walls; with lMag
 // BX = b0 + b_amp*u, BY, BZ = b_amp*u on planes 1..nPz (BZ = 0 on the walls),
 // ghost planes copies of the walls.  u = hash_uniform(seed, field, z, y, x).
-__global__ void __launch_bounds__(256) init_stratified_kernel(Geom g, int nf, double* __restrict__ U,
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) init_stratified_kernel(Geom g, int nf, double* __restrict__ U,
                                                               const double* __restrict__ Tz,
                                                               const double* __restrict__ Rz,
                                                               unsigned long long seed, double amp,
@@ -868,7 +875,7 @@ __global__ void __launch_bounds__(256) init_stratified_kernel(Geom g, int nf, do
 }
 
 // Plain streaming copy (HBM byte-counter calibration): dst[i] = src[i].
-__global__ void __launch_bounds__(256) calibration_copy_kernel(const double* __restrict__ src,
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) calibration_copy_kernel(const double* __restrict__ src,
                                                                double* __restrict__ dst, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -878,7 +885,7 @@ __global__ void __launch_bounds__(256) calibration_copy_kernel(const double* __r
 
 // FP32 copy of a register in the mixed form's deviation layout:
 // dst = (float)(src - background) for RO, TT, BX, BY, BZ, (float)src otherwise.
-__global__ void __launch_bounds__(256) to_f32_dev_kernel(const double* __restrict__ src,
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) to_f32_dev_kernel(const double* __restrict__ src,
                                                          float* __restrict__ dst, Geom g, int nf,
                                                          const double* __restrict__ bg64)
 {
@@ -901,7 +908,7 @@ namespace mhd {
 
 // Global FP64 diagnostics: total mass, total energy, max |div B|.
 // Deterministic: fixed per-block partials, summed in block order.
-__global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, Metric m, int nf,
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, Metric m, int nf,
                                                    const double* __restrict__ U,
                                                    double* __restrict__ part)
 {
@@ -955,7 +962,7 @@ __global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, Metric m, int
     }
 }
 
-__global__ void diag_finalize_kernel(const double* part, int n, double* out)
+MHD_TU_LOCAL __global__ void diag_finalize_kernel(const double* part, int n, double* out)
 {
     if (threadIdx.x != 0) return;
     double a = 0.0, b = 0.0, c = 0.0;

@@ -10,7 +10,14 @@
 // own partial records; ensemble_finish_kernel closes each member's step
 // (TIMC, nIt) and forms its next DT, one block per member.
 #pragma once
+// FP64 arithmetic follows the reference's operation order, so nothing is
+// contracted into FMA; the translation unit of the FP32 (mixed-form) kernels
+// alone allows contraction within an expression.
+#ifdef MHD3D_FP32_FORM_TU
+#pragma clang fp contract(on)
+#else
 #pragma clang fp contract(off)
+#endif
 
 #include "kernels_basic.hpp"
 #include "kernels_stream.hpp"
@@ -108,7 +115,7 @@ __global__ void __launch_bounds__(256)
 // Closes the step of every member (block m = member m): TIMC += DT,
 // nIt += 1 and the DT guard, then the next DT from the member's TSTEP
 // partials.  A halted member keeps its scalars.
-__global__ void __launch_bounds__(256) ensemble_finish_kernel(const FinishArgs* __restrict__ f)
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) ensemble_finish_kernel(const FinishArgs* __restrict__ f)
 {
     const FinishArgs& m = f[blockIdx.x];
     if (m.ds->halted) {
@@ -123,7 +130,7 @@ __global__ void __launch_bounds__(256) ensemble_finish_kernel(const FinishArgs* 
 }
 
 // Non-finite flags of all members, cleared before a call's steps.
-__global__ void ensemble_clear_kernel(const FinishArgs* __restrict__ f, int n)
+MHD_TU_LOCAL __global__ void ensemble_clear_kernel(const FinishArgs* __restrict__ f, int n)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < n) {

@@ -18,7 +18,14 @@
 // (three-slot ring, two buffers); global loads of the next plane are issued
 // before the flux evaluation so they overlap it.
 #pragma once
+// FP64 arithmetic follows the reference's operation order, so nothing is
+// contracted into FMA; the translation unit of the FP32 (mixed-form) kernels
+// alone allows contraction within an expression.
+#ifdef MHD3D_FP32_FORM_TU
+#pragma clang fp contract(on)
+#else
 #pragma clang fp contract(off)
+#endif
 
 #include <type_traits>
 

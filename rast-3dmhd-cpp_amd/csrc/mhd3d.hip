@@ -24,6 +24,7 @@
 #include "kernels_basic.hpp"
 #include "kernels_stream.hpp"
 #include "kernels_ensemble.hpp"
+#include "mixed_launch.hpp"
 
 using namespace mhd;
 
@@ -380,7 +381,7 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
         const int w = ctx->tune_waves[fm][ph], sc = ctx->tune_sched[fm][ph];
         const void* fn;
         if (ctx->mixed)
-            fn = desc->lMag ? stream_fn<float, true, true>(w, sc) : stream_fn<float, false, true>(w, sc);
+            fn = mixed_stream_fn(desc->lMag, w, sc);
         else if (ctx->ext)
             fn = desc->lMag ? stream_ext_fn<true, true>() : stream_ext_fn<false, true>();
         else
@@ -695,8 +696,8 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
     dim3 blk(64, 4, 1);
     const int fm = ctx->mixed ? 1 : 0, ph = MAG ? 1 : 0;
     if (ctx->mixed) {
-        launch_stream_stage<float, MAG, ROT, STAGE>(a, ctx->stream, ctx->tune_waves[fm][ph],
-                                                    ctx->tune_sched[fm][ph], ctx->target_blocks);
+        mixed_stage(MAG, ROT, STAGE, a, ctx->stream, ctx->tune_waves[fm][ph], ctx->tune_sched[fm][ph],
+                    ctx->target_blocks);
     } else if (ctx->ext && ctx->kernel_mode == 0) {
         launch_stream_ext<MAG, ROT, STAGE>(a, ctx->stream, ctx->target_blocks);
     } else if (ctx->kernel_mode == 1) {
@@ -718,7 +719,7 @@ static void launch_wall_t(mhd3d_ctx* ctx, const StageArgs& a)
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
     if (ctx->mixed)
-        stage_wall_kernel<float, MAG, ROT, STAGE><<<grd, blk, 0, ctx->stream>>>(a);
+        mixed_wall(MAG, ROT, STAGE, a, grd, blk, ctx->stream);
     else if (ctx->ext)
         stage_wall_kernel<double, MAG, ROT, STAGE, true><<<grd, blk, 0, ctx->stream>>>(a);
     else
@@ -1179,7 +1180,7 @@ static int ens_resident_blocks(const mhd3d_ensemble* e)
 {
     const void* fn;
     if (e->mixed)
-        fn = e->waves == 1 ? ens_stage_fn<float, 2, 1>(e->sched) : ens_stage_fn<float, 2, 2>(e->sched);
+        fn = mixed_ens_stage_fn(e->waves, e->sched);
     else
         fn = e->waves == 1 ? ens_stage_fn<double, 2, 1>(e->sched) : ens_stage_fn<double, 2, 2>(e->sched);
     int per_cu = 0, dev = 0, ncu = 0;
@@ -1420,7 +1421,9 @@ static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
 {
     for (int k = 1; k <= 3; ++k) {
         const StageArgs* args = e->d_args + ((size_t)p0 * 3 + (k - 1)) * e->n;
-        if (e->mixed) ens_stage<float>(e, k, args);
+        if (e->mixed)
+            mixed_ens_stage(k, e->waves, e->sched, args, e->d_items, e->nitems, e->grid_int, e->d_walls,
+                            e->nwalls, e->grid_wall, e->stream);
         else ens_stage<double>(e, k, args);
     }
     ensemble_finish_kernel<<<e->n, 256, 0, e->stream>>>(e->d_fin);

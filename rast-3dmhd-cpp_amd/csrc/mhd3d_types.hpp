@@ -10,6 +10,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Linkage of the non-template kernels defined in these headers: external in
+// the main translation unit, which launches them; internal (and so not
+// emitted, being unused) in the translation unit of the FP32 kernels, which
+// includes the same headers.
+#ifdef MHD3D_FP32_FORM_TU
+#define MHD_TU_LOCAL static
+#else
+#define MHD_TU_LOCAL
+#endif
+
 namespace mhd {
 
 constexpr int NF_HYDRO = 5;
