@@ -375,9 +375,17 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
                 }
             }
         }
+        // -- fetch plane L+1: EARLY_LOAD issues the loads as soon as plane L's
+        // raw values are consumed, so they are in flight through the barrier
+        // as well (measured faster in FP64 stages 2-3 and FP64 hydro; FP64
+        // MHD stage 1 and the mixed form prefer them after the barrier)
+        constexpr bool EARLY_LOAD = F64 && (STAGE > 1 || !MAG);
+        if (EARLY_LOAD && L < ze) {
+            load_raw<TI, MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
+            if (has_halo) load_raw<TI, MAG>(halo_n, U, fs, (int64_t)(L + 1) * g.plane + gh);
+        }
         lds_barrier();
-        // -- prefetch plane L+1 (before the flux evaluation, so the loads overlap it)
-        if (L < ze) {
+        if (!EARLY_LOAD && L < ze) {
             load_raw<TI, MAG>(own_n, U, fs, (int64_t)(L + 1) * g.plane + go);
             if (has_halo) load_raw<TI, MAG>(halo_n, U, fs, (int64_t)(L + 1) * g.plane + gh);
         }
