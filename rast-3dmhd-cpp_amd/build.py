@@ -23,6 +23,10 @@ FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    # the 2-wave one-barrier variants of the FP64 MHD stage kernels (kept for
+    # the autotuner) need more registers than 2 waves allow; the compiler
+    # builds them for 1 wave and says so for each
+    "-Wno-pass-failed",
 ]
 # per-source additions: the mixed form's single-domain FP32 kernels measured
 # fastest with the instruction-level-parallelism scheduler (MI355X,
