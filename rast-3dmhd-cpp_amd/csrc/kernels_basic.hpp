@@ -252,26 +252,48 @@ __device__ __forceinline__ double fsel(const FluxOutT<T>& o, int f)
     }
 }
 
-// Per-point TSTEP quantities from the state values (3dmhdsub.h:41-88).
+// Per-point TSTEP quantities from the state values (3dmhdsub.h:41-88), with
+// the scalars from K (Phys or CflRec, same field names), DD the point's grid
+// spacing and KC the plane's conduction denominator.
+template <typename K>
+__device__ __forceinline__ void cfl_core(const K& k, bool mag, double RU, double RV, double RW,
+                                         double RO, double TT, double BX, double BY, double BZ,
+                                         double DD, double KC, double w[4], double& um)
+{
+    double UU = (RU * RU + RV * RV + RW * RW) / (RO * RO);
+    um = k.OGAMMA * UU / TT;
+    if (mag) {
+        const double Q = k.GAMMA * TT + (BX * BX + BY * BY + BZ * BZ) * k.OBETA / RO;
+        UU += Q + 2.0 * sqrt(UU * Q);
+    } else {
+        UU += k.GAMMA * TT + 2.0 * sqrt(k.GAMMA * UU * TT);
+    }
+    w[0] = DD / sqrt(UU);
+    w[1] = 0.5 * DD * DD * k.REPR * k.CV * RO / KC;
+    w[2] = 0.375 * DD * DD * k.RE * RO;
+    w[3] = mag ? 0.5 * DD * DD * k.RM : 1e99;
+}
+
 __device__ __forceinline__ void cfl_values(const Phys& c, const Metric& m, bool mag, double RU,
                                            double RV, double RW, double RO, double TT,
                                            double BX, double BY, double BZ, int xg, int yg,
                                            int z, double w[4], double& um)
 {
-    double UU = (RU * RU + RV * RV + RW * RW) / (RO * RO);
-    um = c.OGAMMA * UU / TT;
-    if (mag) {
-        const double Q = c.GAMMA * TT + (BX * BX + BY * BY + BZ * BZ) * c.OBETA / RO;
-        UU += Q + 2.0 * sqrt(UU * Q);
-    } else {
-        UU += c.GAMMA * TT + 2.0 * sqrt(c.GAMMA * UU * TT);
-    }
     double DD = ref_min(m.DDY[yg], m.DDZ[z]);
     if (c.xdim) DD = ref_min(DD, m.DDX[xg]);
-    w[0] = DD / sqrt(UU);
-    w[1] = 0.5 * DD * DD * c.REPR * c.CV * RO / m.KCFL[z];
-    w[2] = 0.375 * DD * DD * c.RE * RO;
-    w[3] = mag ? 0.5 * DD * DD * c.RM : 1e99;
+    cfl_core(c, mag, RU, RV, RW, RO, TT, BX, BY, BZ, DD, m.KCFL[z], w, um);
+}
+
+// The same values with plane z's record r and the column's part of the grid
+// spacing, ddxy = DDY[yg] (min DDX[xg] when x is resolved).  DD is the
+// minimum of the same finite spacings as in cfl_values, so it is the same
+// double.
+__device__ __forceinline__ void cfl_values_rec(const CflRec& r, bool mag, double RU, double RV,
+                                               double RW, double RO, double TT, double BX,
+                                               double BY, double BZ, double ddxy, double w[4],
+                                               double& um)
+{
+    cfl_core(r, mag, RU, RV, RW, RO, TT, BX, BY, BZ, ref_min(ddxy, r.DDZ), r.KCFL, w, um);
 }
 
 __device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Metric& m,

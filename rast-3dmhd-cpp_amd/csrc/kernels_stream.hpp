@@ -286,6 +286,13 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)zb * g.plane + go];
     }
 
+    // stage 3: the column's part of the TSTEP grid spacing (DDZ comes with
+    // each plane's record)
+    double ddxy = 0.0;
+    if (STAGE == 3) {
+        ddxy = a.m.DDY[gy + 1];
+        if (a.c.xdim) ddxy = ref_min(ddxy, a.m.DDX[gx + 1]);
+    }
     // RK update of plane zq from its tendencies o and its Z-type operand zin
     // (Uout = zin + GAM*DT*F, Z = Uout + ZETA*DT*F; stage 3 also the CFL terms)
     auto update = [&](int zq, const FluxOutT<T>& o, const double* zin) {
@@ -304,12 +311,13 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         }
         if (STAGE == 3 && a.cfl_parts) {
             double wp[4], up;
+            const CflRec& r = a.m.cflz[zq];
             if constexpr (MAG)
-                cfl_values(a.c, a.m, true, un[0], un[1], un[2], un[3], un[4], un[5], un[6], un[7],
-                           gx + 1, gy + 1, zq, wp, up);
+                cfl_values_rec(r, true, un[0], un[1], un[2], un[3], un[4], un[5], un[6], un[7], ddxy,
+                               wp, up);
             else
-                cfl_values(a.c, a.m, false, un[0], un[1], un[2], un[3], un[4], 0.0, 0.0, 0.0,
-                           gx + 1, gy + 1, zq, wp, up);
+                cfl_values_rec(r, false, un[0], un[1], un[2], un[3], un[4], 0.0, 0.0, 0.0, ddxy, wp,
+                               up);
             cfl.add(wp, up);
         }
     };

@@ -305,10 +305,12 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(MHD3D_EHIP, "hipStreamCreate failed");
     const int nx = g.nPx + 2, ny = g.nPy + 2, nz = g.nz;
-    // metric block: dxxdx d2xxdx2 DDX | dyydy d2yydy2 DDY | dzzdz d2zzdz2 DDZ RKAPA DKAPA KCFL
-    const size_t nmet = 3 * (size_t)nx + 3 * (size_t)ny + 6 * (size_t)nz;
-    std::vector<double> hm(nmet);
-    size_t o = 0;
+    // metric block: TSTEP records (16 doubles per plane) | dxxdx d2xxdx2 DDX |
+    // dyydy d2yydy2 DDY | dzzdz d2zzdz2 DDZ RKAPA DKAPA KCFL
+    const size_t nrec = (size_t)(sizeof(CflRec) / sizeof(double)) * nz;
+    const size_t nmet = nrec + 3 * (size_t)nx + 3 * (size_t)ny + 6 * (size_t)nz;
+    std::vector<double> hm(nmet, 0.0);
+    size_t o = nrec;
     auto put = [&](const double* src, int n) { std::memcpy(&hm[o], src, n * sizeof(double)); o += n; };
     put(desc->dxxdx, nx); put(desc->d2xxdx2, nx); put(desc->DDX, nx);
     put(desc->dyydy, ny); put(desc->d2yydy2, ny); put(desc->DDY, ny);
@@ -316,10 +318,25 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     put(desc->RKAPA, nz); put(desc->DKAPA, nz);
     // TSTEP conduction denominator: RKAPA[z], or 1.0f + RKAPA[z] with lRem (3dmhdsub.h:78-82)
     for (int z = 0; z < nz; ++z) hm[o++] = desc->lRem ? 1.0f + desc->RKAPA[z] : desc->RKAPA[z];
+    for (int z = 0; z < nz; ++z) {
+        CflRec r{};
+        r.DDZ = desc->DDZ[z];
+        r.KCFL = hm[o - nz + z];
+        r.OGAMMA = ctx->c.OGAMMA;
+        r.GAMMA = ctx->c.GAMMA;
+        r.OBETA = ctx->c.OBETA;
+        r.REPR = ctx->c.REPR;
+        r.CV = ctx->c.CV;
+        r.RE = ctx->c.RE;
+        r.RM = ctx->c.RM;
+        std::memcpy(&hm[(size_t)z * (sizeof(CflRec) / sizeof(double))], &r, sizeof(CflRec));
+    }
     if (hipMalloc(&ctx->dmet, nmet * sizeof(double)) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(metrics) failed");
     if (hipMemcpy(ctx->dmet, hm.data(), nmet * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(MHD3D_EHIP, "metric upload failed");
     double* p = ctx->dmet;
+    ctx->m.cflz = reinterpret_cast<const CflRec*>(p);
+    p += nrec;
     ctx->m.dxxdx = p; p += nx; ctx->m.d2xxdx2 = p; p += nx; ctx->m.DDX = p; p += nx;
     ctx->m.dyydy = p; p += ny; ctx->m.d2yydy2 = p; p += ny; ctx->m.DDY = p; p += ny;
     ctx->m.dzzdz = p; p += nz; ctx->m.d2zzdz2 = p; p += nz; ctx->m.DDZ = p; p += nz;

@@ -59,6 +59,15 @@ struct Phys {
     int izCon, iTCon, iBCon, xdim;
 };
 
+// TSTEP inputs of one plane z in one 128-byte record: its metric terms and
+// the scalars of the limits (3dmhdsub.h:41-88).  The streaming kernel's
+// stage 3 reads the record of the plane it updates, so these values are
+// loaded where they are used instead of being held through the whole sweep.
+struct CflRec {
+    double DDZ, KCFL, OGAMMA, GAMMA, OBETA, REPR, CV, RE, RM;
+    double pad[7];
+};
+
 // 1-D metric arrays on the device, indexed with the reference's ghosted
 // indices (interior x = 1..nPx etc.).
 struct Metric {
@@ -76,6 +85,7 @@ struct Metric {
     // denominator of the TSTEP conduction limit WW2 per plane: RKAPA[z], or
     // 1.0f + RKAPA[z] with lRem (3dmhdsub.h:78-82), formed on the host
     const double* __restrict__ KCFL;
+    const CflRec* __restrict__ cflz;   // one record per plane z
 };
 
 // Off-default branches of fluxes(), BCON() and TSTEP that the point kernel
