@@ -289,9 +289,10 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     // HOIST_METRIC: the column's x and y metric terms (the same on every
     // plane) are loaded once and the z terms of each plane at the top of its
     // iteration, instead of all eight right before the flux evaluation, where
-    // their latency shows.  Measured faster in stage 3 and slower in stages
-    // 1-2 (register budget), so stage 3 only.
-    constexpr bool HOIST_METRIC = STAGE == 3;
+    // their latency shows.  Measured faster in FP64 stage 3 and in every stage
+    // of the mixed form (its FP32 registers leave room), slower in FP64
+    // stages 1-2 (register budget).
+    constexpr bool HOIST_METRIC = STAGE == 3 || !F64;
     double mdxx = 0.0, md2xx = 0.0, mdyy = 0.0, md2yy = 0.0;
     if (HOIST_METRIC) {
         mdxx = a.m.dxxdx[gx + 1];
