@@ -55,10 +55,10 @@ struct mhd3d_ctx {
     // streaming-kernel variant per [form][physics] (form 0 = FP64, 1 = mixed;
     // physics 0 = hydro, 1 = MHD): register budget in waves per SIMD, and
     // schedule bits (1: one barrier per plane, 2: deferred RK update).
-    // Defaults: the fastest variants measured on MI355X at 256x256x128
-    // (launches sized to the resident block count).
-    int tune_waves[2][2] = {{2, 1}, {2, 2}};
-    int tune_sched[2][2] = {{2, 2}, {2, 2}};
+    // Defaults: the fastest variants measured on MI355X at 256x256x128 and
+    // 512x512x256 (launches sized to the resident block count).
+    int tune_waves[2][2] = {{1, 1}, {2, 2}};
+    int tune_sched[2][2] = {{3, 2}, {2, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
     // off-default branches (iD, lRem, lShr, iTCon 2, TP plume): FP64, evaluated
     // by the point and wall kernels
