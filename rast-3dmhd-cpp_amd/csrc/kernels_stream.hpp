@@ -634,20 +634,21 @@ inline int resident_blocks(const void* fn, int threads)
     return per_cu * ncu;
 }
 
-// The off-default branches (iD, lRem, lShr, iTCon 2, the plume walls) run in
-// one FP64 variant: RO of the centre plane in LDS as well, one wave per SIMD
-// (room for the branches' per-point inputs), two barriers, deferred update.
+// The off-default branches of interior points (iD, lRem, lShr, iTCon 2) run
+// in one FP64 variant: RO of the centre plane in LDS as well, one wave per
+// SIMD (room for the branches' per-point inputs), one barrier per plane,
+// deferred update.
 template <bool MAG, bool ROT>
 static const void* stream_ext_fn()
 {
-    return (const void*)stage_stream_kernel<double, MAG, ROT, 2, 1, false, true, true>;
+    return (const void*)stage_stream_kernel<double, MAG, ROT, 2, 1, true, true, true>;
 }
 
 template <bool MAG, bool ROT, int STAGE>
 static void launch_stream_ext(const StageArgs& a, hipStream_t s, int target_blocks)
 {
     const StreamPlan p = stream_plan(a.g, target_blocks);
-    stage_stream_kernel<double, MAG, ROT, STAGE, 1, false, true, true><<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(
+    stage_stream_kernel<double, MAG, ROT, STAGE, 1, true, true, true><<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(
         a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
 }
 

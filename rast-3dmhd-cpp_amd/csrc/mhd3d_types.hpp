@@ -100,6 +100,11 @@ enum : int {
     EXT_ROCOPY = 64, // iD != 0 or lRem: RO ghost planes read as the planes two in (:230-239)
 };
 
+// The flags whose branches act on interior points (fluxes() terms); the
+// others (the plume walls, the RO ghost copy) act on the wall planes only,
+// so a run with none of these keeps the default interior kernel.
+constexpr int EXT_INTERIOR = EXT_DH | EXT_DV | EXT_REM | EXT_SHR | EXT_HEAT;
+
 // Operands of the off-default branches (device pointers, host-formed scalars).
 struct ExtArgs {
     int flags = 0;
