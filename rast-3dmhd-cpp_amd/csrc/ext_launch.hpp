@@ -1,0 +1,44 @@
+// Launches of the off-default interior stage kernels (FP64 only), defined in
+// ext_stream.hip.  Each variant compiles in one set of branches (EXTM mask),
+// so a run with one off-default switch does not carry the registers of the
+// others; a run whose interior switches fit no single set takes the variant
+// with all of them.  Every variant has one wave per SIMD, one barrier per
+// plane and the deferred update.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels_basic.hpp"
+
+namespace mhd {
+
+enum ExtVariant : int {
+    EXTV_NONE = 0,   // no interior branch: the default interior kernel
+    EXTV_ID = 1,     // density diffusion (iD 1/2; DH, DV)
+    EXTV_REM = 2,    // lRem conduction
+    EXTV_SHR = 3,    // lShr carried-over FT
+    EXTV_HEAT = 4,   // iTCon 2 heat loss
+    EXTV_ALL = 5,    // any combination
+};
+
+// The narrowest variant whose branch set holds the interior flags.
+inline int ext_variant(int flags)
+{
+    const int f = flags & EXT_INTERIOR;
+    if (f == 0) return EXTV_NONE;
+    if ((f & ~(EXT_DH | EXT_DV)) == 0) return EXTV_ID;
+    if (f == EXT_REM) return EXTV_REM;
+    if (f == EXT_SHR) return EXTV_SHR;
+    if (f == EXT_HEAT) return EXTV_HEAT;
+    return EXTV_ALL;
+}
+
+// Host stub of the stage-2 kernel of a variant (MHD or hydro, with
+// rotation), for occupancy queries.
+const void* ext_stream_fn(bool mag, int variant);
+
+// Interior planes of one stage with the variant's kernel.
+void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, hipStream_t s,
+               int target_blocks);
+
+}  // namespace mhd

@@ -1,0 +1,81 @@
+// The off-default interior stage kernels (FP64): the streaming kernel with
+// the branches of one ExtVariant compiled in.  Kept in a translation unit of
+// its own so the 60 instantiations build in parallel with the main one; the
+// arithmetic is that of mhd3d.hip's kernels (no contraction).
+#define MHD3D_SECONDARY_TU 1
+
+#include <hip/hip_runtime.h>
+
+#include "ext_launch.hpp"
+#include "kernels_basic.hpp"
+#include "kernels_stream.hpp"
+
+namespace mhd {
+
+namespace {
+
+constexpr int kMask[6] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR};
+
+template <bool MAG, bool ROT, int STAGE, int M>
+void launch_m(const StageArgs& a, hipStream_t s, int target_blocks)
+{
+    const StreamPlan p = stream_plan(a.g, target_blocks);
+    stage_stream_kernel<double, MAG, ROT, STAGE, 1, true, true, true, M>
+        <<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
+}
+
+template <int M>
+void stage_mask(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s, int tb)
+{
+    if (mag && rot) {
+        if (stage == 1) launch_m<true, true, 1, M>(a, s, tb);
+        else if (stage == 2) launch_m<true, true, 2, M>(a, s, tb);
+        else launch_m<true, true, 3, M>(a, s, tb);
+    } else if (mag) {
+        if (stage == 1) launch_m<true, false, 1, M>(a, s, tb);
+        else if (stage == 2) launch_m<true, false, 2, M>(a, s, tb);
+        else launch_m<true, false, 3, M>(a, s, tb);
+    } else if (rot) {
+        if (stage == 1) launch_m<false, true, 1, M>(a, s, tb);
+        else if (stage == 2) launch_m<false, true, 2, M>(a, s, tb);
+        else launch_m<false, true, 3, M>(a, s, tb);
+    } else {
+        if (stage == 1) launch_m<false, false, 1, M>(a, s, tb);
+        else if (stage == 2) launch_m<false, false, 2, M>(a, s, tb);
+        else launch_m<false, false, 3, M>(a, s, tb);
+    }
+}
+
+template <int M>
+const void* fn_mask(bool mag)
+{
+    return mag ? (const void*)stage_stream_kernel<double, true, true, 2, 1, true, true, true, M>
+               : (const void*)stage_stream_kernel<double, false, true, 2, 1, true, true, true, M>;
+}
+
+}  // namespace
+
+const void* ext_stream_fn(bool mag, int variant)
+{
+    switch (variant) {
+    case EXTV_ID: return fn_mask<kMask[EXTV_ID]>(mag);
+    case EXTV_REM: return fn_mask<kMask[EXTV_REM]>(mag);
+    case EXTV_SHR: return fn_mask<kMask[EXTV_SHR]>(mag);
+    case EXTV_HEAT: return fn_mask<kMask[EXTV_HEAT]>(mag);
+    default: return fn_mask<kMask[EXTV_ALL]>(mag);
+    }
+}
+
+void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, hipStream_t s,
+               int target_blocks)
+{
+    switch (variant) {
+    case EXTV_ID: stage_mask<kMask[EXTV_ID]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_REM: stage_mask<kMask[EXTV_REM]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_SHR: stage_mask<kMask[EXTV_SHR]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_HEAT: stage_mask<kMask[EXTV_HEAT]>(mag, rot, stage, a, s, target_blocks); break;
+    default: stage_mask<kMask[EXTV_ALL]>(mag, rot, stage, a, s, target_blocks); break;
+    }
+}
+
+}  // namespace mhd

@@ -139,13 +139,16 @@ struct ExtPt {
 // accessor that gives RO at the neighbours, A::kNeighbourRO), the lRem conduction of
 // TT - TTM(z), the carried-over FT of lShr (its advection, viscous-heating
 // and pressure-work terms are skipped, :293,477-538) and the heat loss of
-// iTCon 2, each at its place in the reference's statement order.
-template <bool ROT, int WALL, typename T, bool DEV, typename A, bool EXT = false>
+// iTCon 2, each at its place in the reference's statement order.  EXTM
+// (a mask of EXT_* bits) limits the branches compiled in; the flags of e must
+// lie within it.
+template <bool ROT, int WALL, typename T, bool DEV, typename A, bool EXT = false,
+          int EXTM = EXT_INTERIOR>
 __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const PointMetric& m, T& FU,
                                            T& FV, T& FW, T& FR, T& FT, const ExtPt* e = nullptr)
 {
     // lShr drops every FT term inside the reference's `#if !lShr` blocks
-    const bool heat_terms = !EXT || !(e->flags & EXT_SHR);
+    const bool heat_terms = !EXT || !((EXTM & EXT_SHR) && (e->flags & EXT_SHR));
     const Ops<T> op = make_ops<T>(c, m);
     const T ORE = T(c.ORE), OCV = T(c.OCV);
     const T ORE_C43 = T(c.ORE_C43);
@@ -168,7 +171,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     // func_mul_extend: (d2 * wwz[z]) * (DH*ORE)).  The reference first copies
     // the planes two in onto the z ghost planes of RO (:230-239), so on a
     // wall W beyond the wall equals W two planes in.
-    if constexpr (EXT && A::kNeighbourRO) {
+    if constexpr (EXT && (EXTM & (EXT_DH | EXT_DV)) && A::kNeighbourRO) {
         if (e->flags & (EXT_DH | EXT_DV)) {
             const double r0 = e->rom[1];
             const double w0 = a.ro(0, 0, 0) - r0;
@@ -234,7 +237,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
 
     // ---- conduction :315-319 (func_d2_mul_array_z: ((d2*g)*a[z])*s)
     tmp = T(c.OCV_REPR);
-    if (EXT && (e->flags & EXT_REM)) {
+    if (EXT && (EXTM & EXT_REM) && (e->flags & EXT_REM)) {
         // lRem :304-313: func_d2_mul of WW1 = TT - TTM(z) ((d2 * RO) * s, RO
         // holding 1/RO), then func_mul_extend of HRAD: (RO * HRAD[z]) * s
         const T m0 = e->ttm[1];
@@ -253,7 +256,8 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
         FT += ((dzT * RI) * T(m.dkap)) * tmp;
     }
     // ---- embedded heat loss, iTCon 2 :443-461: FT -= (((amp*RO)*gx)*gy)*gz
-    if (EXT && (e->flags & EXT_HEAT)) FT -= (((e->heat_amp * RI) * e->hx) * e->hy) * e->hz;
+    if (EXT && (EXTM & EXT_HEAT) && (e->flags & EXT_HEAT))
+        FT -= (((e->heat_amp * RI) * e->hx) * e->hy) * e->hz;
 
     // ---- viscous :462-472
     FU += op.d2x(a.u(1, 0, 0), U0, a.u(-1, 0, 0)) * ORE_C43;
@@ -506,12 +510,13 @@ __device__ __forceinline__ void flux_mag(const A& a, const Phys& c, const PointM
 
 // fluxes() of one point: the hydrodynamic part, then (lMag) the magnetic
 // part with its contributions applied in order.
-template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, bool EXT = false, typename A>
+template <bool MAG, bool ROT, int WALL, typename T, bool DEV = false, bool EXT = false,
+          int EXTM = EXT_INTERIOR, typename A>
 __device__ __forceinline__ void flux_point(const A& a, const Phys& c, const PointMetric& m,
                                            FluxOutT<T>& o, const ExtPt* e = nullptr)
 {
     T FU, FV, FW, FR, FT;
-    flux_hydro<ROT, WALL, T, DEV, A, EXT>(a, c, m, FU, FV, FW, FR, FT, e);
+    flux_hydro<ROT, WALL, T, DEV, A, EXT, EXTM>(a, c, m, FU, FV, FW, FR, FT, e);
     T B1 = T(0), B2 = T(0), B3 = T(0);
     if (MAG) {
         InlineSink<T> sink{FU, FV, FW, FT};

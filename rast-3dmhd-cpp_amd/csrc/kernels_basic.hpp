@@ -204,11 +204,12 @@ __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int
 // wwz[z] (iD 1: 1.0f/ROM[z], iD 2: the host table, 3dmhdsub.h:243-252), the
 // lRem mean-profile term HRAD[z] from TTM (:322-325, same products), the
 // carried-over FT (lShr) and the heat-loss factors (iTCon 2).
+template <int EXTM = EXT_INTERIOR>
 __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int z, int64_t i, ExtPt& e)
 {
     const ExtArgs& X = a.x;
     e.flags = X.flags;
-    if (X.flags & (EXT_DH | EXT_DV)) {
+    if ((EXTM & (EXT_DH | EXT_DV)) && (X.flags & (EXT_DH | EXT_DV))) {
         e.rom[0] = X.means[z - 1];
         e.rom[1] = X.means[z];
         e.rom[2] = X.means[z + 1];
@@ -216,7 +217,7 @@ __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int 
         e.cdh = X.cdh;
         e.cdv = X.cdv;
     }
-    if (X.flags & EXT_REM) {
+    if ((EXTM & EXT_REM) && (X.flags & EXT_REM)) {
         const double* t = X.means + a.g.nz;
         const double tm = t[z - 1], t0 = t[z], tp = t[z + 1];
         e.ttm[0] = tm;
@@ -228,8 +229,8 @@ __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int 
         h += (tp - tm) * a.c.hz * dzz * a.m.DKAPA[z];
         e.hrad = h;
     }
-    if (X.flags & EXT_SHR) e.ft_old = X.ft[i];
-    if (X.flags & EXT_HEAT) {
+    if ((EXTM & EXT_SHR) && (X.flags & EXT_SHR)) e.ft_old = X.ft[i];
+    if ((EXTM & EXT_HEAT) && (X.flags & EXT_HEAT)) {
         e.heat_amp = X.heat_amp;
         e.hx = X.heat_x[x + 1];
         e.hy = X.heat_y[y + 1];

@@ -210,14 +210,17 @@ struct StreamLds {
 // one-sided stencils from global memory would raise this kernel's register
 // budget).  lds: StreamLds::bytes of shared memory.  Stage 3 accumulates the
 // CFL terms of every point it updates into cfl.
-template <typename T, bool MAG, bool ROT, int STAGE, bool ONE_BARRIER, bool DEFER, bool EXT = false>
+template <typename T, bool MAG, bool ROT, int STAGE, bool ONE_BARRIER, bool DEFER, bool EXT = false,
+          int EXTM = EXT_INTERIOR>
 __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int tile_y, int zb,
                                            int ze, char* lds, CflAcc& cfl)
 {
     constexpr bool F64 = std::is_same<T, double>::value;
     static_assert(F64 || !EXT, "the off-default branches are FP64 only");
+    // RO of the centre plane in LDS: only for density diffusion (iD)
+    constexpr bool RO_N = EXT && (EXTM & (EXT_DH | EXT_DV)) != 0;
     constexpr int NR = MAG ? 6 : 3;   // ring fields: u v w [BX BY BZ]
-    constexpr int NS = EXT ? 6 : 5;   // centre-plane fields: RU RV RW TT P [RO]
+    constexpr int NS = RO_N ? 6 : 5;  // centre-plane fields: RU RV RW TT P [RO]
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
     // ring depths: NRING planes of u, v, w (BX, BY, BZ) and NSTAR centre
     // planes, deep enough that one barrier per plane suffices (a thread can be
@@ -266,7 +269,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         U = a.Uin32;
     const double* __restrict__ zsrc = (STAGE == 1) ? a.Uin : a.Z;
     const int64_t fs = g.fs;
-    SAcc<T, MAG, EXT> acc;
+    SAcc<T, MAG, RO_N> acc;
     T row[3] = {T(0), T(0), T(0)};
     T rinv[3] = {T(0), T(0), T(0)};
     T rdv[3] = {T(0), T(0), T(0)};
@@ -371,7 +374,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             ss[2 * SHP + pc] = q.rw;
             ss[3 * SHP + pc] = q.tt;
             ss[4 * SHP + pc] = q.p;
-            if (EXT) ss[5 * SHP + pc] = q.ro;
+            if (RO_N) ss[5 * SHP + pc] = q.ro;
             if (MAG) {
                 su[3 * SHP + pc] = q.bx;
                 su[4 * SHP + pc] = q.by;
@@ -379,7 +382,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             }
             const T nw[5] = {q.ru, q.rv, q.rw, q.tt, q.p};
 #pragma unroll
-            for (int k = 0; k < SAcc<T, MAG, EXT>::NW; ++k) {
+            for (int k = 0; k < SAcc<T, MAG, RO_N>::NW; ++k) {
                 acc.ws[k][0] = acc.ws[k][1];
                 acc.ws[k][1] = acc.ws[k][2];
                 acc.ws[k][2] = nw[k];
@@ -398,7 +401,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
                 ss[2 * SHP + ph] = h.rw;
                 ss[3 * SHP + ph] = h.tt;
                 ss[4 * SHP + ph] = h.p;
-                if (EXT) ss[5 * SHP + ph] = h.ro;
+                if (RO_N) ss[5 * SHP + ph] = h.ro;
                 if (MAG) {
                     su[3 * SHP + ph] = h.bx;
                     su[4 * SHP + ph] = h.by;
@@ -451,7 +454,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             acc.up = &s_uvw[(z + 1) % NRING][0][pc];
             acc.ro0 = row[1];
             acc.ri0 = rinv[1];
-            if constexpr (EXT) {
+            if constexpr (RO_N) {
                 acc.rz[0] = row[0];
                 acc.rz[1] = row[1];
                 acc.rz[2] = row[2];
@@ -487,9 +490,9 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             }
             FluxOutT<T> o;
             ExtPt e;
-            if constexpr (EXT) ext_point(a, gx, gy, z, (int64_t)z * g.plane + go, e);
-            flux_point<MAG, ROT, 0, T, !F64, EXT>(acc, a.c, pm, o, &e);
-            if constexpr (EXT) {
+            if constexpr (EXT) ext_point<EXTM>(a, gx, gy, z, (int64_t)z * g.plane + go, e);
+            flux_point<MAG, ROT, 0, T, !F64, EXT, EXTM>(acc, a.c, pm, o, &e);
+            if constexpr (EXT && (EXTM & EXT_SHR)) {
                 if (valid && (a.x.flags & EXT_SHR)) a.x.ft[(int64_t)z * g.plane + go] = o.FT;
             }
             // FP64 stage 1: Z = U, the centre values of plane z
@@ -528,11 +531,12 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
 // XCDs round-robin) so halo re-reads hit its L2.  Chunk k covers the planes
 // 2 + k*zchunk .. (the last chunk runs to nPz-1).
 template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER,
-          bool EXT = false>
+          bool EXT = false, int EXTM = EXT_INTERIOR>
 __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int nch,
                                                                   int ntx, int nty, int xcd_remap)
 {
-    __shared__ __align__(16) char lds[StreamLds<T, MAG, ONE_BARRIER, EXT>::bytes];
+    __shared__ __align__(16)
+        char lds[StreamLds<T, MAG, ONE_BARRIER, EXT && (EXTM & (EXT_DH | EXT_DV)) != 0>::bytes];
     if (domain_halted(a)) return;
     const int nb = gridDim.x;
     const int bid = blockIdx.x;
@@ -544,7 +548,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     const int zb = 2 + chunk * zchunk;
     const int ze = chunk == nch - 1 ? a.g.nPz : zb + zchunk;
     CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
-    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER, EXT>(a, tile_x, tile_y, zb, ze, lds, cfl);
+    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER, EXT, EXTM>(a, tile_x, tile_y, zb, ze, lds, cfl);
     if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
 }
 
@@ -632,24 +636,6 @@ inline int resident_blocks(const void* fn, int threads)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
         ncu = 256;
     return per_cu * ncu;
-}
-
-// The off-default branches of interior points (iD, lRem, lShr, iTCon 2) run
-// in one FP64 variant: RO of the centre plane in LDS as well, one wave per
-// SIMD (room for the branches' per-point inputs), one barrier per plane,
-// deferred update.
-template <bool MAG, bool ROT>
-static const void* stream_ext_fn()
-{
-    return (const void*)stage_stream_kernel<double, MAG, ROT, 2, 1, true, true, true>;
-}
-
-template <bool MAG, bool ROT, int STAGE>
-static void launch_stream_ext(const StageArgs& a, hipStream_t s, int target_blocks)
-{
-    const StreamPlan p = stream_plan(a.g, target_blocks);
-    stage_stream_kernel<double, MAG, ROT, STAGE, 1, true, true, true><<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(
-        a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
 }
 
 template <typename T, bool MAG, bool ROT, int STAGE>

@@ -25,6 +25,7 @@
 #include "kernels_stream.hpp"
 #include "kernels_ensemble.hpp"
 #include "mixed_launch.hpp"
+#include "ext_launch.hpp"
 
 using namespace mhd;
 
@@ -399,8 +400,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
         const void* fn;
         if (ctx->mixed)
             fn = mixed_stream_fn(desc->lMag, w, sc);
-        else if (ctx->xa.flags & EXT_INTERIOR)
-            fn = desc->lMag ? stream_ext_fn<true, true>() : stream_ext_fn<false, true>();
+        else if (ext_variant(ctx->xa.flags) != EXTV_NONE)
+            fn = ext_stream_fn(desc->lMag, ext_variant(ctx->xa.flags));
         else
             fn = desc->lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
         ctx->target_blocks = resident_blocks(fn, STX * STY);
@@ -715,8 +716,8 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
     if (ctx->mixed) {
         mixed_stage(MAG, ROT, STAGE, a, ctx->stream, ctx->tune_waves[fm][ph], ctx->tune_sched[fm][ph],
                     ctx->target_blocks);
-    } else if (ctx->kernel_mode == 0 && (ctx->xa.flags & EXT_INTERIOR)) {
-        launch_stream_ext<MAG, ROT, STAGE>(a, ctx->stream, ctx->target_blocks);
+    } else if (ctx->kernel_mode == 0 && ext_variant(ctx->xa.flags) != EXTV_NONE) {
+        ext_stage(MAG, ROT, STAGE, ext_variant(ctx->xa.flags), a, ctx->stream, ctx->target_blocks);
     } else if (ctx->kernel_mode == 1) {
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz - 2);
         if (ctx->ext)

@@ -12,9 +12,9 @@
 
 // Linkage of the non-template kernels defined in these headers: external in
 // the main translation unit, which launches them; internal (and so not
-// emitted, being unused) in the translation unit of the FP32 kernels, which
-// includes the same headers.
-#ifdef MHD3D_FP32_FORM_TU
+// emitted, being unused) in the translation units of the FP32 kernels and of
+// the off-default interior kernels, which include the same headers.
+#if defined(MHD3D_FP32_FORM_TU) || defined(MHD3D_SECONDARY_TU)
 #define MHD_TU_LOCAL static
 #else
 #define MHD_TU_LOCAL
