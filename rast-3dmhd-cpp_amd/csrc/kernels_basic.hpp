@@ -736,6 +736,21 @@ MHD_TU_LOCAL __global__ void clock_kernel(DevScalars* s)
     clock_and_guard(s);
 }
 
+// The close of a step whose TSTEP ran inside stage 3: the clock and the DT
+// guard (thread 0), then -- unless the guard halted the run -- the next DT
+// from the stage-3 partials, in one launch instead of clock_kernel followed
+// by cfl_finalize_kernel (same order, same effect).
+MHD_TU_LOCAL __global__ void __launch_bounds__(256) step_close_kernel(const CflPart* __restrict__ part,
+                                                         int nparts, DevScalars* s, Phys c,
+                                                         int mag)
+{
+    if (s->halted) return;
+    if (threadIdx.x == 0) clock_and_guard(s);
+    __syncthreads();
+    if (s->halted) return;
+    cfl_finalize_block(part, nparts, s, c, mag);
+}
+
 MHD_TU_LOCAL __global__ void set_guard_kernel(DevScalars* s, double dt_min)
 {
     s->dt_min = dt_min;

@@ -896,12 +896,13 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
             int rc = run_stage(ctx, k, fuse);
             if (rc) return rc;
         }
-        clock_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
         if (fuse) {
             int ns, nw;
             cfl_part_layout(ctx, ns, nw);
-            cfl_finalize_kernel<<<1, 256, 0, ctx->stream>>>(ctx->cfl_parts, ns + nw, ctx->ds,
-                                                           ctx->c, ctx->d.lMag ? 1 : 0);
+            step_close_kernel<<<1, 256, 0, ctx->stream>>>(ctx->cfl_parts, ns + nw, ctx->ds, ctx->c,
+                                                         ctx->d.lMag ? 1 : 0);
+        } else {
+            clock_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
         }
         ctx->dt_ready = fuse;
         // bound the event pool: fold timings every 64 steps
