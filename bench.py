@@ -83,7 +83,7 @@ def run_cpu_baseline(cfg, state, budget_s=12.0):
 # stages), from rocprofv3 FETCH_SIZE / WRITE_SIZE passes calibrated on a copy
 # kernel of known size (profiles/stage_kernel_mhd256x256x128_fp64.md).  Only
 # the profiled workload has a measured value.
-MEASURED_TRAFFIC = {("mhd", (256, 256, 128)): 1816.0e6}
+MEASURED_TRAFFIC = {("mhd", (256, 256, 128)): 1804.5e6}
 
 
 def measured_traffic(mag, grid):
