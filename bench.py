@@ -241,14 +241,17 @@ def measure_served(n=40, rate=50.0, quantum=10):
     return out
 
 
-def profile_run(steps):
-    """For rocprofv3: `steps` steps of the headline workload, then a plain
-    device copy of 1 GiB whose counters calibrate FETCH/WRITE_SIZE."""
+def profile_run(steps, size=(256, 256, 128), mixed=False):
+    """For rocprofv3: `steps` steps of the headline workload (or, with another
+    size, the MHD+rotation workload at that size, Ra 1e6 as the mixed
+    workload; FP64 or the mixed form), then a plain device copy of 1 GiB
+    whose counters calibrate FETCH/WRITE_SIZE."""
     from mhd3d.desc import Descriptor
     from mhd3d.domain import Domain
     from mhd3d.initial import stratified_config
 
-    cfg = stratified_config(256, 256, 128, Ra=1e5, lMag=True, lRot=True)
+    ra = 1e5 if tuple(size) == (256, 256, 128) else 1e6
+    cfg = stratified_config(*size, Ra=ra, lMag=True, lRot=True, precision=1 if mixed else 0)
     dom = Domain(Descriptor(cfg))
     dom.init_stratified(seed=12345)
     dom.step(steps)
@@ -260,7 +263,8 @@ def profile_run(steps):
     rc = L.lib().mhd3d_calibration_copy(1 << 30, 3, C.byref(ms))
     if rc != L.OK:
         raise L.MhdError(rc, "calibration copy failed")
-    log(f"profile run: {steps} steps, 1 + 3 copies of {1 << 30} bytes, "
+    log(f"profile run: {steps} steps of {'x'.join(map(str, size))} ({'mixed' if mixed else 'FP64'}), "
+        f"1 + 3 copies of {1 << 30} bytes, "
         f"{ms.value:.4f} ms per copy ({2 * (1 << 30) / ms.value / 1e6:.0f} GB/s)")
 
 
@@ -275,7 +279,9 @@ def main():
     ap.add_argument("--workloads", default="mixed,ensemble,served",
                     help="extra workloads measured beside the headline ('' for none)")
     ap.add_argument("--profile", type=int, default=0, metavar="STEPS",
-                    help="run STEPS steps and a calibration copy for rocprofv3, print nothing")
+                    help="run STEPS steps and a calibration copy for rocprofv3, print nothing "
+                         "(MHD+rotation at --size; --mixed for the mixed form)")
+    ap.add_argument("--mixed", action="store_true", help="with --profile: the mixed form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -284,7 +290,7 @@ def main():
         sys.exit(2)
 
     if args.profile:
-        profile_run(args.profile)
+        profile_run(args.profile, tuple(int(v) for v in args.size.split("x")), args.mixed)
         return
 
     from mhd3d.desc import Descriptor
