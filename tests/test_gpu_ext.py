@@ -106,11 +106,14 @@ def test_ten_steps_cfl(case):
 
 
 @pytest.mark.parametrize("case,n", [("iD1", (128, 16, 8)), ("lRem1", (128, 16, 8)),
-                                    ("iD2_mhd", (136, 12, 8)), ("lRem0", (72, 10, 8))])
+                                    ("iD2_mhd", (136, 12, 8)), ("lRem0", (72, 10, 8)),
+                                    ("iD1", (1, 600, 6)), ("lRem1", (3, 200, 6))])
 def test_plane_means_wide_planes(case, n):
     """Planes of 2048 and 1632 values (whole groups of 512 summed by the
-    pipelined path, chunks of 64 that span two rows) and of 720 values (one
-    whole group and a rest): five CFL steps bit for bit."""
+    pipelined path, chunks of 64 that span two rows), of 720 values (one
+    whole group and a rest), and of 1-3 columns (a chunk of 64 spans 22-64
+    rows; 600 values: one whole group and a rest): five CFL steps bit for
+    bit."""
     print(f"ext wide planes {case} {n}", flush=True)
     desc, dom, orc = _pair(n, case, seed=11)
     dts_c = [orc.step() for _ in range(5)]
