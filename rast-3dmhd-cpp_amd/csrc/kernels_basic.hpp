@@ -490,8 +490,12 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     const int zi1 = top ? z + 1 : z - 1;   // first plane inward
     const int zi2 = top ? z + 2 : z - 2;   // second plane inward
     const PointMetric pm = point_metric(a.m, x + 1, y + 1, z);
-    FluxOutT<T> o;
-    if constexpr (F64) {
+    // the mixed form's stage 1 reads the FP64 state: its wall fluxes are
+    // evaluated in FP64 too (faster than forming FP32 deviations at every
+    // stencil read, and no less accurate)
+    using TO = typename std::conditional<F64 || STAGE == 1, double, T>::type;
+    FluxOutT<TO> o;
+    if constexpr (F64 || STAGE == 1) {
         GAcc<double, double, EXT> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
         ExtPt e;
         if constexpr (EXT) {
@@ -505,13 +509,6 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
         if constexpr (EXT) {
             if (a.x.flags & EXT_SHR) a.x.ft[(int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x] = o.FT;
         }
-    } else if constexpr (STAGE == 1) {
-        GAccDev<double> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
-                            a.g.nz, a.bgf, a.bg64};
-        if (top)
-            flux_point<MAG, ROT, 1, float, true>(acc, a.c, pm, o);
-        else
-            flux_point<MAG, ROT, 2, float, true>(acc, a.c, pm, o);
     } else {
         GAccDev<float> acc{a.Uin32, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
                            a.g.nz, a.bgf, a.bg64};
