@@ -1743,6 +1743,16 @@ extern "C" int mhd3d_server_create(int precision, int64_t hbm_budget_bytes, int 
     s->ens->mixed = precision == MHD3D_MIXED;
     s->ens->waves = s->ens->mixed ? 2 : 1;
     s->ens->sched = s->ens->mixed ? 2 : 0;
+    // the ensemble variant override (MHD3D_TUNE_ENS_<FP64|MIXED>=waves,sched[,chunk])
+    // applies to the server's ensemble as well
+    const char* tv = std::getenv(s->ens->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
+    int tw = 0, tsc = 0, tch = 0;
+    if (tv && std::sscanf(tv, "%d,%d,%d", &tw, &tsc, &tch) >= 2 && (tw == 1 || tw == 2) && tsc >= 0 &&
+        tsc <= 3) {
+        s->ens->waves = tw;
+        s->ens->sched = tsc;
+        s->ens->chunk = tch > 0 ? tch : 0;
+    }
     if (hipStreamCreateWithFlags(&s->ens->stream, hipStreamNonBlocking) != hipSuccess) {
         g_srv_create_error = "hipStreamCreate failed";
         mhd3d_server_destroy(s);
