@@ -112,10 +112,13 @@ struct orc_ctx {
     (void)h2; (void)d2u;
 #define MI(dim) ((dim) == DX ? x : (dim) == DY ? y : z)
 
+/* metric index of a first difference along z (mutant 19 shifts it by one plane) */
+#define DU1(dim) (MI(dim) + ((MUT(19) && (dim) == DZ) ? 1 : 0))
+
 static void f_d1(orc_ctx* c, int op, int dim, rng_t r, double* t, const double* f, double sc)
 {
     DIMSETUP(dim);
-    LOOP_OP(op, r, t[i], ((f[i + s] - f[i - s]) * du[MI(dim)]) * h * sc);
+    LOOP_OP(op, r, t[i], ((f[i + s] - f[i - s]) * du[DU1(dim)]) * h * sc);
 }
 
 static void f_d1_mul(orc_ctx* c, int op, int dim, rng_t r, double* t, const double* f,
@@ -132,9 +135,11 @@ static void f_d1_mul_az(orc_ctx* c, int op, int dim, rng_t r, double* t, const d
     LOOP_OP(op, r, t[i], ((f[i + s] - f[i - s]) * du[MI(dim)]) * h * g[i] * az[z] * sc);
 }
 
+/* mutants 16 (metric term of D2z dropped) and 20 (dzzdz not squared) */
 #define D2EXPR(f)                                                              \
-    ((f[i + s] - f[i - s]) * d2u[MI(dim)] * h +                                \
-     ((f[i + s] - 2 * f[i]) + f[i - s]) * (du[MI(dim)] * du[MI(dim)]) * h2)
+    ((f[i + s] - f[i - s]) * ((MUT(16) && (dim) == DZ) ? 0.0 : d2u[MI(dim)]) * h + \
+     ((f[i + s] - 2 * f[i]) + f[i - s]) *                                      \
+         ((MUT(20) && (dim) == DZ) ? du[MI(dim)] : (du[MI(dim)] * du[MI(dim)])) * h2)
 
 static void f_d2(orc_ctx* c, int op, int dim, rng_t r, double* t, const double* f, double sc)
 {
@@ -401,7 +406,7 @@ void orc_fluxes(orc_ctx* c)
     f_d1(c, SUB, DY, R, FR, RV, 1);
     f_d1(c, SET, DZ, R, WW1, RW, 1);
     {
-        const double TMPZ = d->hz * c->dzzdz[zl];                       /* :212-217 */
+        const double TMPZ = d->hz * c->dzzdz[MUT(21) ? zl + 1 : zl];    /* :212-217 */
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++)
                 WW1[IX(zl, y, x)] = (4.0 * RW[IX(zl + 1, y, x)] - RW[IX(zl + 2, y, x)]) * TMPZ;
@@ -653,7 +658,7 @@ void orc_fluxes(orc_ctx* c)
         f_d2(c, SET, DZ, R, S1, BX, 1);
         {
             const double hz = d->hz, h2z = d->h2z;                      /* :640-654 */
-            const double d2t = c->d2zzdz2[zl], dt_ = c->dzzdz[zl];
+            const double d2t = MUT(17) ? c->dzzdz[zl] : c->d2zzdz2[zl], dt_ = c->dzzdz[zl];
             const double d2b = c->d2zzdz2[zr - 1], db = c->dzzdz[zr - 1];
             for (int y = yl; y < yr; y++)
                 for (int x = xl; x < xr; x++) {
@@ -853,7 +858,7 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
         else
             WW2[i] = 0.5 * DD * DD * REPR * CV * RO[i] / RKAPA[z];
         WW3[i] = 0.375 * DD * DD * RE * RO[i];
-        if (d->lMag) VV[i] = 0.5 * DD * DD * RM;
+        if (d->lMag) VV[i] = 0.5 * DD * DD * (MUT(18) ? RE : RM);
     });
     double w[4] = {1e99, 1e99, 1e99, 1e99};
     for (int z = R.zl; z < R.zr; ++z)

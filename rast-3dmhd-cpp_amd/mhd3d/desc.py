@@ -177,7 +177,10 @@ class RunConfig:
     iD: int = 0
     par: np.ndarray = field(default_factory=default_par)
     TB: float = 0.0
-    zgrid: tuple = ("uniform",)   # ("uniform",) or ("tanh", beta)
+    # ("uniform",), ("tanh", beta) or ("arrays", ZEE, dzzdz, d2zzdz2, DDZ): the
+    # z metric arrays of a caller that maps its own grid (zJacobi's outputs,
+    # 3dmhdphsic.h:127-156), each of length nPz + 2
+    zgrid: tuple = ("uniform",)
     precision: int = 0
     TIMI: float = 0.0
     # lRem: the wall temperatures Static() leaves in TU, DZTU, DZTB
@@ -288,6 +291,10 @@ def set_constants(cfg: RunConfig):
         ZEE, dzzdz, d2zzdz2, DDZ = _uniform_axis(cfg.nPz, nPz_g, zMax, ghosts_zero=False)
     elif cfg.zgrid[0] == "tanh":
         ZEE, dzzdz, d2zzdz2, DDZ = tanh_axis(cfg.nPz, zMax, float(cfg.zgrid[1]))
+    elif cfg.zgrid[0] == "arrays":
+        ZEE, dzzdz, d2zzdz2, DDZ = (np.array(a, dtype=np.float64) for a in cfg.zgrid[1:5])
+        if any(a.shape != (cfg.nPz + 2,) for a in (ZEE, dzzdz, d2zzdz2, DDZ)):
+            raise ValueError("z metric arrays must have nPz + 2 entries")
     else:
         raise ValueError(f"unknown z grid {cfg.zgrid!r}")
     nz = cfg.nPz + 2

@@ -3,15 +3,19 @@
 Each mutant is the oracle rebuilt with -DORC_MUTANT=k (see the switch list in
 oracle/mhd_oracle.c): a flipped sign, a transposed operand, a dropped term, a
 derivative taken along the wrong axis, a wrong wall-stencil plane, a wrong
-BCON coefficient, a dropped factor in the CFL.  Every one must make at least
-one check of test_oracle_math.py (mutants 1-8) or test_oracle_ext.py (the
-off-default branches, 9-15) fail, otherwise those checks would not pin what
-they claim to pin.  Runs on the CPU only.
+BCON coefficient, a dropped factor in the CFL, a dropped or misplaced metric
+term of the stretched z grid, the wrong Reynolds number in the magnetic CFL
+limit.  Every one must make at least one check of test_oracle_math.py
+(mutants 1-8), test_oracle_ext.py (the off-default branches, 9-15) or
+test_oracle_stretched.py (the z metric and the magnetic CFL limit, 16-21)
+fail, otherwise those checks would not pin what they claim to pin.  Runs on
+the CPU only.
 """
 import pytest
 
 import test_oracle_ext as te
 import test_oracle_math as tm
+import test_oracle_stretched as ts
 from oracle import oracle as om
 
 MUTANTS = {
@@ -30,6 +34,12 @@ MUTANTS = {
     13: "heat loss times RO instead of 1/RO",
     14: "plume Gaussian with the heat-loss exponent",
     15: "lRem CFL limit with RKAPA in place of 1 + RKAPA",
+    16: "D2z without its d2zzdz2 metric term",
+    17: "top-wall one-sided D2z(BX) with dzzdz in place of d2zzdz2",
+    18: "magnetic CFL limit with RE in place of RM",
+    19: "D1z taking the metric of the next plane",
+    20: "D2z with dzzdz not squared",
+    21: "top-wall one-sided D1z(RW) with the metric of plane 2",
 }
 
 
@@ -55,6 +65,14 @@ def _checks():
     yield te.test_heat_loss_without_time_factor
     yield lambda: te.test_plume_wall(0)
     yield lambda: te.test_plume_wall(1)
+    yield lambda: ts.test_every_flux_equals_its_continuum_value_on_a_mapped_grid(1)
+    yield ts.test_density_varying_along_the_map
+    yield lambda: ts.test_wall_planes_on_a_mapped_grid(0)
+    yield lambda: ts.test_wall_planes_on_a_mapped_grid(1)
+    yield ts.test_z_derivatives_converge_on_the_tanh_grid
+    yield ts.test_wall_d2z_converges_on_the_tanh_grid
+    yield lambda: ts.test_cfl_limits_on_the_tanh_grid(25.0)
+    yield lambda: ts.test_cfl_limits_on_the_tanh_grid(1e-3)
 
 
 def test_clean_oracle_passes_every_check():
