@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X RK3 (magneto)hydrodynamics step.
 
-Workload (one GPU): plain FP64 MHD with rotation on a 256x256x128 grid
-(synthetic stratified convection: polytrope m=1, density contrast 10, Ra 1e5,
-Pr 1, Ta 1e4, Pm 1, tanh z grid), CFL time step every step -- the reference's
-``step()`` (3dmhdsub.h:31-153) as a drop-in on the GPU.  State and metrics are
-resident in HBM before the timed region.
+Headline workload (one GPU): plain FP64 MHD with rotation on a 512x512x256
+grid (synthetic stratified convection: polytrope m=1, density contrast 10,
+Ra 1e6, Pr 1, Ta 1e4, Pm 1, tanh z grid), CFL time step every step -- the
+reference's ``step()`` (3dmhdsub.h:31-153) as a drop-in on the GPU, at the
+largest single-GPU configuration.  State and metrics are resident in HBM
+before the timed region.
 
 Prints one JSON line: Mcell-RK3-steps/s, the HBM roofline of the stage
-kernel (algorithmic bytes / event-timed kernel duration) and the CPU baseline
-(the C restatement in oracle/, timed on this host on a bounded sample).
+kernel (algorithmic bytes / event-timed kernel duration, and the HBM traffic
+per launch measured by rocprofv3), the drop-in rate (one step() call per
+step, as the reference's main loop calls it), the streaming ceiling of the
+memory system, the CPU baseline (the C restatement in oracle/, timed on this
+host on a bounded sample) and the mixed / ensemble / served workloads.
 """
 from __future__ import annotations
 
@@ -48,46 +52,90 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_cpu_baseline(cfg, state, budget_s=12.0):
-    """Time the oracle (C restatement of the reference path) on host cores."""
+def host_cores():
+    """Cores this process may run on (its CPU affinity), and the CPU quota of
+    its cgroup when one is set (cgroup v2 cpu.max)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    return n, quota
+
+
+def run_cpu_baseline(cfg, state, budget_s=15.0):
+    """Time the oracle (C restatement of the reference path) on all the host
+    cores this process may use: steady state, differencing two step counts."""
     from oracle import oracle as om
     from mhd3d.desc import Descriptor
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    n_aff, quota = host_cores()
+    threads = min(n_aff, quota) if quota else n_aff
     desc = Descriptor(cfg)
     o = om.Oracle(desc, threads=threads)
     o.set_state(state)
     o.communicate()
     o.step()                       # warm-up (page-in)
-    n = 0
     t0 = time.perf_counter()
-    while True:
+    o.step()
+    t1 = time.perf_counter()
+    n = 1
+    while time.perf_counter() - t0 < budget_s and n < 50:
         o.step()
         n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 50:
-            break
+    el = time.perf_counter() - t1  # steps 2..n: the difference of two step counts
     o.close()
     cells = cfg.nPx * cfg.nPy * cfg.nPz
     return {
-        "value": cells * n / el / 1e6,
+        "value": cells * (n - 1) / el / 1e6 if n > 1 else cells / (t1 - t0) / 1e6,
         "unit": "Mcell-RK3-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} CFL steps of the same {cfg.nPx}x{cfg.nPy}x{cfg.nPz} workload after 1 warm-up "
-                  f"step, {threads} OpenMP threads on {cpu_model()}",
+        "sample": f"{n - 1} CFL steps timed (after 1 warm-up step and 1 step subtracted) of the same "
+                  f"{cfg.nPx}x{cfg.nPy}x{cfg.nPz} workload, {threads} OpenMP threads = every core "
+                  f"this process may use (affinity {n_aff}, cgroup quota {quota or 'none'}) on {cpu_model()}",
     }
 
 
-# HBM bytes per launch of the interior stage kernel (mean of the three
-# stages), from rocprofv3 FETCH_SIZE / WRITE_SIZE passes calibrated on a copy
-# kernel of known size (profiles/stage_kernel_mhd256x256x128_fp64.md).  Only
-# the profiled workload has a measured value.
-MEASURED_TRAFFIC = {("mhd", (256, 256, 128)): 1804.5e6}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_stage_kernel.json")
 
 
-def measured_traffic(mag, grid):
-    return MEASURED_TRAFFIC.get(("mhd" if mag else "hydro", tuple(grid)))
+def measured_traffic(mag, grid, precision=0):
+    """HBM bytes per launch of the interior stage kernel (mean of the three
+    stages) from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in
+    profiles/traffic_stage_kernel.json (each counter scaled by its reading
+    for a copy kernel of known size), or None for an unprofiled workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    key = f"{'mhd' if mag else 'hydro'} {'x'.join(map(str, grid))} {'mixed' if precision else 'fp64'}"
+    rec = table.get(key)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def measure_ceiling(bytes_per_array=1 << 30, reps=10):
+    """Streaming ceiling of the memory system for the read/write mixes of the
+    RK stages (mhd3d_stream_ceiling): GB/s per mix."""
+    import ctypes as C
+    from mhd3d import _lib as L
+    out = {}
+    for nr, nw in ((1, 0), (1, 1), (1, 2), (2, 2), (2, 1)):
+        ms = C.c_double()
+        rc = L.lib().mhd3d_stream_ceiling(int(bytes_per_array), nr, nw, int(reps), C.byref(ms))
+        if rc != L.OK:
+            raise L.MhdError(rc, "stream ceiling failed")
+        out[f"read{nr}_write{nw}"] = round((nr + nw) * bytes_per_array / (ms.value * 1e-3) / 1e9, 1)
+    return {"gbs": out, "note": f"{bytes_per_array >> 20} MiB per array, 16-byte vector loads, "
+                                "4 in flight per thread and array, 2048 x 256 threads"}
 
 
 def log(msg):
@@ -153,10 +201,23 @@ def measure_mixed(size=(512, 512, 256), steps=20, err_steps=100):
     return out
 
 
-def measure_ensemble(n=64, steps=20):
+def _tolerant_step(obj, n):
+    """step(n) under the reference driver's rules: a non-finite member or a
+    DT-guard halt ends nothing but that run."""
+    from mhd3d import _lib as L
+    try:
+        obj.step(n)
+    except L.MhdError as err:
+        if err.code not in (L.ENONFINITE, L.EHALTED):
+            raise
+
+
+def measure_ensemble(n=64, steps=200):
     """Ensemble workload: n runs of uneven grids / flags / parameters advanced
     together (graph replay, autotuned variant), FP64 and mixed, against the
-    same runs stepped one after another in plain FP64."""
+    same runs stepped one after another in plain FP64.  Every run has the
+    reference driver's DT guard; throughput counts the steps each run took
+    (a run the guard halts stops early)."""
     from mhd3d.desc import Descriptor
     from mhd3d.domain import Domain
     from mhd3d.ensemble import Ensemble
@@ -173,32 +234,41 @@ def measure_ensemble(n=64, steps=20):
             descs0 = descs
         ens = Ensemble(descs)
         _init_ensemble(ens, descs, specs)
-        ens.step(2)
+        ens.set_dt_guard()
+        _tolerant_step(ens, 2)
         best, trials = ens.autotune(steps=3)
         ens.graph(True)
-        ens.step(2)
+        _tolerant_step(ens, 2)
         ens.synchronize()
+        n0 = [ens.clock(i)[0] for i in range(n)]
         t0 = time.perf_counter()
-        ens.step(steps)
+        _tolerant_step(ens, steps)
         ens.synchronize()
         el = time.perf_counter() - t0
+        done = [ens.clock(i)[0] - n0[i] for i in range(n)]
+        cs = sum(s.cells * d for s, d in zip(specs, done))
         out["fp64" if prec == 0 else "mixed"] = {
-            "mcell_steps_per_s": round(cells * steps / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 3),
+            "mcell_steps_per_s": round(cs / el / 1e6, 1), "ms_per_step": round(el / steps * 1e3, 3),
+            "runs_stopped_early": sum(1 for d in done if d < steps),
             "variant": {"waves": best[0], "sched": best[1], "chunk": best[2]}}
         log(f"ensemble: form {prec} {out['fp64' if prec == 0 else 'mixed']}")
         ens.close()
     tot = 0.0
+    cs = 0
     for d, s in zip(descs0, specs):
         dom = Domain(d)
         dom.init_stratified(seed=s.seed)
-        dom.step(2)
+        dom.set_dt_guard()
+        _tolerant_step(dom, 4)
         dom.synchronize()
+        n0 = dom.clock()[0]
         t0 = time.perf_counter()
-        dom.step(steps)
+        _tolerant_step(dom, steps)
         dom.synchronize()
         tot += time.perf_counter() - t0
+        cs += s.cells * (dom.clock()[0] - n0)
         dom.close()
-    out["sequential_fp64"] = {"mcell_steps_per_s": round(cells * steps / tot / 1e6, 1),
+    out["sequential_fp64"] = {"mcell_steps_per_s": round(cs / tot / 1e6, 1),
                               "ms_per_step": round(tot / steps * 1e3, 3)}
     for k in ("fp64", "mixed"):
         out[k]["vs_sequential_fp64"] = round(out[k]["mcell_steps_per_s"] / out["sequential_fp64"]["mcell_steps_per_s"], 3)
@@ -229,7 +299,7 @@ def measure_served(n=40, rate=50.0, quantum=10):
         out["fp64" if prec == 0 else "mixed"] = {
             "p50_s": round(r["p50_s"], 3), "p99_s": round(r["p99_s"], 3),
             "mcell_steps_per_s": round(r["mcell_steps_per_s"], 1), "makespan_s": round(r["makespan_s"], 3),
-            "halted_by_dt_guard": r["failed"]}
+            "halted_by_dt_guard": r["failed"], "evicted_nonfinite": r["evicted_nonfinite"]}
         log(f"served: form {prec} {out['fp64' if prec == 0 else 'mixed']}")
     q = sequential_trace(specs, 0)
     out["sequential_fp64"] = {"p50_s": round(q["p50_s"], 3), "p99_s": round(q["p99_s"], 3),
@@ -241,16 +311,16 @@ def measure_served(n=40, rate=50.0, quantum=10):
     return out
 
 
-def profile_run(steps, size=(256, 256, 128), mixed=False):
-    """For rocprofv3: `steps` steps of the headline workload (or, with another
-    size, the MHD+rotation workload at that size, Ra 1e6 as the mixed
-    workload; FP64 or the mixed form), then a plain device copy of 1 GiB
-    whose counters calibrate FETCH/WRITE_SIZE."""
+def profile_run(steps, size=(512, 512, 256), mixed=False):
+    """For rocprofv3: `steps` steps of the MHD+rotation workload at `size`
+    (the headline workload at 512x512x256, Ra 1e6; Ra 1e5 at other sizes;
+    FP64 or the mixed form), then a plain device copy of 1 GiB whose
+    counters calibrate FETCH/WRITE_SIZE."""
     from mhd3d.desc import Descriptor
     from mhd3d.domain import Domain
     from mhd3d.initial import stratified_config
 
-    ra = 1e5 if tuple(size) == (256, 256, 128) else 1e6
+    ra = 1e6 if tuple(size) == (512, 512, 256) else 1e5
     cfg = stratified_config(*size, Ra=ra, lMag=True, lRot=True, precision=1 if mixed else 0)
     dom = Domain(Descriptor(cfg))
     dom.init_stratified(seed=12345)
@@ -273,7 +343,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--size", default="256x256x128")
+    ap.add_argument("--size", default="512x512x256")
     ap.add_argument("--hydro", action="store_true", help="lMag off, lRot off")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workloads", default="mixed,ensemble,served",
@@ -282,6 +352,7 @@ def main():
                     help="run STEPS steps and a calibration copy for rocprofv3, print nothing "
                          "(MHD+rotation at --size; --mixed for the mixed form)")
     ap.add_argument("--mixed", action="store_true", help="with --profile: the mixed form")
+    ap.add_argument("--ceiling", action="store_true", help="print the streaming ceiling only")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -292,18 +363,22 @@ def main():
     if args.profile:
         profile_run(args.profile, tuple(int(v) for v in args.size.split("x")), args.mixed)
         return
+    if args.ceiling:
+        print(json.dumps(measure_ceiling()), flush=True)
+        return
 
     from mhd3d.desc import Descriptor
     from mhd3d.domain import Domain
-    from mhd3d.initial import stratified_config, stratified_state
+    from mhd3d.initial import stratified_config
 
     nx, ny, nz = (int(v) for v in args.size.split("x"))
     mag = not args.hydro
-    cfg = stratified_config(nx, ny, nz, Ra=1e5, Pr=1.0, lMag=mag, lRot=mag, Ta=1e4, Pm=1.0)
+    ra = 1e6 if (nx, ny, nz) == (512, 512, 256) else 1e5
+    cfg = stratified_config(nx, ny, nz, Ra=ra, Pr=1.0, lMag=mag, lRot=mag, Ta=1e4, Pm=1.0)
     desc = Descriptor(cfg)
-    state = stratified_state(desc, seed=12345)
     dom = Domain(desc)
-    dom.set_state(state)
+    dom.init_stratified(seed=12345)
+    state = None if args.no_cpu_baseline else dom.state()   # host copy for the CPU baseline
     nf = dom.nf
 
     if args.warmup:
@@ -324,6 +399,7 @@ def main():
     bytes_per_launch = alg_bytes_per_cell_step(nf) / 3.0 * interior_cells
     avg_ms = tm["stage_ms_total"] / max(1, tm["stage_launches"])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = measured_traffic(mag, (nx, ny, nz))
     out = {
         "metric": "Mcell-RK3-steps/s",
         "value": round(value, 3),
@@ -339,7 +415,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"plain FP64 {'MHD+rotation' if mag else 'hydro'} RK3 step, "
-                        f"{nx}x{ny}x{nz}, CFL dt, stratified polytrope Ra 1e5",
+                        f"{nx}x{ny}x{nz}, CFL dt, stratified polytrope Ra {ra:g}",
             "grid": [nx, ny, nz],
             "lMag": mag, "lRot": mag,
             "fields": nf,
@@ -352,28 +428,44 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(mag, (nx, ny, nz)),
+            "traffic": traffic,
             "kernel": "interior stage kernel",
             "kernel_avg_ms": round(avg_ms, 5),
             "alg_bytes_per_launch": int(bytes_per_launch),
         },
     }
+    log(f"headline: {value:.1f} Mcell-steps/s, stage kernel {avg_ms:.4f} ms, {achieved:.0f} GB/s")
+    # the drop-in call pattern: one step() per call, DT back to the host after
+    # each (the reference's main loop prints it, 3dmhd.cpp:278-290)
+    dom.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dom.step(1)
+    el1 = time.perf_counter() - t0
+    out["dropin"] = {"mcell_steps_per_s": round(cells * args.steps / el1 / 1e6, 1),
+                     "ms_per_step": round(el1 / args.steps * 1e3, 4),
+                     "vs_batched": round(el / el1, 4),
+                     "note": f"{args.steps} calls of mhd3d_step(ctx, 1, 0, &DT)"}
+    log(f"drop-in: {out['dropin']}")
     # the drop-in boundary hands over host arrays: time one upload and one
     # download of the whole state (host layout, ghosts included)
-    t0 = time.perf_counter()
-    dom.set_state(state)
-    t_set = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    dom.state()
-    t_get = time.perf_counter() - t0
-    out["boundary"] = {
-        "set_state_ms": round(t_set * 1e3, 2), "get_state_ms": round(t_get * 1e3, 2),
-        "pcie_inclusive_mcell_steps_per_s": round(cells * args.steps / (el + t_set + t_get) / 1e6, 1),
-        "note": f"{args.steps} steps between one set_state and one get_state"}
+    if state is not None:
+        t0 = time.perf_counter()
+        dom.set_state(state)
+        t_set = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        dom.state()
+        t_get = time.perf_counter() - t0
+        out["boundary"] = {
+            "set_state_ms": round(t_set * 1e3, 2), "get_state_ms": round(t_get * 1e3, 2),
+            "pcie_inclusive_mcell_steps_per_s": round(cells * args.steps / (el + t_set + t_get) / 1e6, 1),
+            "note": f"{args.steps} steps between one set_state and one get_state"}
     dom.close()
-    log(f"headline: {value:.1f} Mcell-steps/s, stage kernel {avg_ms:.4f} ms, {achieved:.0f} GB/s")
-    if not args.no_cpu_baseline:
+    out["ceiling"] = measure_ceiling()
+    log(f"ceiling: {out['ceiling']}")
+    if state is not None:
         out["cpu_baseline"] = run_cpu_baseline(cfg, state)
+        state = None
         log(f"cpu baseline: {out['cpu_baseline']['value']:.2f} Mcell-steps/s")
     extra = {}
     for w in [w for w in args.workloads.split(",") if w]:

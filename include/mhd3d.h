@@ -171,9 +171,11 @@ int mhd3d_bcon(mhd3d_ctx* ctx);
 /* TSTEP block of step() (3dmhdsub.h:36-106) on the current state. */
 int mhd3d_cfl(mhd3d_ctx* ctx, mhd3d_cfl_info* out);
 
-/* Global diagnostics in FP64: total mass sum(RO), total energy
- * sum(0.5*|RU,RV,RW|^2/RO + CV*RO*TT [+ OBETA*|B|^2/2]) and max|div B|
- * over the interior including walls (no metric volume weights). */
+/* Global diagnostics in FP64: total mass sum(RO*dV), total energy
+ * sum((0.5*|RU,RV,RW|^2/RO + CV*RO*TT [+ OBETA*|B|^2/2])*dV) and max|div B|
+ * over the interior including walls, with the cell volume
+ * dV = DDX[x]*DDY[y]*DDZ[z] of the metric (3dmhddata.h:20-22; DDX = 1 when
+ * nPx == 1). */
 int mhd3d_diagnostics(mhd3d_ctx* ctx, double out[3]);
 
 /* Simulation clock: nIt, TIMT, TIMC (3dmhdsub.h:150-152). */
@@ -231,9 +233,10 @@ int mhd3d_ensemble_graph(mhd3d_ensemble* e, int on);
 int mhd3d_ensemble_set_variant(mhd3d_ensemble* e, int waves, int sched, int chunk);
 int mhd3d_ensemble_get_variant(const mhd3d_ensemble* e, int* waves, int* sched, int* chunk);
 int mhd3d_ensemble_synchronize(mhd3d_ensemble* e);
-/* the DT guard (mhd3d_set_dt_guard) for every member; a halted member stays
- * as it is while the others advance */
+/* the DT guard (mhd3d_set_dt_guard) for every member, or for one member; a
+ * halted member stays as it is while the others advance */
 int mhd3d_ensemble_set_dt_guard(mhd3d_ensemble* e, double dt_min);
+int mhd3d_ensemble_set_dt_guard_member(mhd3d_ensemble* e, int member, double dt_min);
 int mhd3d_ensemble_halted(mhd3d_ensemble* e, int member, int* halted);
 /* Autotuner: times `steps` steps of every variant (waves x sched x chunk, 32
  * candidates) on the current states, restores states and clocks after each
@@ -246,13 +249,20 @@ int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* report, int* n
  * and are admitted first-come-first-served while their device memory fits
  * the HBM budget; their state is built on the device (as
  * mhd3d_init_stratified) and kept there; each tick advances every admitted
- * run by up to `quantum_steps` steps in one ensemble (continuous batching:
- * runs join and leave between ticks).  A run's result is the same as the
- * same run advanced alone.  Times are seconds since the server's creation.
+ * run by `quantum_steps` steps in one ensemble, each run stopping on the
+ * device at its own step count (continuous batching: runs join and leave
+ * between ticks; runs that arrive while a quantum runs are set up on the
+ * host and device meanwhile and join at the next tick).  A run's result is
+ * the same as the same run advanced alone.  A run whose state turns
+ * non-finite is evicted (MHD3D_REQ_NONFINITE).  Times are seconds since the
+ * server's creation.
  * ------------------------------------------------------------------------ */
 typedef struct mhd3d_server mhd3d_server;
 
 enum {
+    MHD3D_REQ_NONFINITE = -2,/* evicted: its state held NaN/Inf after a tick
+                                (the reference's driver would keep stepping
+                                it, its minima ignore NaN, 3dmhdutils.h:61-79) */
     MHD3D_REQ_FAILED = -1,   /* halted by the reference's DT guard (every
                                 run is admitted with MHD3D_REFERENCE_DT_MIN) */
     MHD3D_REQ_QUEUED = 0,
@@ -282,8 +292,8 @@ int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, const double* T
                         int nsteps, int64_t* id);
 int mhd3d_server_tick(mhd3d_server* s, int* n_active, int* n_queued);
 int mhd3d_server_poll(mhd3d_server* s, int64_t id, mhd3d_request_info* info);
-/* copies the final state of a finished run (fields may be NULL) and releases
- * its device memory */
+/* copies the final state of a finished run (fields may be NULL; only DONE
+ * runs copy) and releases the device memory of any finished run */
 int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* fields);
 
 /* nPx, nPy, nPz and the number of fields of a context. */
@@ -336,6 +346,14 @@ int mhd3d_static_state_flags(int nPx, int nPy, int nPz_global, int nPEy, int nPE
  * calibrate a profiler's HBM byte counters against a known traffic;
  * *ms_per_copy gets the event-timed average. */
 int mhd3d_calibration_copy(int64_t bytes, int reps, double* ms_per_copy);
+
+/* Measurement support (no counterpart in the reference): the streaming
+ * ceiling of the memory system for the read/write mixes of the RK stages --
+ * `reps` launches of a tuned kernel (16-byte vector loads, 4 in flight per
+ * thread and array) reading nr (1-2) arrays and writing nw (0-2) arrays of
+ * `bytes_per_array` bytes each; *ms_per_launch gets the event-timed average,
+ * so the rate is (nr + nw) * bytes_per_array / time. */
+int mhd3d_stream_ceiling(int64_t bytes_per_array, int nr, int nw, int reps, double* ms_per_launch);
 
 #ifdef __cplusplus
 }

@@ -75,9 +75,14 @@ struct DevScalars {
     double umach;
     double wmin[4];
     unsigned int nonfinite;
-    unsigned int halted;   // the DT guard stopped this domain
+    unsigned int halted;   // HALT_GUARD: the DT guard stopped this domain;
+                           // HALT_STOP: it reached its step budget nit_stop
     double dt_min;         // DT guard threshold (0: no guard)
+    double nit_call0;      // nIt at the start of the current step call
+    double nit_stop;       // stop after this many steps in all (0: no budget)
 };
+
+enum : unsigned int { HALT_GUARD = 1u, HALT_STOP = 2u };
 
 // Background profiles of the mixed form (index k * nz + z).
 // FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
@@ -259,7 +264,7 @@ __device__ __forceinline__ double fsel(const FluxOutT<T>& o, int f)
 template <typename K>
 __device__ __forceinline__ void cfl_core(const K& k, bool mag, double RU, double RV, double RW,
                                          double RO, double TT, double BX, double BY, double BZ,
-                                         double DD, double KC, double w[4], double& um)
+                                         double DD, double KC, double w[5], double& um)
 {
     double UU = (RU * RU + RV * RV + RW * RW) / (RO * RO);
     um = k.OGAMMA * UU / TT;
@@ -273,12 +278,14 @@ __device__ __forceinline__ void cfl_core(const K& k, bool mag, double RU, double
     w[1] = 0.5 * DD * DD * k.REPR * k.CV * RO / KC;
     w[2] = 0.375 * DD * DD * k.RE * RO;
     w[3] = mag ? 0.5 * DD * DD * k.RM : 1e99;
+    // non-finite probe of the point's state: NaN or +-Inf if any field is
+    w[4] = ((RU + RV) + (RW + RO)) + ((TT + BX) + (BY + BZ));
 }
 
 __device__ __forceinline__ void cfl_values(const Phys& c, const Metric& m, bool mag, double RU,
                                            double RV, double RW, double RO, double TT,
                                            double BX, double BY, double BZ, int xg, int yg,
-                                           int z, double w[4], double& um)
+                                           int z, double w[5], double& um)
 {
     double DD = ref_min(m.DDY[yg], m.DDZ[z]);
     if (c.xdim) DD = ref_min(DD, m.DDX[xg]);
@@ -291,7 +298,7 @@ __device__ __forceinline__ void cfl_values(const Phys& c, const Metric& m, bool 
 // double.
 __device__ __forceinline__ void cfl_values_rec(const CflRec& r, bool mag, double RU, double RV,
                                                double RW, double RO, double TT, double BX,
-                                               double BY, double BZ, double ddxy, double w[4],
+                                               double BY, double BZ, double ddxy, double w[5],
                                                double& um)
 {
     cfl_core(r, mag, RU, RV, RW, RO, TT, BX, BY, BZ, ref_min(ddxy, r.DDZ), r.KCFL, w, um);
@@ -299,7 +306,7 @@ __device__ __forceinline__ void cfl_values_rec(const CflRec& r, bool mag, double
 
 __device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Metric& m,
                                           const double* __restrict__ U, bool mag, int x, int y,
-                                          int z, double w[4], double& um)
+                                          int z, double w[5], double& um)
 {
     const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
     const int64_t fs = g.fs;
@@ -315,11 +322,14 @@ __device__ __forceinline__ void cfl_point(const Geom& g, const Phys& c, const Me
 struct CflAcc {
     double w[4] = {1e99, 1e99, 1e99, 1e99};
     double um = 0.0, bad = 0.0;
-    __device__ __forceinline__ void add(const double wp[4], double up)
+    // wp[4]: the point's non-finite probe (cfl_core).  Only NaN/Inf in the
+    // state sets the flag: a finite state whose limits are NaN (a negative
+    // temperature under the square root) is what the reference steps on
+    __device__ __forceinline__ void add(const double wp[5], double up)
     {
         for (int k = 0; k < 4; ++k) w[k] = ref_min(w[k], wp[k]);
         um = ref_max(um, up);
-        if (!(wp[0] - wp[0] == 0.0) || !(wp[1] - wp[1] == 0.0)) bad = 1.0;
+        if (!(wp[4] - wp[4] == 0.0)) bad = 1.0;
     }
 };
 
@@ -550,7 +560,7 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
         if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * fsel(o, f);
     }
     if (STAGE == 3 && a.cfl_parts) {
-        double wp[4], up;
+        double wp[5], up;
         cfl_values(c, a.m, MAG, un[0], un[1], un[2], un[3], un[4], MAG ? un[5] : 0.0,
                    MAG ? un[6] : 0.0, MAG ? un[7] : 0.0, x + 1, y + 1, z, wp, up);
         cfl.add(wp, up);
@@ -656,7 +666,7 @@ __global__ void __launch_bounds__(256) cfl_kernel(Geom g, Phys c, Metric m,
     CflAcc acc;
     if (x < g.nPx && y < g.nPy) {
         for (int z = 1 + blockIdx.z; z <= g.nPz; z += gridDim.z) {
-            double wp[4], up;
+            double wp[5], up;
             cfl_point(g, c, m, U, MAG, x, y, z, wp, up);
             acc.add(wp, up);
         }
@@ -673,7 +683,8 @@ __device__ __forceinline__ void clock_and_guard(DevScalars* s)
     s->timc += s->dt;
     s->nit += 1.0;
     s->dt_last = s->dt;
-    if (s->dt_min > 0.0 && s->dt < s->dt_min) s->halted = 1u;
+    if (s->dt_min > 0.0 && s->dt < s->dt_min) s->halted |= HALT_GUARD;
+    if (s->nit_stop > 0.0 && s->nit >= s->nit_stop) s->halted |= HALT_STOP;
 }
 
 // Reduces the per-block partials in a fixed order (one block of 256 threads:
@@ -733,8 +744,9 @@ MHD_TU_LOCAL __global__ void clock_kernel(DevScalars* s)
     clock_and_guard(s);
 }
 
-// The close of a step whose TSTEP ran inside stage 3: the clock and the DT
-// guard (thread 0), then -- unless the guard halted the run -- the next DT
+// The close of a step whose TSTEP ran inside stage 3: the clock, the DT
+// guard and the step budget (thread 0), then -- unless the guard halted the
+// run -- the next DT and the non-finite flag
 // from the stage-3 partials, in one launch instead of clock_kernel followed
 // by cfl_finalize_kernel (same order, same effect).
 MHD_TU_LOCAL __global__ void __launch_bounds__(256) step_close_kernel(const CflPart* __restrict__ part,
@@ -742,19 +754,37 @@ MHD_TU_LOCAL __global__ void __launch_bounds__(256) step_close_kernel(const CflP
                                                          int mag)
 {
     if (s->halted) return;
+    __syncthreads();   // every thread has read the flag before thread 0 may set it
     if (threadIdx.x == 0) clock_and_guard(s);
     __syncthreads();
-    if (s->halted) return;
+    if (s->halted & HALT_GUARD) return;
     cfl_finalize_block(part, nparts, s, c, mag);
 }
 
 MHD_TU_LOCAL __global__ void set_guard_kernel(DevScalars* s, double dt_min)
 {
     s->dt_min = dt_min;
-    s->halted = 0u;
+    s->halted &= ~HALT_GUARD;
 }
 
 MHD_TU_LOCAL __global__ void clear_halt_kernel(DevScalars* s) { s->halted = 0u; }
+
+// Start of a step call: the non-finite flag of this call, and nIt at its
+// start (the steps a call actually took decide which register holds the
+// state when the DT guard or the step budget stopped the domain early).
+MHD_TU_LOCAL __global__ void call_begin_kernel(DevScalars* s)
+{
+    s->nonfinite = 0u;
+    s->nit_call0 = s->nit;
+}
+
+// Step budget of a domain: it stops after nit_stop steps in all.
+MHD_TU_LOCAL __global__ void set_stop_kernel(DevScalars* s, double nit_stop)
+{
+    s->nit_stop = nit_stop;
+    s->halted &= ~HALT_STOP;
+    if (nit_stop > 0.0 && s->nit >= nit_stop) s->halted |= HALT_STOP;
+}
 
 // horizontal_mean (3dmhdsub.h:698-718) of RO (blockIdx.y = 0) and TT (1) on
 // every plane of register U, into out[k * nz + z]: the sum over the plane's
@@ -918,6 +948,48 @@ MHD_TU_LOCAL __global__ void __launch_bounds__(256) calibration_copy_kernel(cons
         dst[i] = src[i];
 }
 
+// Streaming ceiling of the memory system: each thread moves U 16-byte
+// vectors per array and iteration (global_load_dwordx4, U independent loads
+// in flight per input array), NR input arrays summed into NW output arrays
+// of n vectors each (NR:NW = 1:2, 2:2, 2:1 are the read/write mixes of the
+// three RK stages).  NW = 0: read only; the sum is kept alive by a store the
+// data never triggers.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+template <int NR, int NW, int U>
+__global__ void __launch_bounds__(256) stream_ceiling_kernel(const dvec2* __restrict__ in,
+                                                             dvec2* __restrict__ out, int64_t n)
+{
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    dvec2 acc = {0.0, 0.0};
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += T * U) {
+        dvec2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = dvec2{0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = i0 + u * T;
+                if (i < n) v[u] += in[r * n + i];
+            }
+        if constexpr (NW == 0) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += v[u];
+        } else {
+#pragma unroll
+            for (int w = 0; w < NW; ++w)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t i = i0 + u * T;
+                    if (i < n) out[w * n + i] = v[u] * (double)(w + 1);
+                }
+        }
+    }
+    if constexpr (NW == 0)
+        if (acc.x == -1.2345e300) out[0] = acc;
+}
+
 // FP32 copy of a register in the mixed form's deviation layout:
 // dst = (float)(src - background) for RO, TT, BX, BY, BZ, (float)src otherwise.
 MHD_TU_LOCAL __global__ void __launch_bounds__(256) to_f32_dev_kernel(const double* __restrict__ src,
@@ -941,8 +1013,10 @@ MHD_TU_LOCAL __global__ void __launch_bounds__(256) to_f32_dev_kernel(const doub
 
 namespace mhd {
 
-// Global FP64 diagnostics: total mass, total energy, max |div B|.
-// Deterministic: fixed per-block partials, summed in block order.
+// Global FP64 diagnostics: total mass and total energy, each point weighted
+// by its cell volume DDX[x] * DDY[y] * DDZ[z] (DDX = 1 when x is not
+// resolved), and max |div B|.  Deterministic: fixed per-block partials,
+// summed in block order.
 MHD_TU_LOCAL __global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, Metric m, int nf,
                                                    const double* __restrict__ U,
                                                    double* __restrict__ part)
@@ -958,7 +1032,8 @@ MHD_TU_LOCAL __global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, 
         const int64_t i = (int64_t)z * g.plane + (int64_t)y * g.pitch + x;
         const double RU = U[i], RV = U[g.fs + i], RW = U[2 * g.fs + i];
         const double RO = U[3 * g.fs + i], TT = U[4 * g.fs + i];
-        ms += RO;
+        const double dV = (c.xdim ? m.DDX[x + 1] : 1.0) * m.DDY[y + 1] * m.DDZ[z];
+        ms += RO * dV;
         double e = 0.5 * (RU * RU + RV * RV + RW * RW) / RO + c.CV * RO * TT;
         if (nf == NF_MAG) {
             const double BX = U[5 * g.fs + i], BY = U[6 * g.fs + i], BZ = U[7 * g.fs + i];
@@ -976,7 +1051,7 @@ MHD_TU_LOCAL __global__ void __launch_bounds__(256) diag_kernel(Geom g, Phys c, 
                 db = fmax(db, fabs(dbx + dby + dbz));
             }
         }
-        en += e;
+        en += e * dV;
     }
     sh[0][threadIdx.x] = ms;
     sh[1][threadIdx.x] = en;

@@ -113,8 +113,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // Closes the step of every member (block m = member m): TIMC += DT,
-// nIt += 1 and the DT guard, then the next DT from the member's TSTEP
-// partials.  A halted member keeps its scalars.
+// nIt += 1, the DT guard and the step budget, then -- unless the guard
+// halted it -- the next DT and the non-finite flag from the member's TSTEP
+// partials (as step_close_kernel of a single domain).  A stopped member
+// keeps its scalars.
 MHD_TU_LOCAL __global__ void __launch_bounds__(256) ensemble_finish_kernel(const FinishArgs* __restrict__ f)
 {
     const FinishArgs& m = f[blockIdx.x];
@@ -125,16 +127,18 @@ MHD_TU_LOCAL __global__ void __launch_bounds__(256) ensemble_finish_kernel(const
     __syncthreads();   // every thread has read the flag before thread 0 may set it
     if (threadIdx.x == 0) clock_and_guard(m.ds);
     __syncthreads();
-    cfl_finalize_block(m.parts, m.nparts, m.ds, m.c, m.mag);
+    if (!(m.ds->halted & HALT_GUARD)) cfl_finalize_block(m.parts, m.nparts, m.ds, m.c, m.mag);
     if (threadIdx.x == 0) *m.mirror = *m.ds;   // thread 0 wrote the scalars
 }
 
-// Non-finite flags of all members, cleared before a call's steps.
+// Start of a step call for every member: non-finite flag cleared, nIt at
+// the start recorded (as call_begin_kernel of a single domain).
 MHD_TU_LOCAL __global__ void ensemble_clear_kernel(const FinishArgs* __restrict__ f, int n)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < n) {
         f[m].ds->nonfinite = 0u;
+        f[m].ds->nit_call0 = f[m].ds->nit;
         *f[m].mirror = *f[m].ds;
     }
 }

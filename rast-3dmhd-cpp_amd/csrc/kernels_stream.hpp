@@ -327,7 +327,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * F;
         }
         if (STAGE == 3 && a.cfl_parts) {
-            double wp[4], up;
+            double wp[5], up;
             const CflRec& r = a.m.cflz[zq];
             if constexpr (MAG)
                 cfl_values_rec(r, true, un[0], un[1], un[2], un[3], un[4], un[5], un[6], un[7], ddxy,

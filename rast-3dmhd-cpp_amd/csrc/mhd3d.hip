@@ -45,6 +45,7 @@ struct mhd3d_ctx {
     std::vector<double> h_dzz, h_d2zz;   // host copies of dzzdz, d2zzdz2
     double* F = nullptr;
     DevScalars* ds = nullptr;
+    DevScalars* hs = nullptr;       // pinned host copy of *ds (read back after a step call)
     CflPart* cfl_parts = nullptr;
     int cfl_parts_cap = 0;
     bool dt_ready = false;          // ds->dt holds TSTEP of the current state
@@ -371,7 +372,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (hipMalloc(&ctx->ds, sizeof(DevScalars)) != hipSuccess) return cleanup(MHD3D_ENOMEM, "hipMalloc(scalars) failed");
     if (hipMemset(ctx->Z, 0, reg) != hipSuccess || hipMemset(ctx->ds, 0, sizeof(DevScalars)) != hipSuccess)
         return cleanup(MHD3D_EHIP, "hipMemset failed");
-    if (hipHostMalloc(&ctx->host_stage, (size_t)g.fs * sizeof(double)) != hipSuccess)
+    if (hipHostMalloc(&ctx->host_stage, (size_t)g.fs * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&ctx->hs, sizeof(DevScalars)) != hipSuccess)
         return cleanup(MHD3D_ENOMEM, "hipHostMalloc failed");
     ctx->bytes = (int64_t)((nU64 + 1) * reg + (ctx->mixed ? 2 * reg32 : 0) + nmet * sizeof(double) +
                            sizeof(DevScalars));
@@ -429,6 +431,7 @@ extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
     (void)hipFree(ctx->ext_buf);
     (void)hipFree(ctx->ft);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+    if (ctx->hs) (void)hipHostFree(ctx->hs);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MHD3D_OK;
@@ -872,21 +875,24 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
 {
     if (!ctx || nsteps < 0) return MHD3D_EINVAL;
     if (fixed_dt && !dt_inout) return fail(ctx, MHD3D_EINVAL, "fixed_dt needs *dt_inout");
+    const int cur0 = ctx->cur;
+    call_begin_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
     if (fixed_dt) {
         set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, *dt_inout);
         HIPCHK(ctx, hipGetLastError());
     }
     // TSTEP of each step is evaluated on the state the previous step's
     // stage 3 produced, inside that launch (fused); a standalone pass runs
-    // only when no fused value for the current state exists.
+    // only when no fused value for the current state exists.  The fused
+    // partials carry the non-finite flag of the state they were taken on,
+    // so the last step's close tells whether the final state is finite.
     const bool fuse = !fixed_dt && ctx->kernel_mode == 0 && ctx->fuse_cfl;
     for (int n = 0; n < nsteps; ++n) {
         if (ctx->ext_timedep) {
             // the heat-loss / plume factors of this step need TIMT on the host
-            DevScalars hs;
-            HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-            step_factors(ctx, ctx->d.TIMI + hs.timc);
+            step_factors(ctx, ctx->d.TIMI + ctx->hs->timc);
         }
         if (!fixed_dt && !(fuse && ctx->dt_ready)) {
             int rc = run_cfl(ctx);
@@ -912,21 +918,27 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
         }
     }
     HIPCHK(ctx, hipGetLastError());
-    ctx->d.reserved = 0;
-    DevScalars hs;
-    {
+    if (!fuse || nsteps == 0) {
+        // no fused partials of the final state: scan it
         const Geom& g = ctx->g;
-        clear_flag_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
         dim3 blk(64, 4, 1);
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz < 8 ? g.nPz : 8);
         nonfinite_kernel<<<grd, blk, 0, ctx->stream>>>(g, ctx->nf, ctx->U[ctx->cur], ctx->ds);
+        HIPCHK(ctx, hipGetLastError());
     }
-    HIPCHK(ctx, hipMemcpyAsync(&hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const DevScalars hs = *ctx->hs;
+    // the register holding the state: each step taken flips it (three
+    // stages); kernels of a stopped domain return at once, so only the steps
+    // actually taken count
+    const int64_t taken = (int64_t)(hs.nit - hs.nit_call0);
+    if (!ctx->mixed) ctx->cur = cur0 ^ (int)(taken & 1);
+    if (taken < nsteps) ctx->dt_ready = false;
     int rc = collect_timing(ctx);
     if (rc) return rc;
     if (dt_inout && nsteps > 0) *dt_inout = hs.dt_last;
-    if (hs.halted) return fail(ctx, MHD3D_EHALTED, "DT too small, halt (DT guard)");
+    if (hs.halted & HALT_GUARD) return fail(ctx, MHD3D_EHALTED, "DT too small, halt (DT guard)");
     if (hs.nonfinite) return fail(ctx, MHD3D_ENONFINITE, "state is not finite after the step");
     return MHD3D_OK;
 }
@@ -947,7 +959,9 @@ extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
     set_dt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt);
     int rc = run_stage(ctx, stage);
     if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->hs->halted) ctx->cur = 1 - ctx->cur;   // the kernels of a stopped domain did nothing
     return collect_timing(ctx);
 }
 
@@ -1104,6 +1118,9 @@ extern "C" int mhd3d_synchronize(mhd3d_ctx* ctx)
 extern "C" int mhd3d_set_dt_guard(mhd3d_ctx* ctx, double dt_min)
 {
     if (!ctx || !(dt_min >= 0.0)) return MHD3D_EINVAL;
+    // a halted domain's DT is that of the step that tripped the guard, not
+    // the TSTEP of its state: resuming takes a fresh TSTEP
+    ctx->dt_ready = false;
     set_guard_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, dt_min);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1128,7 +1145,9 @@ struct mhd3d_ensemble {
     int nitems = 0, nwalls = 0;
     int grid_int = 0, grid_wall = 0;
     int waves = 1, sched = 0, chunk = 0;   // kernel variant; chunk: planes per item (0 = auto)
-    int parity = 0;                    // FP64: U buffer holding every member's state
+    int parity = 0;                    // FP64: register parity of the step tables
+    std::vector<int> off;              // member m's state is U[parity ^ off[m]] (FP64)
+    size_t cap_items = 0, cap_walls = 0, cap_fin = 0, cap_args = 0;   // device table capacities
     bool use_graph = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int64_t cells = 0;
@@ -1220,6 +1239,42 @@ static void ens_drop_graphs(mhd3d_ensemble* e)
         }
 }
 
+// Device room for n records of `size` bytes in *p (capacity cap, in records).
+static int ens_reserve(mhd3d_ensemble* e, void** p, size_t& cap, size_t size, size_t n)
+{
+    if (n <= cap && *p) return MHD3D_OK;
+    const size_t want = std::max(n, cap + cap / 2);
+    (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    ENSCHK(e, hipMalloc(p, size * (want ? want : 1)));
+    cap = want;
+    return MHD3D_OK;
+}
+
+// Operand tables [p0][k-1][m]: during stage k of a step that starts with
+// table parity p0, member m reads U[p0 ^ off[m] ^ ((k-1) & 1)] (FP64).
+static int ens_upload_args(mhd3d_ensemble* e)
+{
+    std::vector<StageArgs> args((size_t)2 * 3 * e->n);
+    for (int p0 = 0; p0 < 2; ++p0)
+        for (int k = 1; k <= 3; ++k)
+            for (int m = 0; m < e->n; ++m) {
+                mhd3d_ctx* c = e->mem[m];
+                const int save = c->cur;
+                c->cur = e->mixed ? 0 : (p0 ^ e->off[m] ^ ((k - 1) & 1));
+                StageArgs a = stage_args(c, k);
+                c->cur = save;
+                a.dt = &c->ds->dt;
+                a.ds = c->ds;
+                a.cfl_parts = k == 3 ? c->cfl_parts : nullptr;
+                a.cfl_base = 0;
+                args[((size_t)p0 * 3 + (k - 1)) * e->n + m] = a;
+            }
+    ENSCHK(e, hipMemcpy(e->d_args, args.data(), sizeof(StageArgs) * args.size(), hipMemcpyHostToDevice));
+    return MHD3D_OK;
+}
+
 // Work lists, operand tables and grid sizes for the current variant.
 static int ens_build(mhd3d_ensemble* e)
 {
@@ -1281,35 +1336,20 @@ static int ens_build(mhd3d_ensemble* e)
         const int cb = (b.ze - b.zb) * ((b.flags & 1) ? 2 : 1);
         return ca > cb;
     });
-    // operand tables: [p0][k-1][m], member cur during stage k = p0 ^ ((k-1) & 1)
-    std::vector<StageArgs> args((size_t)2 * 3 * e->n);
-    for (int p0 = 0; p0 < 2; ++p0)
-        for (int k = 1; k <= 3; ++k)
-            for (int m = 0; m < e->n; ++m) {
-                mhd3d_ctx* c = e->mem[m];
-                const int save = c->cur;
-                c->cur = e->mixed ? 0 : (p0 ^ ((k - 1) & 1));
-                StageArgs a = stage_args(c, k);
-                c->cur = save;
-                a.dt = &c->ds->dt;
-                a.ds = c->ds;
-                a.cfl_parts = k == 3 ? c->cfl_parts : nullptr;
-                a.cfl_base = 0;
-                args[((size_t)p0 * 3 + (k - 1)) * e->n + m] = a;
-            }
-    (void)hipFree(e->d_items);
-    (void)hipFree(e->d_walls);
-    (void)hipFree(e->d_fin);
-    (void)hipFree(e->d_args);
-    e->d_items = nullptr; e->d_walls = nullptr; e->d_fin = nullptr; e->d_args = nullptr;
-    ENSCHK(e, hipMalloc(&e->d_items, sizeof(WorkItem) * items.size()));
-    ENSCHK(e, hipMalloc(&e->d_walls, sizeof(WallItem) * walls.size()));
-    ENSCHK(e, hipMalloc(&e->d_fin, sizeof(FinishArgs) * fin.size()));
-    ENSCHK(e, hipMalloc(&e->d_args, sizeof(StageArgs) * args.size()));
+    // device tables grow only: a server's ensemble changes membership often
+    // and reuses the room it has
+    int rc = ens_reserve(e, (void**)&e->d_items, e->cap_items, sizeof(WorkItem), items.size());
+    if (!rc) rc = ens_reserve(e, (void**)&e->d_walls, e->cap_walls, sizeof(WallItem), walls.size());
+    if (!rc) rc = ens_reserve(e, (void**)&e->d_fin, e->cap_fin, sizeof(FinishArgs), fin.size());
+    if (!rc) rc = ens_reserve(e, (void**)&e->d_args, e->cap_args, sizeof(StageArgs), (size_t)6 * e->n);
+    if (rc) return rc;
     ENSCHK(e, hipMemcpy(e->d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_fin, fin.data(), sizeof(FinishArgs) * fin.size(), hipMemcpyHostToDevice));
-    ENSCHK(e, hipMemcpy(e->d_args, args.data(), sizeof(StageArgs) * args.size(), hipMemcpyHostToDevice));
+    e->off.assign(e->n, 0);
+    for (int m = 0; m < e->n; ++m) e->off[m] = e->mixed ? 0 : (e->parity ^ e->mem[m]->cur);
+    rc = ens_upload_args(e);
+    if (rc) return rc;
     e->nitems = (int)items.size();
     e->nwalls = (int)walls.size();
     const int res = ens_resident_blocks(e);
@@ -1391,7 +1431,6 @@ extern "C" int mhd3d_ensemble_set_state(mhd3d_ensemble* e, int member, const dou
     if (!e || member < 0 || member >= e->n) return MHD3D_EINVAL;
     ENSCHK(e, hipStreamSynchronize(e->stream));
     mhd3d_ctx* c = e->mem[member];
-    c->cur = e->mixed ? 0 : e->parity;
     int rc = mhd3d_set_state(c, fields);
     if (rc) return ens_fail(e, rc, c->err);
     // a fresh state needs its TSTEP (set_state cleared the member's
@@ -1406,7 +1445,6 @@ extern "C" int mhd3d_ensemble_get_state(mhd3d_ensemble* e, int member, double* c
     if (!e || member < 0 || member >= e->n) return MHD3D_EINVAL;
     ENSCHK(e, hipStreamSynchronize(e->stream));
     mhd3d_ctx* c = e->mem[member];
-    c->cur = e->mixed ? 0 : e->parity;
     int rc = mhd3d_get_state(c, fields);
     if (rc) return ens_fail(e, rc, c->err);
     return MHD3D_OK;
@@ -1424,7 +1462,7 @@ static int ens_cfl_all(mhd3d_ensemble* e)
         const int zchunks = g.nPz < 8 ? g.nPz : 8;
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, zchunks);
         const int nparts = (int)(grd.x * grd.y * grd.z);   // within the capacity ens_build ensured
-        const double* U = c->U[e->mixed ? 0 : e->parity];
+        const double* U = c->U[c->cur];
         if (c->d.lMag)
             cfl_kernel<true><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
         else
@@ -1448,9 +1486,9 @@ static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
     ensemble_finish_kernel<<<e->n, 256, 0, e->stream>>>(e->d_fin);
 }
 
-extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
+// Enqueues nsteps steps of every member (no host synchronisation).
+static int ens_step_enqueue(mhd3d_ensemble* e, int nsteps)
 {
-    if (!e || nsteps < 0) return MHD3D_EINVAL;
     {
         int rc = ens_cfl_all(e);   // members without a TSTEP of their current state
         if (rc) return rc;
@@ -1477,11 +1515,42 @@ extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
     ENSCHK(e, hipGetLastError());
     e->last.resize(e->n);
     ENSCHK(e, hipMemcpyAsync(e->last.data(), e->mirror, sizeof(DevScalars) * e->n, hipMemcpyDeviceToHost, e->stream));
+    return MHD3D_OK;
+}
+
+// Waits for the enqueued steps and settles each member's register: a
+// member the DT guard or its step budget stopped took fewer steps than the
+// others, and its state sits in the register of the steps it took.
+static int ens_step_finish(mhd3d_ensemble* e, int nsteps)
+{
     ENSCHK(e, hipStreamSynchronize(e->stream));
+    bool refresh = false;
     for (int m = 0; m < e->n; ++m) {
-        e->mem[m]->cur = e->mixed ? 0 : e->parity;
-        e->mem[m]->dt_ready = true;
+        mhd3d_ctx* c = e->mem[m];
+        const int64_t taken = (int64_t)(e->last[m].nit - e->last[m].nit_call0);
+        c->dt_ready = taken == nsteps;
+        if (e->mixed) continue;
+        const int cur0 = e->parity ^ (nsteps & 1) ^ e->off[m];   // the register at the call's start
+        c->cur = cur0 ^ (int)(taken & 1);
+        const int off = e->parity ^ c->cur;
+        if (off != e->off[m]) {
+            e->off[m] = off;
+            refresh = true;
+        }
     }
+    if (refresh) {
+        int rc = ens_upload_args(e);
+        if (rc) return rc;
+    }
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
+{
+    if (!e || nsteps < 0) return MHD3D_EINVAL;
+    int rc = ens_step_enqueue(e, nsteps);
+    if (!rc) rc = ens_step_finish(e, nsteps);
+    if (rc) return rc;
     for (int m = 0; m < e->n; ++m)
         if (e->last[m].nonfinite)
             return ens_fail(e, MHD3D_ENONFINITE, "member " + std::to_string(m) + ": state is not finite after the step");
@@ -1542,6 +1611,16 @@ extern "C" int mhd3d_ensemble_set_dt_guard(mhd3d_ensemble* e, double dt_min)
     return MHD3D_OK;
 }
 
+extern "C" int mhd3d_ensemble_set_dt_guard_member(mhd3d_ensemble* e, int member, double dt_min)
+{
+    if (!e || member < 0 || member >= e->n || !(dt_min >= 0.0)) return MHD3D_EINVAL;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    mhd3d_ctx* c = e->mem[member];
+    int rc = mhd3d_set_dt_guard(c, dt_min);
+    if (rc) return ens_fail(e, rc, c->err);
+    return MHD3D_OK;
+}
+
 // Whether the DT guard stopped member `member` (after its last step call).
 extern "C" int mhd3d_ensemble_halted(mhd3d_ensemble* e, int member, int* halted)
 {
@@ -1549,7 +1628,7 @@ extern "C" int mhd3d_ensemble_halted(mhd3d_ensemble* e, int member, int* halted)
     DevScalars hs;
     ENSCHK(e, hipMemcpyAsync(&hs, e->mem[member]->ds, sizeof(hs), hipMemcpyDeviceToHost, e->stream));
     ENSCHK(e, hipStreamSynchronize(e->stream));
-    *halted = hs.halted ? 1 : 0;
+    *halted = (hs.halted & HALT_GUARD) ? 1 : 0;
     return MHD3D_OK;
 }
 
@@ -1574,7 +1653,11 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
     std::vector<double*> snap(e->n, nullptr);
     const int p_save = e->parity;
     std::vector<char> ready_save(e->n);
-    for (int m = 0; m < e->n; ++m) ready_save[m] = e->mem[m]->dt_ready;
+    std::vector<int> cur_save(e->n);
+    for (int m = 0; m < e->n; ++m) {
+        ready_save[m] = e->mem[m]->dt_ready;
+        cur_save[m] = e->mem[m]->cur;
+    }
     DevScalars* ds_snap = nullptr;
     auto cleanup = [&]() {
         for (double* p : snap) (void)hipFree(p);
@@ -1584,7 +1667,7 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
         mhd3d_ctx* c = e->mem[m];
         const size_t reg = (size_t)c->g.fs * c->nf * sizeof(double);
         if (hipMalloc(&snap[m], reg) != hipSuccess) { cleanup(); return ens_fail(e, MHD3D_ENOMEM, "autotune snapshot"); }
-        if (hipMemcpyAsync(snap[m], c->U[e->mixed ? 0 : e->parity], reg, hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
+        if (hipMemcpyAsync(snap[m], c->U[c->cur], reg, hipMemcpyDeviceToDevice, e->stream) != hipSuccess) {
             cleanup();
             return ens_fail(e, MHD3D_EHIP, "autotune snapshot copy");
         }
@@ -1597,8 +1680,8 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
         for (int m = 0; m < e->n; ++m) {
             mhd3d_ctx* c = e->mem[m];
             const size_t reg = (size_t)c->g.fs * c->nf * sizeof(double);
-            ENSCHK(e, hipMemcpyAsync(c->U[e->mixed ? 0 : e->parity], snap[m], reg, hipMemcpyDeviceToDevice, e->stream));
-            c->cur = e->mixed ? 0 : e->parity;
+            c->cur = cur_save[m];
+            ENSCHK(e, hipMemcpyAsync(c->U[c->cur], snap[m], reg, hipMemcpyDeviceToDevice, e->stream));
         }
         for (int m = 0; m < e->n; ++m)
             ENSCHK(e, hipMemcpyAsync(e->mem[m]->ds, ds_snap + m, sizeof(DevScalars), hipMemcpyDeviceToDevice, e->stream));
@@ -1682,6 +1765,7 @@ struct ServerReq {
     uint64_t seed = 0;
     double amp = 0, b0 = 0, b_amp = 0;
     int nsteps = 0, done = 0;
+    int cur = 0;                      // register holding the final state (DONE)
     int status = MHD3D_REQ_QUEUED;
     mhd3d_ctx* ctx = nullptr;
     int64_t bytes = 0;
@@ -1850,6 +1934,15 @@ static int srv_admit(mhd3d_server* s)
         if (rc) { s->err = g_create_error; return rc; }
         rc = mhd3d_init_stratified(c, r.Tz.data(), r.Rz.data(), r.seed, r.amp, r.b0, r.b_amp);
         if (!rc) rc = mhd3d_set_dt_guard(c, MHD3D_REFERENCE_DT_MIN);
+        if (!rc) {
+            // the run stops on the device after its own step count, whatever
+            // the quantum of the ensemble it runs in
+            set_stop_kernel<<<1, 1, 0, c->stream>>>(c->ds, (double)r.nsteps);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+                c->err = "step budget setup failed";
+                rc = MHD3D_EHIP;
+            }
+        }
         if (rc) { s->err = c->err; mhd3d_destroy(c); return rc; }
         r.ctx = c;
         r.status = MHD3D_REQ_RUNNING;
@@ -1862,58 +1955,83 @@ static int srv_admit(mhd3d_server* s)
     return MHD3D_OK;
 }
 
+// Rebuilds the server's ensemble over the active runs after joins or leaves.
+static int srv_rebuild(mhd3d_server* s)
+{
+    mhd3d_ensemble* e = s->ens;
+    e->mem.clear();
+    e->cells = 0;
+    for (int64_t id : s->active) {
+        mhd3d_ctx* c = s->reqs[id].ctx;
+        e->mem.push_back(c);
+        e->cells += (int64_t)c->g.nPx * c->g.nPy * c->g.nPz;
+    }
+    e->n = (int)e->mem.size();
+    int rc = ens_build(e);
+    if (rc) { s->err = e->err; return rc; }
+    s->dirty = false;
+    return MHD3D_OK;
+}
+
 extern "C" int mhd3d_server_tick(mhd3d_server* s, int* n_active, int* n_queued)
 {
     if (!s) return MHD3D_EINVAL;
-    int rc = srv_admit(s);
-    if (rc) return rc;
+    int rc = MHD3D_OK;
+    if (s->active.empty()) {
+        rc = srv_admit(s);
+        if (rc) return rc;
+    }
     if (!s->active.empty()) {
         mhd3d_ensemble* e = s->ens;
         if (s->dirty) {
-            e->mem.clear();
-            e->cells = 0;
-            for (int64_t id : s->active) {
-                mhd3d_ctx* c = s->reqs[id].ctx;
-                c->cur = e->mixed ? 0 : e->parity;   // members share the ensemble's buffer parity
-                e->mem.push_back(c);
-                e->cells += (int64_t)c->g.nPx * c->g.nPy * c->g.nPz;
-            }
-            e->n = (int)e->mem.size();
-            rc = ens_build(e);
-            if (rc) { s->err = e->err; return rc; }
-            s->dirty = false;
+            rc = srv_rebuild(s);
+            if (rc) return rc;
         }
-        int k = s->quantum;
-        for (int64_t id : s->active) {
-            const ServerReq& r = s->reqs[id];
-            if (r.nsteps - r.done < k) k = r.nsteps - r.done;
-        }
-        rc = mhd3d_ensemble_step(e, k);
+        // one quantum for every active run; each stops on the device at its
+        // own step count (set_stop_kernel), so a run near its end does not
+        // shorten the others' quantum
+        const int k = s->quantum;
+        rc = ens_step_enqueue(e, k);
+        if (rc) { s->err = e->err; return rc; }
+        // admission overlaps the quantum on the device: new runs are created
+        // and initialised on their own streams and join at the next tick
+        int rca = srv_admit(s);
+        rc = ens_step_finish(e, k);
+        if (rc) { s->err = e->err; return rc; }
         const double now = srv_now(s);
-        if (rc == MHD3D_ENONFINITE) rc = MHD3D_OK;   // as the reference's driver, keep stepping
         std::vector<int64_t> keep;
-        for (int m = 0; m < (int)s->active.size(); ++m) {
+        for (int m = 0; m < e->n; ++m) {
             ServerReq& r = s->reqs[s->active[m]];
             const DevScalars& hs = e->last[m];
             r.dt_last = hs.dt_last;
             const int now_done = (int)hs.nit;
             s->cell_steps += (int64_t)(now_done - r.done) * r.ctx->g.nPx * r.ctx->g.nPy * r.ctx->g.nPz;
             r.done = now_done;
-            if (hs.halted) {   // the reference's driver halts such a run (3dmhd.cpp:292-330)
+            if (hs.halted & HALT_GUARD) {   // the reference's driver halts such a run (3dmhd.cpp:292-330)
                 r.status = MHD3D_REQ_FAILED;
                 r.t_finish = now;
-                continue;
-            }
-            if (r.done >= r.nsteps) {
+            } else if (hs.nonfinite) {      // evicted: its state holds NaN/Inf
+                r.status = MHD3D_REQ_NONFINITE;
+                r.t_finish = now;
+            } else if (r.done >= r.nsteps) {
                 r.status = MHD3D_REQ_DONE;
+                r.cur = r.ctx->cur;
                 r.t_finish = now;
             } else {
                 keep.push_back(r.id);
             }
         }
-        if (rc) { s->err = e->err; return rc; }
-        if (keep.size() != s->active.size()) s->dirty = true;
+        for (int64_t id : s->active) {
+            ServerReq& r = s->reqs[id];
+            if (r.status == MHD3D_REQ_RUNNING) continue;
+            s->dirty = true;
+        }
+        // runs admitted during the quantum follow the runs still active
+        std::vector<int64_t> joined(s->active.begin() + e->n, s->active.end());
         s->active.swap(keep);
+        s->active.insert(s->active.end(), joined.begin(), joined.end());
+        if (!joined.empty()) s->dirty = true;
+        if (rca) return rca;
     }
     if (n_active) *n_active = (int)s->active.size();
     if (n_queued) *n_queued = (int)s->queue.size();
@@ -1941,10 +2059,13 @@ extern "C" int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* f
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) { s->err = "unknown request"; return MHD3D_EINVAL; }
     ServerReq& r = it->second;
-    if (r.status != MHD3D_REQ_DONE && r.status != MHD3D_REQ_FAILED) { s->err = "request not finished"; return MHD3D_EINVAL; }
+    if (r.status != MHD3D_REQ_DONE && r.status != MHD3D_REQ_FAILED && r.status != MHD3D_REQ_NONFINITE) {
+        s->err = "request not finished";
+        return MHD3D_EINVAL;
+    }
     int rc = MHD3D_OK;
     if (fields && r.status == MHD3D_REQ_DONE) {
-        r.ctx->cur = s->ens->mixed ? 0 : s->ens->parity;
+        r.ctx->cur = r.cur;   // other runs' ticks since it finished do not move it
         rc = mhd3d_get_state(r.ctx, fields);
         if (rc) s->err = r.ctx->err;
     }
@@ -1983,6 +2104,53 @@ extern "C" int mhd3d_calibration_copy(int64_t bytes, int reps, double* ms_per_co
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (ms_per_copy) *ms_per_copy = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
+// Streaming ceiling (stream_ceiling_kernel): `reps` launches moving nr input
+// and nw output arrays of `bytes_per_array` bytes each, 4 independent 16-byte
+// loads per thread and array in flight, 2048 blocks of 256 threads (8 per CU
+// on 256 CUs); *ms_per_launch: event-timed average.
+extern "C" int mhd3d_stream_ceiling(int64_t bytes_per_array, int nr, int nw, int reps, double* ms_per_launch)
+{
+    if (bytes_per_array < 16 || reps < 1 || nr < 1 || nr > 2 || nw < 0 || nw > 2) return MHD3D_EINVAL;
+    const int64_t n = bytes_per_array / 16;
+    dvec2 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MHD3D_OK;
+    if (hipMalloc(&a, n * 16 * nr) != hipSuccess || hipMalloc(&b, n * 16 * (nw > 0 ? nw : 1)) != hipSuccess)
+        rc = MHD3D_ENOMEM;
+    if (!rc && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = MHD3D_EHIP;
+    if (!rc && (hipMemsetAsync(a, 0, n * 16 * nr, st) != hipSuccess ||
+                hipMemsetAsync(b, 0, n * 16 * (nw > 0 ? nw : 1), st) != hipSuccess))
+        rc = MHD3D_EHIP;
+    auto launch = [&]() {
+        const dim3 grd(2048), blk(256);
+        if (nr == 1 && nw == 0) stream_ceiling_kernel<1, 0, 4><<<grd, blk, 0, st>>>(a, b, n);
+        else if (nr == 1 && nw == 1) stream_ceiling_kernel<1, 1, 4><<<grd, blk, 0, st>>>(a, b, n);
+        else if (nr == 1 && nw == 2) stream_ceiling_kernel<1, 2, 4><<<grd, blk, 0, st>>>(a, b, n);
+        else if (nr == 2 && nw == 0) stream_ceiling_kernel<2, 0, 4><<<grd, blk, 0, st>>>(a, b, n);
+        else if (nr == 2 && nw == 1) stream_ceiling_kernel<2, 1, 4><<<grd, blk, 0, st>>>(a, b, n);
+        else stream_ceiling_kernel<2, 2, 4><<<grd, blk, 0, st>>>(a, b, n);
+    };
+    if (!rc) {
+        launch();   // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) launch();
+        (void)hipEventRecord(e1, st);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = MHD3D_EHIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms_per_launch) *ms_per_launch = ms / reps;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

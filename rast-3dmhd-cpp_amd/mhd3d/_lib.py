@@ -51,7 +51,9 @@ SIGNATURES = {
     "mhd3d_dump_param": (C.c_int, [C.c_char_p, C.c_int, _dp]),
     "mhd3d_load_param": (C.c_int, [C.c_char_p, C.c_int, _dp]),
     "mhd3d_calibration_copy": (C.c_int, [C.c_int64, C.c_int, _dp]),
+    "mhd3d_stream_ceiling": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.c_int, _dp]),
     "mhd3d_ensemble_set_dt_guard": (C.c_int, [_vp, C.c_double]),
+    "mhd3d_ensemble_set_dt_guard_member": (C.c_int, [_vp, C.c_int, C.c_double]),
     "mhd3d_ensemble_halted": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "mhd3d_init_stratified": (C.c_int, [_vp, _dp, _dp, C.c_uint64, C.c_double, C.c_double, C.c_double]),
     "mhd3d_ensemble_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
@@ -81,7 +83,7 @@ SIGNATURES = {
     "mhd3d_server_result": (C.c_int, [_vp, C.c_int64, _dpp]),
 }
 
-REQ_FAILED, REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_RELEASED = -1, 0, 1, 2, 3
+REQ_NONFINITE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_RELEASED = -2, -1, 0, 1, 2, 3
 
 
 class RequestInfo(C.Structure):

@@ -95,8 +95,12 @@ class Ensemble:
                        ms_per_step=float(rep[4 * k + 3])) for k in range(n.value)]
         return self.variant(), trials
 
-    def set_dt_guard(self, dt_min=L.REFERENCE_DT_MIN):
-        self._check(L.lib().mhd3d_ensemble_set_dt_guard(self.h, float(dt_min)))
+    def set_dt_guard(self, dt_min=L.REFERENCE_DT_MIN, member=None):
+        """The DT guard for every member, or for one member."""
+        if member is None:
+            self._check(L.lib().mhd3d_ensemble_set_dt_guard(self.h, float(dt_min)))
+        else:
+            self._check(L.lib().mhd3d_ensemble_set_dt_guard_member(self.h, int(member), float(dt_min)))
 
     def halted(self, i):
         h = C.c_int()

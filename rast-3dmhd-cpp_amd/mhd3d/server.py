@@ -2,7 +2,9 @@
 
 The server admits runs first-come-first-served while their device memory fits
 an HBM budget, keeps their state on the device and advances all admitted runs
-together, a quantum of steps per tick (continuous batching).  `serve_trace`
+together, a quantum of steps per tick, each run stopping on the device at its
+own step count (continuous batching); runs whose state turns non-finite are
+evicted.  `serve_trace`
 replays a timed arrival trace against it and reports completion latencies and
 aggregate throughput; `sequential_trace` runs the same trace one run at a
 time in the plain form, the baseline.
@@ -108,6 +110,7 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
     lat = {}
     results = {}
     failed = []
+    evicted = []
     peak_bytes = 0
     while len(lat) < len(specs):
         now = srv.now() - t0
@@ -123,13 +126,15 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
         peak_bytes = max(peak_bytes, srv.bytes_in_use)
         for rid in list(open_ids):
             info = srv.poll(rid)
-            if info["status"] in (L.REQ_DONE, L.REQ_FAILED):
+            if info["status"] in (L.REQ_DONE, L.REQ_FAILED, L.REQ_NONFINITE):
                 i = ids[rid]
                 lat[i] = info["t_finish"] - t0 - specs[i].arrival
-                results[i] = srv.result(rid, fetch=fetch)
+                results[i] = srv.result(rid, fetch=fetch and info["status"] == L.REQ_DONE)
                 open_ids.discard(rid)
                 if info["status"] == L.REQ_FAILED:
                     failed.append(i)
+                elif info["status"] == L.REQ_NONFINITE:
+                    evicted.append(i)
         if na == 0 and nq == 0 and nxt < len(order):
             wait = specs[order[nxt]].arrival - (srv.now() - t0)
             if wait > 0:
@@ -142,7 +147,7 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
     return {"p50_s": _percentile(v, 50), "p99_s": _percentile(v, 99), "max_s": max(v),
             "mcell_steps_per_s": cell_steps / (t_end - t_begin) / 1e6, "makespan_s": t_end - t_begin,
             "cell_steps": cell_steps, "peak_device_bytes": peak_bytes, "failed": sorted(failed),
-            "latencies": v, "results": results}
+            "evicted_nonfinite": sorted(evicted), "latencies": v, "results": results}
 
 
 def sequential_trace(specs, precision=0):
