@@ -149,6 +149,19 @@ int mhd3d_get_state(mhd3d_ctx* ctx, double* const* fields);
  * gives the steps done). */
 int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout);
 
+/* mhd3d_step without device allocation.  No counterpart in the reference
+ * (its step(), 3dmhdsub.h:31, works on static arrays): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments, results
+ * and status codes as mhd3d_step, bit for bit; the context and its state
+ * are the caller's (created beforehand).  Its TSTEP partial records live in
+ * one process-wide array of 32768 records (1.5 MiB) per GPU instead of the
+ * context's: grids that need more (nPx*nPy beyond about 1M points) get
+ * MHD3D_EINVAL with a message, as does a call with kernel timing enabled or
+ * from a device other than the context's.  Calls on one GPU share that
+ * array: calls that overlap in time (other streams or threads) race. */
+int mhd3d_step_noalloc(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout);
+
 /* DT guard of the reference's driver (3dmhd.cpp:292-330): after any step
  * with DT < dt_min the run halts -- later steps do nothing until the next
  * set_state.  dt_min = 0 (the default) disables it; MHD3D_REFERENCE_DT_MIN is

@@ -871,7 +871,7 @@ static int collect_timing(mhd3d_ctx* ctx)
     return MHD3D_OK;
 }
 
-extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
+static int step_core(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
 {
     if (!ctx || nsteps < 0) return MHD3D_EINVAL;
     if (fixed_dt && !dt_inout) return fail(ctx, MHD3D_EINVAL, "fixed_dt needs *dt_inout");
@@ -941,6 +941,84 @@ extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_i
     if (hs.halted & HALT_GUARD) return fail(ctx, MHD3D_EHALTED, "DT too small, halt (DT guard)");
     if (hs.nonfinite) return fail(ctx, MHD3D_ENONFINITE, "state is not finite after the step");
     return MHD3D_OK;
+}
+
+extern "C" int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
+{
+    return step_core(ctx, nsteps, fixed_dt, dt_inout);
+}
+
+// ---------------------------------------------------------------------------
+// Step without device allocation (mhd3d_step_noalloc): the TSTEP partial
+// records of the standalone and fused passes live in `scratch`, a file-scope
+// device array each GPU holds once for the whole process, instead of the
+// context's records that mhd3d_step allocates on first use.  Host code
+// never passes `scratch` itself (its host-side name is no device address):
+// the device address comes from hipGetSymbolAddress, once per device, and
+// is kept in g_scratch_dev.  Every partial record a kernel reads was written
+// earlier in the same call, so nothing in it needs clearing; calls that
+// overlap on one GPU (other streams, other threads) share the array and
+// race.
+
+#include <mutex>
+
+constexpr int kScratchParts = 32768;   // 32768 records x 48 B = 1.5 MiB per GPU
+static __device__ __align__(256) CflPart scratch[kScratchParts];
+
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static CflPart* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once).
+static int scratch_records(mhd3d_ctx* ctx, CflPart** out)
+{
+    int dev = -1;
+    HIPCHK(ctx, hipGetDevice(&dev));
+    if (dev != ctx->device)
+        return fail(ctx, MHD3D_EINVAL, "mhd3d_step_noalloc: the context's device is not the current device");
+    if (dev < 0 || dev >= kMaxDevices) return fail(ctx, MHD3D_EINVAL, "mhd3d_step_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ctx, MHD3D_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<CflPart*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_step_noalloc(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
+{
+    if (!ctx || nsteps < 0) return MHD3D_EINVAL;
+    if (ctx->timing)
+        return fail(ctx, MHD3D_EINVAL, "mhd3d_step_noalloc: kernel timing creates events; disable it first");
+    // records the call needs: the fused pass (streaming blocks + wall
+    // blocks) or the standalone TSTEP pass, whichever is larger
+    int ns, nw;
+    cfl_part_layout(ctx, ns, nw);
+    const Geom& g = ctx->g;
+    const int nsa = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * (g.nPz < 8 ? g.nPz : 8);
+    const int need = std::max(ns + nw, nsa);
+    if (need > kScratchParts)
+        return fail(ctx, MHD3D_EINVAL,
+                    "mhd3d_step_noalloc: this grid needs " + std::to_string(need) +
+                        " TSTEP partial records; the no-alloc step holds at most " +
+                        std::to_string(kScratchParts) + " (nPx*nPy up to about 1M points)");
+    CflPart* rec = nullptr;
+    int rc = scratch_records(ctx, &rec);
+    if (rc) return rc;
+    // the context's records are set aside for the call (no allocation:
+    // the capacity covers every pass of the call)
+    CflPart* own = ctx->cfl_parts;
+    const int own_cap = ctx->cfl_parts_cap;
+    ctx->cfl_parts = rec;
+    ctx->cfl_parts_cap = kScratchParts;
+    rc = step_core(ctx, nsteps, fixed_dt, dt_inout);
+    ctx->cfl_parts = own;
+    ctx->cfl_parts_cap = own_cap;
+    return rc;
 }
 
 extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)

@@ -30,6 +30,7 @@ SIGNATURES = {
     "mhd3d_set_state": (C.c_int, [_vp, _dpp]),
     "mhd3d_get_state": (C.c_int, [_vp, _dpp]),
     "mhd3d_step": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
+    "mhd3d_step_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _dp]),
     "mhd3d_fluxes": (C.c_int, [_vp, _dpp]),
     "mhd3d_stage": (C.c_int, [_vp, C.c_int, C.c_double]),
     "mhd3d_bcon": (C.c_int, [_vp]),
