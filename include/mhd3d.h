@@ -66,9 +66,13 @@ enum {
  */
 typedef struct mhd3d_desc {
     int32_t nPx, nPy, nPz;
-    int32_t lRot, lMag, lPot, lRem, lShr;        /* 3dmhdparam.h:8-17         */
-    int32_t ixCon, iyCon, izCon, iTCon, iBCon;   /* 3dmhdparam.h:86-107       */
-    int32_t iD;                                  /* 3dmhdparam.h:118          */
+    /* the switches lRot, lMag, lPot, lRem, lShr (3dmhdparam.h:8-17), ixCon,
+     * iyCon, izCon, iTCon, iBCon (:86-107) and iD (:118); the flag_ prefix
+     * keeps the names apart from the reference's macros of the same names,
+     * so this header compiles inside the reference's translation unit */
+    int32_t flag_lRot, flag_lMag, flag_lPot, flag_lRem, flag_lShr;
+    int32_t flag_ixCon, flag_iyCon, flag_izCon, flag_iTCon, flag_iBCon;
+    int32_t flag_iD;
     int32_t precision;                           /* MHD3D_FP64 / MHD3D_MIXED  */
     int32_t reserved;
     /* GRID / RUNGKU / SFF (3dmhd.cpp:354-369, 3dmhdparam.h:111) */
@@ -340,7 +344,7 @@ int mhd3d_load_param(const char* path, int idx, double* par1);
  * 2.0f/par[7]; 3dmhdphsic.h:682-699), or B = 0 (tube() with nTube = 0).
  * Host-only; needs no GPU. */
 int mhd3d_static_state(int nPx, int nPy, int nPz_global, int nPEy, int nPEz, const double* par,
-                       int lMag, double* const* fields);
+                       int mag, double* const* fields);
 
 /* Static() with the lShr and lRem switches as well (3dmhdphsic.h:565-616,
  * 737-742): lShr starts from T = RO = 1 and puts the RAN2 perturbation into
@@ -351,7 +355,7 @@ int mhd3d_static_state(int nPx, int nPy, int nPz_global, int nPEy, int nPEz, con
  * central differences), and walls = {TU, DZTU, DZTB} for BCON; RKAPA, DKAPA
  * and walls may be NULL without lRem.  lShr with lRem: MHD3D_EUNSUPPORTED. */
 int mhd3d_static_state_flags(int nPx, int nPy, int nPz_global, int nPEy, int nPEz,
-                             const double* par, int lMag, int lShr, int lRem,
+                             const double* par, int mag, int shr, int rem,
                              double* const* fields, double* RKAPA, double* DKAPA, double* walls);
 
 /* Measurement support (no counterpart in the reference): `reps` launches of

@@ -355,7 +355,7 @@ void orc_communicate(orc_ctx* c)
     periodic_fill(c, c->RW);
     periodic_fill(c, c->TT);
     periodic_fill(c, c->RO);
-    if (c->d.lMag) {
+    if (c->d.flag_lMag) {
         periodic_fill(c, c->BX);
         periodic_fill(c, c->BY);
         periodic_fill(c, c->BZ);
@@ -393,7 +393,7 @@ void orc_fluxes(orc_ctx* c)
     const mhd3d_desc* d = &c->d;
     const rng_t R = R_in(c), RE = R_ext(c), REXY = R_exy(c), REX = R_ex(c), REY = R_ey(c);
     const int zl = 1, zr = c->nz - 1, yl = 1, yr = c->ny - 1, xl = 1, xr = c->nx - 1;
-    const int shr = d->lShr != 0;
+    const int shr = d->flag_lShr != 0;
     double tmp;
     double *RU = c->RU, *RV = c->RV, *RW = c->RW, *RO = c->RO, *TT = c->TT, *RI = c->RI;
     double *UU = c->UU, *VV = c->VV, *WW = c->WW;
@@ -420,7 +420,7 @@ void orc_fluxes(orc_ctx* c)
     f_base(c, SUB, R, FR, WW1, 1);
 
     /* density diffusion :229-264 */
-    if (d->iD != 0 || d->lRem) {
+    if (d->flag_iD != 0 || d->flag_lRem) {
         const size_t pl = (size_t)c->nx * c->ny;
         if (!MUT(9)) {
             memcpy(&RO[IX(0, 0, 0)], &RO[IX(2, 0, 0)], pl * sizeof(double));               /* :230-234 */
@@ -430,8 +430,8 @@ void orc_fluxes(orc_ctx* c)
         f_base(c, SET, RE, WW1, RO, 1);
         if (!MUT(10)) f_base_ext(c, SUB, RE, WW1, c->ROM, 1);
     }
-    if (d->iD != 0) {
-        if (d->iD == 1) {
+    if (d->flag_iD != 0) {
+        if (d->flag_iD == 1) {
             for (int z = 0; z < c->nz; z++) c->WWZ[z] = 1.0 / c->ROM[z];
         } else {
             const double SGM = 1.0;
@@ -479,12 +479,12 @@ void orc_fluxes(orc_ctx* c)
         f_d1_mul(c, SUB, DY, R, FT, TT, VV, 1);
         f_d1_mul(c, SUB, DZ, R, FT, TT, WW, 1);
     }
-    if (!shr && d->lRem) {                                             /* :297-302 */
+    if (!shr && d->flag_lRem) {                                             /* :297-302 */
         hmean(c, c->TTM, TT);
         LOOP(RE, WW1[i] = TT[i] - c->TTM[z]);
     }
     tmp = OCV / d->REPR;
-    if (d->lRem) {                                                     /* :304-313 */
+    if (d->flag_lRem) {                                                     /* :304-313 */
         f_d2_mul(c, ADD, DX, R, FT, WW1, RI, tmp);
         f_d2_mul(c, ADD, DY, R, FT, WW1, RI, tmp);
         f_d2_mul(c, ADD, DZ, R, FT, WW1, RI, tmp);
@@ -501,7 +501,7 @@ void orc_fluxes(orc_ctx* c)
         f_d2_mul_az(c, ADD, DZ, R, FT, TT, RI, c->RKAPA, tmp);
         f_d1_mul_az(c, ADD, DZ, R, FT, TT, RI, c->DKAPA, tmp);
     }
-    if (d->iTCon == 2) {                                               /* :443-461 */
+    if (d->flag_iTCon == 2) {                                               /* :443-461 */
         const double HH = d->HH, CLN = -4.0 * ln2f() / HH / HH;
         const double *EXX = c->EXX, *WYY = c->WYY, *ZEE = c->ZEE;
         if (d->TC != 0.0) {
@@ -580,7 +580,7 @@ void orc_fluxes(orc_ctx* c)
     }
 
     /* Coriolis :540-545 */
-    if (d->lRot) {
+    if (d->flag_lRot) {
         f_base(c, ADD, R, FU, RV, d->OMZ);
         f_base(c, SUB, R, FV, RU, d->OMZ);
         f_base(c, ADD, R, FV, RW, d->OMX);
@@ -588,7 +588,7 @@ void orc_fluxes(orc_ctx* c)
     }
 
     /* induction, Lorentz force, Joule heating :547-695 */
-    if (d->lMag) {
+    if (d->flag_lMag) {
         double *BX = c->BX, *BY = c->BY, *BZ = c->BZ;
         double *S1 = c->S1, *S2 = c->S2, *S3 = c->S3, *S4 = c->S4;
         const double OBETA = d->OBETA, ORM = d->ORM;
@@ -729,7 +729,7 @@ void orc_bcon(orc_ctx* c)
     /* top wall, myPEz == 0 */
     for (int y = yl; y < yr; y++)
         for (int x = xl; x < xr; x++) {
-            if (d->lShr) {
+            if (d->flag_lShr) {
                 RU[IX(zl, y, x)] = 0.0;
                 RV[IX(zl, y, x)] = RRT * cos(RPP * c->TIMT) * RO[IX(zl, y, x)];
             } else {
@@ -741,10 +741,10 @@ void orc_bcon(orc_ctx* c)
         }
     for (int y = yl; y < yr; y++)
         for (int x = xl; x < xr; x++) RW[IX(zl, y, x)] = 0.0;
-    if (d->TP != 0.0 && (d->iTCon == 0 || d->iTCon == 1)) {             /* :750-776 */
+    if (d->TP != 0.0 && (d->flag_iTCon == 0 || d->flag_iTCon == 1)) {             /* :750-776 */
         const double TPR = d->TC != 0.0 ? d->TP + (1.0 - d->TP) * exp(-c->TIMT / d->TC) : d->TP;
         const double HH = d->HH, CLN = -4.0 * ln2f() / HH / HH;
-        if (d->iTCon == 0) {
+        if (d->flag_iTCon == 0) {
             for (int y = yl; y < yr; y++)
                 for (int x = xl; x < xr; x++)
                     TT[IX(zl, y, x)] = 1.0 - (1.0 - TPR) * exp(CLN * sq(c->EXX[x] - d->XP) / (MUT(14) ? HH : 1.0)) / HH *
@@ -757,35 +757,35 @@ void orc_bcon(orc_ctx* c)
                                        C23 * DZ * (d->THETA - (d->THETA - TPR) * exp(CLN * sq(c->EXX[x] - d->XP)) / HH *
                                                                   exp(CLN * sq(c->WYY[y] - d->YP)) / HH);
         }
-    } else if (d->iTCon == 0 || d->iTCon == 2 || d->iTCon == 4) {      /* :777-795 */
+    } else if (d->flag_iTCon == 0 || d->flag_iTCon == 2 || d->flag_iTCon == 4) {      /* :777-795 */
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) {
-                if (d->lRem) {
+                if (d->flag_lRem) {
                     TT[IX(zl, y, x)] = d->TU;
                 } else {
                     TT[IX(zl, y, x)] = 1.0;
-                    if (d->iTCon == 4 && x < nx / 2) TT[IX(zl, y, x)] = 1.0 - d->TP;
+                    if (d->flag_iTCon == 4 && x < nx / 2) TT[IX(zl, y, x)] = 1.0 - d->TP;
                 }
             }
-    } else if (d->iTCon == 1) {                                          /* :796-809 */
+    } else if (d->flag_iTCon == 1) {                                          /* :796-809 */
         const double DZ = d->ORZF / c->dzzdz[zl];
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) {
-                if (d->lRem)
+                if (d->flag_lRem)
                     TT[IX(zl, y, x)] = C43 * TT[IX(zl + 1, y, x)] - C13 * TT[IX(zl + 2, y, x)] + d->DZTU;
                 else
                     TT[IX(zl, y, x)] = C43 * TT[IX(zl + 1, y, x)] - C13 * TT[IX(zl + 2, y, x)] -
                                        C23 * DZ * d->THETA;
             }
     }
-    if (d->lMag && d->iBCon == 0)
+    if (d->flag_lMag && d->flag_iBCon == 0)
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) BZ[IX(zl, y, x)] = 0.0;
 
     /* bottom wall, myPEz == nPEz-1 */
     for (int y = yl; y < yr; y++)
         for (int x = xl; x < xr; x++) {
-            if (d->lShr) {
+            if (d->flag_lShr) {
                 RU[IX(zr - 1, y, x)] = 0.0;
                 RV[IX(zr - 1, y, x)] = RRB * cos(RPP * c->TIMT) * RO[IX(zr - 1, y, x)];
             } else {
@@ -797,10 +797,10 @@ void orc_bcon(orc_ctx* c)
         }
     for (int y = yl; y < yr; y++)
         for (int x = xl; x < xr; x++) RW[IX(zr - 1, y, x)] = 0.0;
-    if (d->izCon == 0) {                                                 /* :842-847 */
+    if (d->flag_izCon == 0) {                                                 /* :842-847 */
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) TT[IX(zr - 1, y, x)] = d->TB;
-    } else if (d->lRem) {                                                /* :850-854 */
+    } else if (d->flag_lRem) {                                                /* :850-854 */
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++)
                 TT[IX(zr - 1, y, x)] = C43 * TT[IX(zr - 2, y, x)] - C13 * TT[IX(zr - 3, y, x)] + d->DZTB;
@@ -811,7 +811,7 @@ void orc_bcon(orc_ctx* c)
                 TT[IX(zr - 1, y, x)] = C43 * TT[IX(zr - 2, y, x)] - (MUT(7) ? C23 : C13) * TT[IX(zr - 3, y, x)] -
                                        C23 * DZ * d->THETA / c->RKAPA[zr - 1];
     }
-    if (d->lMag && d->iBCon == 0)
+    if (d->flag_lMag && d->flag_iBCon == 0)
         for (int y = yl; y < yr; y++)
             for (int x = xl; x < xr; x++) BZ[IX(zr - 1, y, x)] = 0.0;
 }
@@ -841,7 +841,7 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
                 um = smax(um, OGAMMA * UU[i] / TT[i]);
             }
     c->UMACH = sqrt(um);
-    if (d->lMag) {
+    if (d->flag_lMag) {
         LOOP(R, UU[i] += GAMMA * TT[i] + (BX[i] * BX[i] + BY[i] * BY[i] + BZ[i] * BZ[i]) * OBETA / RO[i] +
                          2.0 * sqrt(UU[i] * (GAMMA * TT[i] + (BX[i] * BX[i] + BY[i] * BY[i] + BZ[i] * BZ[i]) * OBETA / RO[i])));
     } else {
@@ -853,12 +853,12 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
         double DD = smin(DDY[y], DDZ[z]);
         if (xdim) DD = smin(DD, DDX[x]);
         WW1[i] = DD / sqrt(UU[i]);
-        if (d->lRem)                                                    /* :78-82 */
+        if (d->flag_lRem)                                                    /* :78-82 */
             WW2[i] = 0.5 * DD * DD * REPR * CV * RO[i] / ((MUT(15) ? 0.0 : 1.0) + RKAPA[z]);
         else
             WW2[i] = 0.5 * DD * DD * REPR * CV * RO[i] / RKAPA[z];
         WW3[i] = 0.375 * DD * DD * RE * RO[i];
-        if (d->lMag) VV[i] = 0.5 * DD * DD * (MUT(18) ? RE : RM);
+        if (d->flag_lMag) VV[i] = 0.5 * DD * DD * (MUT(18) ? RE : RM);
     });
     double w[4] = {1e99, 1e99, 1e99, 1e99};
     for (int z = R.zl; z < R.zr; ++z)
@@ -868,10 +868,10 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
                 w[0] = smin(w[0], WW1[i]);
                 w[1] = smin(w[1], WW2[i]);
                 w[2] = smin(w[2], WW3[i]);
-                if (d->lMag) w[3] = smin(w[3], VV[i]);
+                if (d->flag_lMag) w[3] = smin(w[3], VV[i]);
             }
     double DT;
-    if (d->lMag)
+    if (d->flag_lMag)
         DT = d->SFF * smin(smin(w[0], w[1]), smin(w[2], w[3]));
     else
         DT = d->SFF * smin(smin(w[0], w[1]), w[2]);
@@ -887,7 +887,7 @@ void orc_stage(orc_ctx* c, int k, double DT)
 {
     const mhd3d_desc* d = &c->d;
     const rng_t R = R_in(c);
-    const int mag = d->lMag != 0;
+    const int mag = d->flag_lMag != 0;
     if (k == 1) {                                                      /* :155-166 */
         f_copy(c, R, c->ZRU, c->RU);
         f_copy(c, R, c->ZRV, c->RV);

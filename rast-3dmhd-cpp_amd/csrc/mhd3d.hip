@@ -106,13 +106,13 @@ static int fail(mhd3d_ctx* ctx, int code, const std::string& msg)
 static int ext_flags(const mhd3d_desc* d)
 {
     int f = 0;
-    if (d->iD != 0 && d->DH != 0.0) f |= EXT_DH;   // `#if DH != 0.0` (3dmhdsub.h:253)
-    if (d->iD != 0 && d->DV != 0.0) f |= EXT_DV;   // `#if DV != 0.0` (:259)
-    if (d->lRem) f |= EXT_REM;
-    if (d->lShr) f |= EXT_SHR;
-    if (d->iTCon == 2) f |= EXT_HEAT;
-    if ((d->iTCon == 0 || d->iTCon == 1) && d->TP != 0.0) f |= EXT_PLUME;
-    if (d->iD != 0 || d->lRem) f |= EXT_ROCOPY;
+    if (d->flag_iD != 0 && d->DH != 0.0) f |= EXT_DH;   // `#if DH != 0.0` (3dmhdsub.h:253)
+    if (d->flag_iD != 0 && d->DV != 0.0) f |= EXT_DV;   // `#if DV != 0.0` (:259)
+    if (d->flag_lRem) f |= EXT_REM;
+    if (d->flag_lShr) f |= EXT_SHR;
+    if (d->flag_iTCon == 2) f |= EXT_HEAT;
+    if ((d->flag_iTCon == 0 || d->flag_iTCon == 1) && d->TP != 0.0) f |= EXT_PLUME;
+    if (d->flag_iD != 0 || d->flag_lRem) f |= EXT_ROCOPY;
     return f;
 }
 
@@ -126,30 +126,30 @@ static int validate(const mhd3d_desc* d, std::string& why)
         why = "metric array pointer missing";
         return MHD3D_EINVAL;
     }
-    if (d->ixCon != 0 || d->iyCon != 0) { why = "fluxes: Non-periodic horizontal boundaries (3dmhdsub.h:208)"; return MHD3D_EUNSUPPORTED; }
+    if (d->flag_ixCon != 0 || d->flag_iyCon != 0) { why = "fluxes: Non-periodic horizontal boundaries (3dmhdsub.h:208)"; return MHD3D_EUNSUPPORTED; }
     if (d->RLAX != 0.0) {
         why = "RLAX relaxation is refused: the reference exits in it (\"yk not implement\", 3dmhdsub.h:427-439) "
               "or rejects the wall combination (:344,416)";
         return MHD3D_EUNSUPPORTED;
     }
-    if (d->lShr && d->lRem) {
+    if (d->flag_lShr && d->flag_lRem) {
         why = "lShr with lRem: the reference's lShr skips the TTM update lRem's conduction reads (3dmhdsub.h:293-303)";
         return MHD3D_EUNSUPPORTED;
     }
-    if (d->iD < 0 || d->iD > 2) { why = "iD must be 0, 1 or 2 (3dmhdsub.h:243-252)"; return MHD3D_EINVAL; }
-    if (d->iTCon < 0 || d->iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
-    if (d->izCon != 0 && d->izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
+    if (d->flag_iD < 0 || d->flag_iD > 2) { why = "iD must be 0, 1 or 2 (3dmhdsub.h:243-252)"; return MHD3D_EINVAL; }
+    if (d->flag_iTCon < 0 || d->flag_iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
+    if (d->flag_izCon != 0 && d->flag_izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
     const int ef = ext_flags(d);
     if (ef && d->precision == MHD3D_MIXED) {
         why = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) are built for the FP64 form only";
         return MHD3D_EUNSUPPORTED;
     }
-    if (((ef & (EXT_DH | EXT_DV)) && d->iD == 2 && !d->ZEE) ||
+    if (((ef & (EXT_DH | EXT_DV)) && d->flag_iD == 2 && !d->ZEE) ||
         ((ef & EXT_HEAT) && (!d->EXX || !d->WYY || !d->ZEE)) || ((ef & EXT_PLUME) && (!d->EXX || !d->WYY))) {
         why = "coordinate array (EXX, WYY or ZEE) missing";
         return MHD3D_EINVAL;
     }
-    if (d->lMag && d->iBCon != 0) { why = "iBCon != 0 is not built"; return MHD3D_EUNSUPPORTED; }
+    if (d->flag_lMag && d->flag_iBCon != 0) { why = "iBCon != 0 is not built"; return MHD3D_EUNSUPPORTED; }
     if (d->precision != MHD3D_FP64 && d->precision != MHD3D_MIXED) { why = "invalid precision"; return MHD3D_EINVAL; }
     return MHD3D_OK;
 }
@@ -176,7 +176,7 @@ static Phys make_phys(const mhd3d_desc* d)
     c.C43_OCV_ORE = d->C43 * d->OCV * d->ORE;
     c.KJ = d->OBETA * d->ORM * d->OCV;
     c.KJ2 = -2.0 * d->OBETA * d->ORM * d->OCV;
-    c.izCon = d->izCon; c.iTCon = d->iTCon; c.iBCon = d->iBCon;
+    c.izCon = d->flag_izCon; c.iTCon = d->flag_iTCon; c.iBCon = d->flag_iBCon;
     c.xdim = (d->nPx + 2) > 3;   // `nx > ix + 1` (3dmhdsub.h:77)
     return c;
 }
@@ -213,9 +213,9 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
     ExtArgs& X = ctx->xa;
     X = ExtArgs{};
     X.flags = ext_flags(d);
-    X.iD = d->iD;
-    X.lRem = d->lRem ? 1 : 0;
-    X.lShr = d->lShr ? 1 : 0;
+    X.iD = d->flag_iD;
+    X.lRem = d->flag_lRem ? 1 : 0;
+    X.lShr = d->flag_lShr ? 1 : 0;
     X.TU = d->TU;
     X.DZTU = d->DZTU;
     X.DZTB = d->DZTB;
@@ -239,7 +239,7 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
     double* px = hz + nz;
     double* py = px + nx;
     auto sq = [](double v) { return v * v; };
-    if ((X.flags & (EXT_DH | EXT_DV)) && d->iD == 2) {
+    if ((X.flags & (EXT_DH | EXT_DV)) && d->flag_iD == 2) {
         const double SGM = 1.0f;
         const double CLN = -4.0f * std::log(2.0f) / SGM / SGM;
         for (int z = 0; z < nz; ++z)
@@ -290,7 +290,7 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (rc != MHD3D_OK) { g_create_error = why; return rc; }
     mhd3d_ctx* ctx = new mhd3d_ctx();
     ctx->d = *desc;
-    ctx->nf = desc->lMag ? NF_MAG : NF_HYDRO;
+    ctx->nf = desc->flag_lMag ? NF_MAG : NF_HYDRO;
     Geom& g = ctx->g;
     g.nPx = desc->nPx; g.nPy = desc->nPy; g.nPz = desc->nPz;
     g.nz = desc->nPz + 2;
@@ -319,7 +319,7 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     put(desc->dzzdz, nz); put(desc->d2zzdz2, nz); put(desc->DDZ, nz);
     put(desc->RKAPA, nz); put(desc->DKAPA, nz);
     // TSTEP conduction denominator: RKAPA[z], or 1.0f + RKAPA[z] with lRem (3dmhdsub.h:78-82)
-    for (int z = 0; z < nz; ++z) hm[o++] = desc->lRem ? 1.0f + desc->RKAPA[z] : desc->RKAPA[z];
+    for (int z = 0; z < nz; ++z) hm[o++] = desc->flag_lRem ? 1.0f + desc->RKAPA[z] : desc->RKAPA[z];
     for (int z = 0; z < nz; ++z) {
         CflRec r{};
         r.DDZ = desc->DDZ[z];
@@ -397,15 +397,15 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (fc && std::atoi(fc) == 0) ctx->fuse_cfl = false;
     if (ctx->mixed) ctx->kernel_mode = 0;   // the mixed form has only the streaming kernel
     if (ctx->target_blocks == 0) {
-        const int fm = ctx->mixed ? 1 : 0, ph = desc->lMag ? 1 : 0;
+        const int fm = ctx->mixed ? 1 : 0, ph = desc->flag_lMag ? 1 : 0;
         const int w = ctx->tune_waves[fm][ph], sc = ctx->tune_sched[fm][ph];
         const void* fn;
         if (ctx->mixed)
-            fn = mixed_stream_fn(desc->lMag, w, sc);
+            fn = mixed_stream_fn(desc->flag_lMag, w, sc);
         else if (ext_variant(ctx->xa.flags) != EXTV_NONE)
-            fn = ext_stream_fn(desc->lMag, ext_variant(ctx->xa.flags));
+            fn = ext_stream_fn(desc->flag_lMag, ext_variant(ctx->xa.flags));
         else
-            fn = desc->lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
+            fn = desc->flag_lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
         ctx->target_blocks = resident_blocks(fn, STX * STY);
     }
     *out = ctx;
@@ -750,7 +750,7 @@ static void launch_wall_t(mhd3d_ctx* ctx, const StageArgs& a)
 template <int STAGE>
 static void launch_stage_k(mhd3d_ctx* ctx, const StageArgs& a, bool wall)
 {
-    const bool mag = ctx->d.lMag, rot = ctx->d.lRot;
+    const bool mag = ctx->d.flag_lMag, rot = ctx->d.flag_lRot;
     if (!wall) {
         if (mag && rot) launch_stage_t<true, true, STAGE>(ctx, a);
         else if (mag) launch_stage_t<true, false, STAGE>(ctx, a);
@@ -847,12 +847,12 @@ static int run_cfl(mhd3d_ctx* ctx)
     const int nparts = (int)(grd.x * grd.y * grd.z);
     int rc = ensure_cfl_parts(ctx, nparts);
     if (rc) return rc;
-    if (ctx->d.lMag)
+    if (ctx->d.flag_lMag)
         cfl_kernel<true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], ctx->cfl_parts);
     else
         cfl_kernel<false><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->U[ctx->cur], ctx->cfl_parts);
     cfl_finalize_kernel<<<1, 256, 0, ctx->stream>>>(ctx->cfl_parts, nparts, ctx->ds, ctx->c,
-                                                   ctx->d.lMag ? 1 : 0);
+                                                   ctx->d.flag_lMag ? 1 : 0);
     HIPCHK(ctx, hipGetLastError());
     return MHD3D_OK;
 }
@@ -906,7 +906,7 @@ static int step_core(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
             int ns, nw;
             cfl_part_layout(ctx, ns, nw);
             step_close_kernel<<<1, 256, 0, ctx->stream>>>(ctx->cfl_parts, ns + nw, ctx->ds, ctx->c,
-                                                         ctx->d.lMag ? 1 : 0);
+                                                         ctx->d.flag_lMag ? 1 : 0);
         } else {
             clock_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);
         }
@@ -1062,7 +1062,7 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
     StageArgs a = stage_args(ctx);
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz);
-    const bool mag = ctx->d.lMag, rot = ctx->d.lRot;
+    const bool mag = ctx->d.flag_lMag, rot = ctx->d.flag_lRot;
     if (ctx->ext) {
         int rc = run_means(ctx, a.Uin);
         if (rc) return rc;
@@ -1105,7 +1105,7 @@ extern "C" int mhd3d_bcon(mhd3d_ctx* ctx)
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
-    const bool mag = ctx->d.lMag;
+    const bool mag = ctx->d.flag_lMag;
     if (ctx->ext) {
         if (mag) bcon_kernel<true, true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
         else bcon_kernel<false, true><<<grd, blk, 0, ctx->stream>>>(g, ctx->c, ctx->m, ctx->xa, ctx->ds, ctx->U[ctx->cur]);
@@ -1371,7 +1371,7 @@ static int ens_build(mhd3d_ensemble* e)
     for (int m = 0; m < e->n; ++m) {
         mhd3d_ctx* c = e->mem[m];
         const Geom& g = c->g;
-        const int flags = (c->d.lMag ? 1 : 0) | (c->d.lRot ? 2 : 0);
+        const int flags = (c->d.flag_lMag ? 1 : 0) | (c->d.flag_lRot ? 2 : 0);
         StreamPlan p = stream_plan(g, 256);
         if (e->chunk > 0) {
             const int nint = g.nPz - 2;
@@ -1403,7 +1403,7 @@ static int ens_build(mhd3d_ensemble* e)
         if (rc) return ens_fail(e, rc, c->err);
         fin[m].parts = c->cfl_parts;
         fin[m].nparts = part;
-        fin[m].mag = c->d.lMag ? 1 : 0;
+        fin[m].mag = c->d.flag_lMag ? 1 : 0;
         fin[m].ds = c->ds;
         fin[m].mirror = e->mirror + m;
         fin[m].c = c->c;
@@ -1541,12 +1541,12 @@ static int ens_cfl_all(mhd3d_ensemble* e)
         dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, zchunks);
         const int nparts = (int)(grd.x * grd.y * grd.z);   // within the capacity ens_build ensured
         const double* U = c->U[c->cur];
-        if (c->d.lMag)
+        if (c->d.flag_lMag)
             cfl_kernel<true><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
         else
             cfl_kernel<false><<<grd, blk, 0, e->stream>>>(g, c->c, c->m, U, c->cfl_parts);
         cfl_finalize_kernel<<<1, 256, 0, e->stream>>>(c->cfl_parts, nparts, c->ds, c->c,
-                                                      c->d.lMag ? 1 : 0);
+                                                      c->d.flag_lMag ? 1 : 0);
     }
     ENSCHK(e, hipGetLastError());
     return MHD3D_OK;
@@ -1876,7 +1876,7 @@ static double srv_now(const mhd3d_server* s)
 // the TSTEP partial records of the fused and standalone passes).
 static int64_t ctx_bytes_estimate(const mhd3d_desc* d)
 {
-    const int nf = d->lMag ? NF_MAG : NF_HYDRO;
+    const int nf = d->flag_lMag ? NF_MAG : NF_HYDRO;
     const int64_t pitch = (d->nPx + 7) / 8 * 8;
     const int64_t plane = pitch * d->nPy;
     const int64_t fs = (plane * (d->nPz + 2) + 31) / 32 * 32;

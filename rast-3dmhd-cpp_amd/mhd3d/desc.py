@@ -34,10 +34,10 @@ class MhdDesc(C.Structure):
 
     _fields_ = [
         ("nPx", C.c_int32), ("nPy", C.c_int32), ("nPz", C.c_int32),
-        ("lRot", C.c_int32), ("lMag", C.c_int32), ("lPot", C.c_int32),
-        ("lRem", C.c_int32), ("lShr", C.c_int32),
-        ("ixCon", C.c_int32), ("iyCon", C.c_int32), ("izCon", C.c_int32),
-        ("iTCon", C.c_int32), ("iBCon", C.c_int32), ("iD", C.c_int32),
+        ("flag_lRot", C.c_int32), ("flag_lMag", C.c_int32), ("flag_lPot", C.c_int32),
+        ("flag_lRem", C.c_int32), ("flag_lShr", C.c_int32),
+        ("flag_ixCon", C.c_int32), ("flag_iyCon", C.c_int32), ("flag_izCon", C.c_int32),
+        ("flag_iTCon", C.c_int32), ("flag_iBCon", C.c_int32), ("flag_iD", C.c_int32),
         ("precision", C.c_int32), ("reserved", C.c_int32),
         ("C13", C.c_double), ("C23", C.c_double), ("C43", C.c_double),
         ("GAM1", C.c_double), ("GAM2", C.c_double), ("GAM3", C.c_double),
@@ -201,10 +201,10 @@ class Descriptor:
         m = self.metrics
         d = MhdDesc()
         d.nPx, d.nPy, d.nPz = cfg.nPx, cfg.nPy, cfg.nPz
-        d.lRot, d.lMag, d.lPot = int(cfg.lRot), int(cfg.lMag), int(cfg.lPot)
-        d.lRem, d.lShr = int(cfg.lRem), int(cfg.lShr)
-        d.ixCon, d.iyCon, d.izCon = cfg.ixCon, cfg.iyCon, cfg.izCon
-        d.iTCon, d.iBCon, d.iD = cfg.iTCon, cfg.iBCon, cfg.iD
+        d.flag_lRot, d.flag_lMag, d.flag_lPot = int(cfg.lRot), int(cfg.lMag), int(cfg.lPot)
+        d.flag_lRem, d.flag_lShr = int(cfg.lRem), int(cfg.lShr)
+        d.flag_ixCon, d.flag_iyCon, d.flag_izCon = cfg.ixCon, cfg.iyCon, cfg.izCon
+        d.flag_iTCon, d.flag_iBCon, d.flag_iD = cfg.iTCon, cfg.iBCon, cfg.iD
         d.precision = cfg.precision
         for k, v in self.consts.items():
             setattr(d, k, v)

@@ -1,0 +1,78 @@
+// mhd3d_shim.h -- the binding a maintainer adds to the reference program to
+// run its step() on the GPU through libmhd3d.so (include/mhd3d.h).
+//
+// Include it in the reference's 3dmhd.cpp after its own headers (it reads
+// the reference's globals: the arrays of 3dmhddata.h:8-17, the scalars
+// set() leaves, 3dmhd.cpp:353-495, and the compile-time switches of
+// 3dmhdparam.h:8-118), then in main (3dmhd.cpp:278-338) call gpu_init()
+// once before the time loop, gpu_step() where the loop calls step()
+// (3dmhd.cpp:280), and gpu_sync_host() before each dump().  Link with
+// -L<repo>/rast-3dmhd-cpp_amd/lib -lmhd3d and run with one MPI rank: the
+// library holds the whole domain, both z walls included.
+#ifndef MHD3D_SHIM_H_
+#define MHD3D_SHIM_H_
+
+#include "mhd3d.h"
+
+static mhd3d_ctx* g_gpu = nullptr;
+
+static void gpu_fields(double* f[8])
+{
+    f[0] = &RU[0][0][0]; f[1] = &RV[0][0][0]; f[2] = &RW[0][0][0];
+    f[3] = &RO[0][0][0]; f[4] = &TT[0][0][0];
+    f[5] = &BX[0][0][0]; f[6] = &BY[0][0][0]; f[7] = &BZ[0][0][0];   // read only with lMag
+}
+
+// after set() and the initial communicate()
+static void gpu_init()
+{
+    mhd3d_desc d = {};
+    d.nPx = nx - 2; d.nPy = ny - 2; d.nPz = nz - 2;
+    d.flag_lRot = lRot; d.flag_lMag = lMag; d.flag_lPot = lPot; d.flag_lRem = lRem; d.flag_lShr = lShr;
+    d.flag_ixCon = ixCon; d.flag_iyCon = iyCon; d.flag_izCon = izCon;
+    d.flag_iTCon = iTCon; d.flag_iBCon = iBCon; d.flag_iD = iD;
+    d.precision = MHD3D_FP64;
+    d.C13 = C13; d.C23 = C23; d.C43 = C43;
+    d.GAM1 = GAM1; d.GAM2 = GAM2; d.GAM3 = GAM3; d.ZETA1 = ZETA1; d.ZETA2 = ZETA2;
+    d.SFF = SFF; d.hx = hx; d.h2x = h2x; d.hy = hy; d.h2y = h2y; d.hz = hz; d.h2z = h2z;
+    d.ORZF = 1.0f / float(nPz - 1);   // the expression BCON uses for DZ (3dmhdsub.h:797,849)
+    d.RE = RE; d.ORE = ORE; d.REPR = REPR; d.CV = CV; d.OCV = OCV;
+    d.GAMMA = GAMMA; d.THETA = THETA; d.GRAV = GRAV; d.OMX = OMX; d.OMZ = OMZ;
+    d.RM = RM; d.ORM = ORM; d.OBETA = OBETA; d.TP = TP; d.TB = TB; d.TC = TC;
+    d.RLAX = RLAX; d.TIMI = TIMI;
+    d.dxxdx = dxxdx; d.d2xxdx2 = d2xxdx2; d.DDX = DDX;
+    d.dyydy = dyydy; d.d2yydy2 = d2yydy2; d.DDY = DDY;
+    d.dzzdz = dzzdz; d.d2zzdz2 = d2zzdz2; d.DDZ = DDZ;
+    d.RKAPA = RKAPA; d.DKAPA = DKAPA;
+    // off-default switches (read only when iD, lRem, iTCon 2 or TP != 0 ask)
+    d.DH = DH; d.DV = DV; d.zMax = zMax;
+    d.TU = TU; d.DZTU = DZTU; d.DZTB = DZTB;   // left by Static() under lRem
+    d.XP = XP; d.YP = YP; d.ZP = ZP; d.QFH = QFH; d.HH = HH;
+    d.ORZD = 1.0f / double(nPz - 1);   // the plume's DZ (3dmhdsub.h:768)
+    d.EXX = EXX; d.WYY = WYY; d.ZEE = ZEE;
+    if (mhd3d_create(&d, &g_gpu) != MHD3D_OK) exit_err(mhd3d_last_error(nullptr));
+    double* f[8];
+    gpu_fields(f);
+    if (mhd3d_set_state(g_gpu, f) != MHD3D_OK) exit_err(mhd3d_last_error(g_gpu));
+}
+
+// replaces step() in the main loop: DT, TIMT, TIMC and nIt come back into
+// the reference's globals, so the loop's printout and DT guard work as before
+static void gpu_step()
+{
+    int rc = mhd3d_step(g_gpu, 1, 0, &DT);
+    if (rc != MHD3D_OK && rc != MHD3D_ENONFINITE) exit_err(mhd3d_last_error(g_gpu));
+    int64_t it;
+    mhd3d_clock(g_gpu, &it, &TIMT, &TIMC);
+    nIt = (int)it;
+}
+
+// before dump(): bring the arrays back
+static void gpu_sync_host()
+{
+    double* f[8];
+    gpu_fields(f);
+    if (mhd3d_get_state(g_gpu, f) != MHD3D_OK) exit_err(mhd3d_last_error(g_gpu));
+}
+
+#endif
