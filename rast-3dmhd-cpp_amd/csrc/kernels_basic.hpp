@@ -183,11 +183,23 @@ struct StageArgs {
 // Forward: whether the DT guard has stopped the domain (defined below).
 __device__ __forceinline__ bool domain_halted(const StageArgs& a);
 
+// A value every thread of a workgroup reads at the same address (a plane's
+// metric term or background value, the plane's TSTEP record), read through
+// the constant address space: the compiler then issues a scalar load, which
+// waits on its own counter instead of the in-order vector-memory counter
+// the stencil loads of the next plane are counted on.  Only for memory no
+// kernel writes while it runs.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p)
+{
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // Background subtracted from field f of plane z when the FP32 copy is written.
 __device__ __forceinline__ double bg_sub(const StageArgs& a, int f, int z)
 {
     const int k = bg64_slot(f);
-    return k < 0 ? 0.0 : a.bg64[k * a.g.nz + z];
+    return k < 0 ? 0.0 : uniform_load(a.bg64 + k * a.g.nz + z);
 }
 
 __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int yg, int zg)

@@ -27,6 +27,7 @@
 #pragma clang fp contract(off)
 #endif
 
+#include <cstddef>
 #include <type_traits>
 
 #include "kernels_basic.hpp"
@@ -92,9 +93,9 @@ __device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const
     q.rw = T(raw.rw);
     if constexpr (!std::is_same<T, double>::value && std::is_same<TI, double>::value && MAG) {
         const int nz = a.g.nz;
-        q.bx = T(raw.bx - a.bg64[(BG64_B0 + 0) * nz + z]);
-        q.by = T(raw.by - a.bg64[(BG64_B0 + 1) * nz + z]);
-        q.bz = T(raw.bz - a.bg64[(BG64_B0 + 2) * nz + z]);
+        q.bx = T(raw.bx - uniform_load(a.bg64 + (BG64_B0 + 0) * nz + z));
+        q.by = T(raw.by - uniform_load(a.bg64 + (BG64_B0 + 1) * nz + z));
+        q.bz = T(raw.bz - uniform_load(a.bg64 + (BG64_B0 + 2) * nz + z));
     } else {
         q.bx = T(raw.bx);
         q.by = T(raw.by);
@@ -107,11 +108,11 @@ __device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const
         q.rdev = T(0);
     } else {
         const int nz = a.g.nz;
-        const T R0 = a.bgf[BG_R0 * nz + z], T0 = a.bgf[BG_T0 * nz + z];
+        const T R0 = uniform_load(a.bgf + BG_R0 * nz + z), T0 = uniform_load(a.bgf + BG_T0 * nz + z);
         T r, t;
         if constexpr (std::is_same<TI, double>::value) {
-            r = T(raw.ro - a.bg64[BG64_R0 * nz + z]);
-            t = T(raw.tt - a.bg64[BG64_T0 * nz + z]);
+            r = T(raw.ro - uniform_load(a.bg64 + BG64_R0 * nz + z));
+            t = T(raw.tt - uniform_load(a.bg64 + BG64_T0 * nz + z));
         } else {
             r = raw.ro;
             t = raw.tt;
@@ -273,9 +274,13 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     T row[3] = {T(0), T(0), T(0)};
     T rinv[3] = {T(0), T(0), T(0)};
     T rdv[3] = {T(0), T(0), T(0)};
-    RawPt<TI, MAG> own, halo, own_n, halo_n;
-    load_raw<TI, MAG>(own, U, fs, (int64_t)(zb - 1) * g.plane + go);
-    if (has_halo) load_raw<TI, MAG>(halo, U, fs, (int64_t)(zb - 1) * g.plane + gh);
+    // raw values of two planes, used in turn: the loop body for plane L
+    // consumes one pair and loads plane L+1 into the other, so no value a
+    // load is still writing is ever copied (a copy would wait for every
+    // memory operation issued before it)
+    RawPt<TI, MAG> rawA, haloA, rawB, haloB;
+    load_raw<TI, MAG>(rawA, U, fs, (int64_t)(zb - 1) * g.plane + go);
+    if (has_halo) load_raw<TI, MAG>(haloA, U, fs, (int64_t)(zb - 1) * g.plane + gh);
 
     const double DT = *a.dt;
     const double cg = a.c.GAM[STAGE - 1] * DT;
@@ -289,20 +294,13 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)zb * g.plane + go];
     }
 
-    // HOIST_METRIC: the column's x and y metric terms (the same on every
-    // plane) are loaded once and the z terms of each plane at the top of its
-    // iteration, instead of all eight right before the flux evaluation, where
-    // their latency shows.  Measured faster in FP64 stage 3 and in every stage
-    // of the mixed form (its FP32 registers leave room), slower in FP64
-    // stages 1-2 (register budget).
-    constexpr bool HOIST_METRIC = STAGE == 3 || !F64;
-    double mdxx = 0.0, md2xx = 0.0, mdyy = 0.0, md2yy = 0.0;
-    if (HOIST_METRIC) {
-        mdxx = a.m.dxxdx[gx + 1];
-        md2xx = a.m.d2xxdx2[gx + 1];
-        mdyy = a.m.dyydy[gy + 1];
-        md2yy = a.m.d2yydy2[gy + 1];
-    }
+    // The column's x and y metric terms (the same on every plane) are loaded
+    // once; the z terms of each plane, the same for the whole workgroup, by
+    // scalar loads at the top of its iteration.
+    const double mdxx = a.m.dxxdx[gx + 1];
+    const double md2xx = a.m.d2xxdx2[gx + 1];
+    const double mdyy = a.m.dyydy[gy + 1];
+    const double md2yy = a.m.d2yydy2[gy + 1];
     // stage 3: the column's part of the TSTEP grid spacing (DDZ comes with
     // each plane's record)
     double ddxy = 0.0;
@@ -328,7 +326,15 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         }
         if (STAGE == 3 && a.cfl_parts) {
             double wp[5], up;
-            const CflRec& r = a.m.cflz[zq];
+            CflRec r;
+            {
+                // the plane's record, field by field through scalar loads
+                double* dst = reinterpret_cast<double*>(&r);
+                const double* src = reinterpret_cast<const double*>(a.m.cflz + zq);
+#pragma unroll
+                for (int k = 0; k < (int)(offsetof(CflRec, pad) / sizeof(double)); ++k)
+                    dst[k] = uniform_load(src + k);
+            }
             if constexpr (MAG)
                 cfl_values_rec(r, true, un[0], un[1], un[2], un[3], un[4], un[5], un[6], un[7], ddxy,
                                wp, up);
@@ -349,16 +355,16 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     int zp = -1;
 #pragma unroll
     for (int f = 0; f < NF; ++f) zpend[f] = 0.0;
-    for (int L = zb - 1; L <= ze; ++L) {
-        // HOIST_METRIC: z metric terms of plane L-1, whose fluxes this
-        // iteration evaluates (when L-1 >= zb); L-1 >= zb-2 >= 0 is in range
-        double mdzz = 0.0, md2zz = 0.0, mkap = 0.0, mdkap = 0.0;
-        if (HOIST_METRIC) {
-            mdzz = a.m.dzzdz[L - 1];
-            md2zz = a.m.d2zzdz2[L - 1];
-            mkap = a.m.RKAPA[L - 1];
-            mdkap = a.m.DKAPA[L - 1];
-        }
+    // one plane L: derive and publish it from (own, halo), fetch plane L+1
+    // into (own_n, halo_n), evaluate and update plane L-1
+    auto plane = [&](int L, const RawPt<TI, MAG>& own, const RawPt<TI, MAG>& halo, RawPt<TI, MAG>& own_n,
+                     RawPt<TI, MAG>& halo_n) __attribute__((always_inline)) {
+        // z metric terms of plane L-1, whose fluxes this iteration evaluates
+        // (when L-1 >= zb); L-1 >= zb-2 >= 0 is in range
+        const double mdzz = uniform_load(a.m.dzzdz + (L - 1));
+        const double md2zz = uniform_load(a.m.d2zzdz2 + (L - 1));
+        const double mkap = uniform_load(a.m.RKAPA + (L - 1));
+        const double mdkap = uniform_load(a.m.DKAPA + (L - 1));
         // -- derive plane L and publish it
         {
             const PrepPt<T> q = prep_point<T, MAG, TI>(own, a, L);
@@ -433,10 +439,12 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         // faster in FP64; in FP32 the 2-wave register budget prefers the load
         // after the flux)
         constexpr bool ZEARLY = DEFER && ZLOAD && F64;
-        double zearly[NF];
         if (ZEARLY && z >= zb) {
+            // straight into the pending operand (the update above consumed the
+            // previous one): a copy of a value still in flight would wait for
+            // every memory operation issued before its load
 #pragma unroll
-            for (int f = 0; f < NF; ++f) zearly[f] = zsrc[f * fs + (int64_t)z * g.plane + go];
+            for (int f = 0; f < NF; ++f) zpend[f] = zsrc[f * fs + (int64_t)z * g.plane + go];
         }
         if (!DEFER && ZLOAD && z >= zb) {
             // this iteration updates plane z; fetch the operand of plane z+1
@@ -462,32 +470,28 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             if constexpr (!F64) {
                 const int nz = g.nz;
                 acc.rdev_ = rdv[1];
-                acc.hb_ = a.bgf[BG_HB * nz + z];
-                acc.dzt0_ = a.bgf[BG_DZT0 * nz + z];
-                acc.d2zt0_ = a.bgf[BG_D2ZT0 * nz + z];
-                acc.t0_ = a.bgf[BG_T0 * nz + z];
+                acc.hb_ = uniform_load(a.bgf + BG_HB * nz + z);
+                acc.dzt0_ = uniform_load(a.bgf + BG_DZT0 * nz + z);
+                acc.d2zt0_ = uniform_load(a.bgf + BG_D2ZT0 * nz + z);
+                acc.t0_ = uniform_load(a.bgf + BG_T0 * nz + z);
                 if (MAG) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        acc.b0_[k] = a.bgf[(BG_B0 + k) * nz + z];
-                        acc.dzb_[k] = a.bgf[(BG_DZB + k) * nz + z];
-                        acc.d2zb_[k] = a.bgf[(BG_D2ZB + k) * nz + z];
+                        acc.b0_[k] = uniform_load(a.bgf + (BG_B0 + k) * nz + z);
+                        acc.dzb_[k] = uniform_load(a.bgf + (BG_DZB + k) * nz + z);
+                        acc.d2zb_[k] = uniform_load(a.bgf + (BG_D2ZB + k) * nz + z);
                     }
                 }
             }
             PointMetric pm;
-            if constexpr (HOIST_METRIC) {
-                pm.dxx = mdxx;
-                pm.d2xx = md2xx;
-                pm.dyy = mdyy;
-                pm.d2yy = md2yy;
-                pm.dzz = mdzz;
-                pm.d2zz = md2zz;
-                pm.kap = mkap;
-                pm.dkap = mdkap;
-            } else {
-                pm = point_metric(a.m, gx + 1, gy + 1, z);
-            }
+            pm.dxx = mdxx;
+            pm.d2xx = md2xx;
+            pm.dyy = mdyy;
+            pm.d2yy = md2yy;
+            pm.dzz = mdzz;
+            pm.d2zz = md2zz;
+            pm.kap = mkap;
+            pm.dkap = mdkap;
             FluxOutT<T> o;
             ExtPt e;
             if constexpr (EXT) ext_point<EXTM>(a, gx, gy, z, (int64_t)z * g.plane + go, e);
@@ -512,16 +516,26 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             if constexpr (DEFER) {
                 fpend = o;
                 zp = z;
+                if constexpr (!ZEARLY) {
 #pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    zpend[f] = ZEARLY ? zearly[f] : (ZLOAD ? zsrc[f * fs + (int64_t)z * g.plane + go] : uc[f]);
+                    for (int f = 0; f < NF; ++f)
+                        zpend[f] = ZLOAD ? zsrc[f * fs + (int64_t)z * g.plane + go] : uc[f];
+                }
             } else {
                 update(z, o, ZLOAD ? zcur : uc);
             }
         }
         if (!ONE_BARRIER) lds_barrier();
-        own = own_n;
-        if (has_halo) halo = halo_n;
+    };
+    // Everything loaded before the loop (the first plane, the column's metric
+    // terms, DT) is waited for here, once.  Otherwise the wait-count pass
+    // merges those loads into the loop header's state and makes iterations
+    // wait for them with counts that also drain the next plane's loads.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int L = zb - 1; L <= ze; L += 2) {
+        plane(L, rawA, haloA, rawB, haloB);
+        if (L + 1 > ze) break;
+        plane(L + 1, rawB, haloB, rawA, haloA);
     }
     if (DEFER && zp >= 0) update(zp, fpend, zpend);
 }
