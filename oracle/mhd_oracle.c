@@ -479,7 +479,7 @@ void orc_fluxes(orc_ctx* c)
         f_d1_mul(c, SUB, DY, R, FT, TT, VV, 1);
         f_d1_mul(c, SUB, DZ, R, FT, TT, WW, 1);
     }
-    if (!shr && d->flag_lRem) {                                             /* :297-302 */
+    if ((!shr || MUT(22)) && d->flag_lRem) {                                /* :297-302 */
         hmean(c, c->TTM, TT);
         LOOP(RE, WW1[i] = TT[i] - c->TTM[z]);
     }

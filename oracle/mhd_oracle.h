@@ -21,7 +21,11 @@
  * Domain = one z slab holding both walls: the one-sided wall stencils of both
  * walls apply (the reference's nPEz>=2 behaviour, 3dmhdsub.h:211-226,572-584).
  *
- * Pinning (which check fixes which function) is listed in oracle/README.md.
+ * What pins each function: the published iteration-9 answers
+ * (tests/test_oracle_known_answers.py), closed forms on uniform and
+ * stretched grids (tests/test_oracle_math.py, test_oracle_stretched.py,
+ * test_oracle_ext.py), and planted mistakes those checks must catch
+ * (tests/test_oracle_mutants.py, the ORC_MUTANT switches of mhd_oracle.c).
  */
 #ifndef MHD_ORACLE_H_
 #define MHD_ORACLE_H_

@@ -238,14 +238,25 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
     // ---- conduction :315-319 (func_d2_mul_array_z: ((d2*g)*a[z])*s)
     tmp = T(c.OCV_REPR);
     if (EXT && (EXTM & EXT_REM) && (e->flags & EXT_REM)) {
-        // lRem :304-313: func_d2_mul of WW1 = TT - TTM(z) ((d2 * RO) * s, RO
-        // holding 1/RO), then func_mul_extend of HRAD: (RO * HRAD[z]) * s
-        const T m0 = e->ttm[1];
-        const T t0 = T0 - m0;
-        FT += (op.d2x(a.tt(1, 0, 0) - m0, t0, a.tt(-1, 0, 0) - m0) * RI) * tmp;
-        FT += (op.d2y(a.tt(0, 1, 0) - m0, t0, a.tt(0, -1, 0) - m0) * RI) * tmp;
-        FT += (op.d2z(a.tt(0, 0, 1) - e->ttm[2], t0, a.tt(0, 0, -1) - e->ttm[0]) * RI) * tmp;
-        FT += (RI * e->hrad) * tmp;
+        if ((EXTM & EXT_SHR) && (e->flags & EXT_SHR)) {
+            // lRem with lShr: the TTM update and WW1 = TT - TTM sit inside
+            // `#if !lShr` (:297-302), so WW1 still holds P = RO*TT (:267)
+            // and the static TTM stays zero, which makes HRAD zero
+            const T p0 = a.p(0, 0, 0);
+            FT += (op.d2x(a.p(1, 0, 0), p0, a.p(-1, 0, 0)) * RI) * tmp;
+            FT += (op.d2y(a.p(0, 1, 0), p0, a.p(0, -1, 0)) * RI) * tmp;
+            FT += (op.d2z(a.p(0, 0, 1), p0, a.p(0, 0, -1)) * RI) * tmp;
+            FT += (RI * T(0)) * tmp;
+        } else {
+            // lRem :304-313: func_d2_mul of WW1 = TT - TTM(z) ((d2 * RO) * s, RO
+            // holding 1/RO), then func_mul_extend of HRAD: (RO * HRAD[z]) * s
+            const T m0 = e->ttm[1];
+            const T t0 = T0 - m0;
+            FT += (op.d2x(a.tt(1, 0, 0) - m0, t0, a.tt(-1, 0, 0) - m0) * RI) * tmp;
+            FT += (op.d2y(a.tt(0, 1, 0) - m0, t0, a.tt(0, -1, 0) - m0) * RI) * tmp;
+            FT += (op.d2z(a.tt(0, 0, 1) - e->ttm[2], t0, a.tt(0, 0, -1) - e->ttm[0]) * RI) * tmp;
+            FT += (RI * e->hrad) * tmp;
+        }
     } else {
         const T kap = T(m.kap);
         FT += ((op.d2x(a.tt(1, 0, 0), T0, a.tt(-1, 0, 0)) * RI) * kap) * tmp;

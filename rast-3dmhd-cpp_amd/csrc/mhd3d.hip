@@ -132,10 +132,6 @@ static int validate(const mhd3d_desc* d, std::string& why)
               "or rejects the wall combination (:344,416)";
         return MHD3D_EUNSUPPORTED;
     }
-    if (d->flag_lShr && d->flag_lRem) {
-        why = "lShr with lRem: the reference's lShr skips the TTM update lRem's conduction reads (3dmhdsub.h:293-303)";
-        return MHD3D_EUNSUPPORTED;
-    }
     if (d->flag_iD < 0 || d->flag_iD > 2) { why = "iD must be 0, 1 or 2 (3dmhdsub.h:243-252)"; return MHD3D_EINVAL; }
     if (d->flag_iTCon < 0 || d->flag_iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
     if (d->flag_izCon != 0 && d->flag_izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
@@ -1465,8 +1461,11 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
     }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(MHD3D_EHIP, "hipStreamCreate failed");
-    e->waves = e->mixed ? 2 : 1;
-    e->sched = e->mixed ? 2 : 0;
+    // default variant: two waves per SIMD with the deferred update, the
+    // fastest measured for the FP64 and the mixed ensemble kernels on MI355X
+    // (the autotuner may pick another)
+    e->waves = 2;
+    e->sched = 2;
     const char* tv = std::getenv(e->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
     int w = 0, sc = 0, ch = 0;
     if (tv && std::sscanf(tv, "%d,%d,%d", &w, &sc, &ch) >= 2 && (w == 1 || w == 2) && sc >= 0 && sc <= 3) {
@@ -1903,8 +1902,8 @@ extern "C" int mhd3d_server_create(int precision, int64_t hbm_budget_bytes, int 
     s->ens = new mhd3d_ensemble();
     s->ens->owns = false;
     s->ens->mixed = precision == MHD3D_MIXED;
-    s->ens->waves = s->ens->mixed ? 2 : 1;
-    s->ens->sched = s->ens->mixed ? 2 : 0;
+    s->ens->waves = 2;
+    s->ens->sched = 2;
     // the ensemble variant override (MHD3D_TUNE_ENS_<FP64|MIXED>=waves,sched[,chunk])
     // applies to the server's ensemble as well
     const char* tv = std::getenv(s->ens->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");

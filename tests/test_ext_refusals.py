@@ -3,9 +3,8 @@ the descriptor before its first HIP call, so a refused descriptor returns
 without a device).
 
 Refused: RLAX != 0 (the reference's relaxation exits, 3dmhdsub.h:427-439, or
-rejects the wall combination, :344,416), lShr with lRem (lShr skips the TTM
-update lRem reads, :293-303), the branches in the mixed form, iD outside
-0..2 and missing coordinate arrays.
+rejects the wall combination, :344,416), the branches in the mixed form, iD
+outside 0..2 and missing coordinate arrays.
 """
 import ctypes as C
 import dataclasses
@@ -41,11 +40,6 @@ def test_rlax_refused():
     par[59] = 0.1
     rc, msg = _create(dataclasses.replace(cfg, par=par))
     assert rc == L.EUNSUPPORTED and "RLAX" in msg
-
-
-def test_lshr_with_lrem_refused():
-    rc, msg = _create(_base(lShr=True, lRem=True))
-    assert rc == L.EUNSUPPORTED and "lShr" in msg
 
 
 def test_mixed_form_refuses_branches():

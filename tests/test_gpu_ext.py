@@ -51,6 +51,9 @@ def _cfg(n, case):
                   kappa=_kappa(n[2] + 2))
     elif case.startswith("lRem0"):
         kw = dict(lRem=True, iTCon=0, izCon=0, TU=1.05, kappa=_kappa(n[2] + 2))
+    elif case.startswith("lShrRem"):
+        kw = dict(lShr=True, lRem=True, iTCon=1, izCon=1, TU=1.0, DZTU=-0.02, DZTB=0.03,
+                  kappa=_kappa(n[2] + 2))
     elif case.startswith("lShr"):
         kw = dict(lShr=True)
     elif case.startswith("heat"):
@@ -69,8 +72,8 @@ def _cfg(n, case):
     return dataclasses.replace(cfg, par=par, **kw)
 
 
-CASES = ["iD1", "iD2_mhd", "iDh_mhd", "lRem1", "lRem0", "lShr", "lShr_mhd", "heatT", "heat0_mhd",
-         "plume0", "plume1"]
+CASES = ["iD1", "iD2_mhd", "iDh_mhd", "lRem1", "lRem0", "lShr", "lShr_mhd", "lShrRem", "lShrRem_mhd",
+         "heatT", "heat0_mhd", "plume0", "plume1"]
 
 
 def _pair(n, case, seed=7):
@@ -134,7 +137,7 @@ def test_point_kernel_override(case, monkeypatch):
     _same(dom, orc, case)
 
 
-@pytest.mark.parametrize("case", ["iD2_mhd", "lRem1", "lShr_mhd", "heatT", "plume0"])
+@pytest.mark.parametrize("case", ["iD2_mhd", "lRem1", "lShr_mhd", "lShrRem", "heatT", "plume0"])
 def test_stages_and_fluxes(case):
     print(f"ext stages {case}", flush=True)
     desc, dom, orc = _pair((20, 16, 10), case, seed=3)

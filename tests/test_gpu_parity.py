@@ -99,12 +99,6 @@ def test_unsupported_flags_fail_loudly():
     from mhd3d.domain import Domain
     from mhd3d.initial import stratified_config
     cfg = stratified_config(16, 16, 8, lMag=True)
-    cfg.lShr = True
-    cfg.lRem = True          # lShr skips the TTM update lRem reads
-    with pytest.raises(L.MhdError) as ei:
-        Domain(Descriptor(cfg))
-    assert ei.value.code == L.EUNSUPPORTED
-    cfg = stratified_config(16, 16, 8, lMag=True)
     cfg.iBCon = 1            # only the reference's iBCon 0 wall field is built
     with pytest.raises(L.MhdError) as ei:
         Domain(Descriptor(cfg))

@@ -229,6 +229,28 @@ def test_lshr_carries_ft_and_keeps_only_conduction():
     assert np.abs(Fp["FT"] - cond)[1:-1, 1:-1, 1:-1].max() > 1e-4
 
 
+def test_lshr_with_lrem_conducts_the_pressure():
+    """lShr with lRem (3dmhdsub.h:293-313): the TTM update and WW1 = TT - TTM
+    sit inside `#if !lShr`, so lRem's conduction differentiates what WW1
+    still holds, P = RO*TT (:267), and HRAD comes from the static TTM, zero.
+    At rest with RO = rho and TT quadratic in z, P'' = rho*TT'' and FT =
+    OCV/REPR * P''/rho = OCV/REPR * TT'' -- rho times the lShr-only value,
+    with no RKAPA factor -- and it is carried over to the next call."""
+    nz = NPZ + 2
+    kap = (1.0 + 0.5 * np.linspace(0, 1, nz), 0.3 * np.ones(nz))
+    desc = _desc(lShr=True, lRem=True, iTCon=1, izCon=1, TU=1.0, kappa=kap)
+    c = desc.consts
+    X, Y, Z = _coords(desc)
+    rho = 1.4
+    TT = 2.0 + 0.3 * Z + 0.25 * Z ** 2
+    st = _rest_state(desc, np.full(desc.shape, rho), TT)
+    F1, _ = _fluxes(desc, st, names=("FT",), calls=1)
+    F2, _ = _fluxes(desc, st, names=("FT",), calls=2)
+    cond = (c["OCV"] / c["REPR"]) * (0.5 * np.ones(desc.shape))
+    _close(F1["FT"], cond, 1e-10)
+    _close(F2["FT"], 2 * cond, 1e-10)
+
+
 def test_lshr_walls_move():
     desc = _desc(lShr=True)
     o = om.Oracle(desc)

@@ -6,7 +6,7 @@ derivative taken along the wrong axis, a wrong wall-stencil plane, a wrong
 BCON coefficient, a dropped factor in the CFL, a dropped or misplaced metric
 term of the stretched z grid, the wrong Reynolds number in the magnetic CFL
 limit.  Every one must make at least one check of test_oracle_math.py
-(mutants 1-8), test_oracle_ext.py (the off-default branches, 9-15) or
+(mutants 1-8), test_oracle_ext.py (the off-default branches, 9-15 and 22) or
 test_oracle_stretched.py (the z metric and the magnetic CFL limit, 16-21)
 fail, otherwise those checks would not pin what they claim to pin.  Runs on
 the CPU only.
@@ -40,6 +40,7 @@ MUTANTS = {
     19: "D1z taking the metric of the next plane",
     20: "D2z with dzzdz not squared",
     21: "top-wall one-sided D1z(RW) with the metric of plane 2",
+    22: "lShr with lRem updating TTM as without lShr",
 }
 
 
@@ -61,6 +62,7 @@ def _checks():
     yield lambda: te.test_lrem_walls_keep_a_linear_profile(4)
     yield te.test_lshr_carries_ft_and_keeps_only_conduction
     yield te.test_lshr_walls_move
+    yield te.test_lshr_with_lrem_conducts_the_pressure
     yield te.test_heat_loss_gaussian_and_amplitude
     yield te.test_heat_loss_without_time_factor
     yield lambda: te.test_plume_wall(0)
