@@ -41,7 +41,10 @@ enum {
     MHD3D_ENOMEM = 4,       /* device allocation failed                         */
     MHD3D_ENONFINITE = 5,   /* state holds NaN/Inf after the step (the DT guard
                                of 3dmhd.cpp:292 cannot see it: minVal ignores NaN) */
-    MHD3D_EHALTED = 6       /* the DT guard stopped the run (mhd3d_set_dt_guard) */
+    MHD3D_EHALTED = 6,      /* the DT guard stopped the run (mhd3d_set_dt_guard) */
+    MHD3D_EEXIT = 7         /* the reference's program exits at this point of the
+                               run (exit_err in the RLAX relaxation, 3dmhdsub.h:344,
+                               416,427-439); the context stops (see mhd3d_step)   */
 };
 
 /* DT guard threshold of the reference's driver: DT < 1.0e-8f (3dmhd.cpp:292). */
@@ -74,7 +77,11 @@ typedef struct mhd3d_desc {
     int32_t flag_ixCon, flag_iyCon, flag_izCon, flag_iTCon, flag_iBCon;
     int32_t flag_iD;
     int32_t precision;                           /* MHD3D_FP64 / MHD3D_MIXED  */
-    int32_t reserved;
+    /* RLAX > 0 only: the z ranks nPEz of the reference run (3dmhdparam.h:21)
+     * whose relaxation arithmetic is reproduced -- the scan of CORRECT runs
+     * per rank and meets at the rank boundaries (3dmhdsub.h:354-421); >= 2,
+     * dividing nPz.  Ignored when RLAX == 0. */
+    int32_t relax_nPEz;
     /* GRID / RUNGKU / SFF (3dmhd.cpp:354-369, 3dmhdparam.h:111) */
     double C13, C23, C43;
     double GAM1, GAM2, GAM3, ZETA1, ZETA2;
@@ -86,9 +93,12 @@ typedef struct mhd3d_desc {
     double OMX, OMZ;
     double RM, ORM, OBETA;
     double TP, TB, TC;
-    double RLAX;        /* must be 0: the relaxation of 3dmhdsub.h:322-442 is refused
-                           (the reference itself exits in its first fluxes() call
-                           of a run that starts at a TIMT multiple of 25, :427-439) */
+    /* heat-flux relaxation (3dmhdsub.h:322-442): RLAX = par[59], TSTART =
+     * par[57], TOFF = par[58] (3dmhd.cpp:428-430); FP64 form, single domains,
+     * with ZEE.  The reference exits in the first fluxes() call of a step
+     * that starts within DT after a multiple of 25 in TIMT (:427-439), so a
+     * run can only use it from a restart; MHD3D_EEXIT reports that point. */
+    double RLAX, TSTART, TOFF;
     double TIMI;        /* initial simulation time (3dmhd.cpp:114)                   */
     /* AJACOBI (3dmhddata.h:20-22): lengths nPx+2, nPy+2, nPz+2 */
     const double *dxxdx, *d2xxdx2, *DDX;
@@ -194,6 +204,17 @@ int mhd3d_cfl(mhd3d_ctx* ctx, mhd3d_cfl_info* out);
  * dV = DDX[x]*DDY[y]*DDZ[z] of the metric (3dmhddata.h:20-22; DDX = 1 when
  * nPx == 1). */
 int mhd3d_diagnostics(mhd3d_ctx* ctx, double out[3]);
+
+/* State of the RLAX relaxation, the function statics of fluxes()
+ * (3dmhdsub.h:195-198): out[0] = TFAC, out[1] = TTM_MAX, out[2] = TTM_MIN,
+ * *icall = ICALL (fluxes() calls so far), *exited = 1 after the reference's
+ * exit point (then every step call returns MHD3D_EEXIT and the state is that
+ * at the start of the step in which the exit fell, or, for an exit in a
+ * later stage, that stage's input), else 0.  A context starts as a fresh
+ * process does: TFAC = 1, TTM_MAX = -1e99, TTM_MIN = 1e99, ICALL = 0; every
+ * fluxes() evaluation (each RK stage, mhd3d_fluxes) advances it as the
+ * reference's call does. */
+int mhd3d_relax_state(mhd3d_ctx* ctx, double out[3], int64_t* icall, int* exited);
 
 /* Simulation clock: nIt, TIMT, TIMC (3dmhdsub.h:150-152). */
 int mhd3d_clock(mhd3d_ctx* ctx, int64_t* nIt, double* timt, double* timc);

@@ -67,6 +67,14 @@ struct orc_ctx {
     double *EXX, *WYY, *ZEE;        /* coordinates (off-default branches)   */
     /* function-local statics of fluxes()/horizontal_mean (3dmhdsub.h:200,699) */
     double *ROM, *TTM, *HRAD, *WWZ;
+    /* RLAX relaxation (3dmhdsub.h:195-200): the statics ICALL, TTM_MIN,
+     * TTM_MAX, TFAC, the profiles FCONM, FRADM, CORRECT over the whole domain,
+     * and the exit point the reference's program stops at (1: "yk not
+     * implement", :427-439; 2: "Relax: check boundary temperature", :344,416) */
+    double *FCONM, *FRADM, *CORRECT;
+    double TFAC, TTM_MAX, TTM_MIN;
+    long long ICALL;
+    int relax_exit;
     /* CTIM / ITER */
     double DT, TIMT, TIMC, TIMI, UMACH;
     long long nIt;
@@ -262,6 +270,12 @@ orc_ctx* orc_create(const mhd3d_desc* d)
     c->TTM = dup_arr(NULL, c->nz);
     c->HRAD = dup_arr(NULL, c->nz);
     c->WWZ = dup_arr(NULL, c->nz);
+    c->FCONM = dup_arr(NULL, c->nz);
+    c->FRADM = dup_arr(NULL, c->nz);
+    c->CORRECT = dup_arr(NULL, c->nz);
+    c->TFAC = 1.0;                              /* 3dmhdsub.h:196-198 */
+    c->TTM_MAX = -1e99;
+    c->TTM_MIN = 1e99;
     c->d.EXX = c->d.WYY = c->d.ZEE = NULL;   /* the copies above are used */
     c->TIMI = d->TIMI;
     c->TIMT = d->TIMI;
@@ -277,7 +291,8 @@ void orc_destroy(orc_ctx* c)
                      c->WW2, c->WW3, c->BX, c->BY, c->BZ, c->ZBX, c->ZBY, c->ZBZ, c->RI,
                      c->S1, c->S2, c->S3, c->S4, c->dxxdx, c->d2xxdx2, c->DDX, c->dyydy,
                      c->d2yydy2, c->DDY, c->dzzdz, c->d2zzdz2, c->DDZ, c->RKAPA, c->DKAPA,
-                     c->EXX, c->WYY, c->ZEE, c->ROM, c->TTM, c->HRAD, c->WWZ};
+                     c->EXX, c->WYY, c->ZEE, c->ROM, c->TTM, c->HRAD, c->WWZ,
+                     c->FCONM, c->FRADM, c->CORRECT};
     for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); ++k) free(all[k]);
     free(c);
 }
@@ -318,6 +333,31 @@ int orc_get(orc_ctx* c, const char* name, double* a)
     double* p = lookup(c, name);
     if (!p || !a) return -1;
     memcpy(a, p, c->n * sizeof(double));
+    return 0;
+}
+
+void orc_set_time(orc_ctx* c, double dt, double timt)
+{
+    c->DT = dt;
+    c->TIMT = timt;
+    c->TIMC = timt - c->TIMI;
+}
+
+int orc_relax_state(orc_ctx* c, double out[3], long long* icall, int* exited, double* correct,
+                    double* fconm, double* fradm)
+{
+    if (!c) return -1;
+    if (out) {
+        out[0] = c->TFAC;
+        out[1] = c->TTM_MAX;
+        out[2] = c->TTM_MIN;
+    }
+    if (icall) *icall = c->ICALL;
+    if (exited) *exited = c->relax_exit;
+    const size_t b = (size_t)c->nz * sizeof(double);
+    if (correct) memcpy(correct, c->CORRECT, b);
+    if (fconm) memcpy(fconm, c->FCONM, b);
+    if (fradm) memcpy(fradm, c->FRADM, b);
     return 0;
 }
 
@@ -384,6 +424,113 @@ static double sq(double v) { return v * v; }   /* square() of 3dmhdutils.h:213 *
  * (3dmhdphsic.h:11), whose C++ wrapper puts the float overload of log() in
  * the global namespace, so log(2.0f) there is the float logarithm. */
 static double ln2f(void) { return (double)logf(2.0f); }
+
+/* ------------------------------------------------------------------------ */
+/* RLAX heat-flux relaxation of fluxes() — 3dmhdsub.h:322-442.               */
+/* The reference runs it on nPEz z ranks (relax_nPEz here) of nRz planes     */
+/* each; rank r's local plane l (0..nRz+1) is global plane r*nRz + l.  Each   */
+/* rank scans CORRECT over its planes 1..nRz+1 (its upper ghost included),    */
+/* with FCONM = 0 on that ghost (WW2 is never written on z ghost planes, so   */
+/* the static array's zero stays there) and FRADM = the next rank's first     */
+/* plane (received, :366-376) or, on the last rank, 0 (never received); the   */
+/* ranks' end values are then added up (:383-421).  WW1 holds T' = TT - TTM   */
+/* (:297-301), or under lShr the pressure RO*TT of :267 with TTM never        */
+/* computed (zero); UU, VV, WW are the velocities of :274-277.                */
+
+static void relax(orc_ctx* c)
+{
+    const mhd3d_desc* d = &c->d;
+    const int zl = 1, zr = c->nz - 1;
+    const int combo = (d->flag_iTCon == 0 && d->flag_izCon == 1) ? 1
+                    : (d->flag_iTCon == 1 && d->flag_izCon == 0) ? 2 : 0;
+    const double* TTM = c->TTM;
+    double *FCONM = c->FCONM, *FRADM = c->FRADM, *CORRECT = c->CORRECT;
+    const rng_t R = R_in(c);
+    if ((c->TIMT > d->TSTART) && (c->TFAC > (double)1e-4f)) {               /* :323 */
+        const double step = c->DT / ((MUT(27) ? 1.0 : 3.0) * d->TOFF);
+        if (combo == 1) {                                                   /* :324-334 */
+            if (TTM[zr - 1] <= c->TTM_MAX) {
+                c->TFAC -= step;
+            } else {
+                c->TTM_MAX = TTM[zr - 1];
+                c->TFAC += step;
+                if (2.0 < c->TFAC) c->TFAC = 2.0;                           /* min(TFAC, 2.0f) */
+            }
+        } else if (combo == 2) {                                            /* :336-346 */
+            if (TTM[zl] >= c->TTM_MIN) {
+                c->TFAC -= step;
+            } else {
+                c->TTM_MIN = TTM[zl];
+                c->TFAC += step;
+                if (2.0 < c->TFAC) c->TFAC = 2.0;
+            }
+        } else {
+            c->relax_exit = 2;                                              /* :347-348 */
+            return;
+        }
+        f_mul2(c, SET, R, c->WW2, c->RW, MUT(23) ? c->TT : c->WW1, -2.5);   /* :349-353 */
+        f_mul3(c, ADD, R, c->WW2, c->RW, c->UU, c->UU, -0.5);
+        f_mul3(c, ADD, R, c->WW2, c->RW, c->VV, c->VV, -0.5);
+        f_mul3(c, ADD, R, c->WW2, c->RW, c->WW, c->WW, -0.5);
+        hmean(c, FCONM, c->WW2);
+        const double hz = d->hz, REPR = d->REPR;
+        const double *dzz = c->dzzdz, *RK = c->RKAPA;
+        for (int z = zl; z < zr; z++)                                       /* :356-358 */
+            FRADM[z] = (TTM[z + 1] - TTM[z - 1]) * hz * dzz[z] * RK[z] / REPR;
+        FRADM[zl] = (-3.0 * TTM[zl] + 4.0 * TTM[zl + 1] - TTM[zl + 2]) * hz * dzz[zl] * RK[zl] / REPR;
+        FRADM[zr - 1] = (3.0 * TTM[zr - 1] - 4.0 * TTM[zr - 2] + TTM[zr - 3]) * hz * dzz[zr - 1] *
+                        RK[zr - 1] / REPR;
+        double FTOT;                                                        /* :377-382 */
+        if (d->flag_lRem) {
+            const float k = 8.07f * 0.4f;   /* a float product, then promoted */
+            FTOT = (double)k * d->THETA;
+        } else {
+            FTOT = d->THETA / REPR;
+        }
+        const int P = d->relax_nPEz, nRz = d->nPz / P;
+        double* loc = (double*)calloc((size_t)(nRz + 2) * 2, sizeof(double));
+        double* endval = (double*)calloc((size_t)P, sizeof(double));
+        if (!loc || !endval) { free(loc); free(endval); c->relax_exit = 3; return; }
+        double* ALPHA = loc;
+        double* CL = loc + nRz + 2;
+        for (int r = 0; r < P; ++r) {
+            const int g0 = r * nRz;
+            for (int l = 1; l <= nRz + 1; ++l) {                            /* :383-384 */
+                const int g = g0 + l;
+                const double fc = (l <= nRz || MUT(25)) ? FCONM[g] : 0.0;
+                const double fr = (l <= nRz || r < P - 1) ? FRADM[g] : 0.0;
+                double a = 1.0 - (fc + fr) / FTOT;
+                if (!MUT(24)) a = a / (1.0 + 4.0 * fabs(a));
+                ALPHA[l] = a;
+            }
+            CL[1] = 0;                                                      /* :385-388 */
+            for (int l = 2; l <= nRz + 1; ++l)
+                CL[l] = CL[l - 1] + 0.5 * (ALPHA[l] + ALPHA[l - 1]) * (c->ZEE[g0 + l] - c->ZEE[g0 + l - 1]);
+            endval[r] = CL[nRz + 1];                                        /* :389-403 */
+            for (int l = 1; l <= nRz; ++l) CORRECT[g0 + l] = CL[l];
+        }
+        for (int r = 0; r < P; ++r) {                                       /* :405-421 */
+            double SUM = 0.0;
+            if (combo == 1) {
+                for (int k = 0; k < r; ++k) SUM += endval[k];
+            } else {
+                for (int k = P - 1; k >= r; --k) {
+                    SUM -= endval[k];
+                    if (!MUT(26)) SUM -= endval[k];
+                }
+            }
+            for (int l = 1; l <= nRz; ++l) CORRECT[r * nRz + l] += SUM;
+        }
+        free(loc);
+        free(endval);
+    } else {
+        for (int z = zl; z < zr; z++) CORRECT[z] = 0;                       /* :423-425 */
+    }
+    f_base_ext(c, ADD, R, c->FT, CORRECT, d->RLAX * c->TFAC);              /* :426 */
+    if (combo != 0 && fmod(c->TIMT, 25.0) <= c->DT && (c->ICALL % 3 == 0)) /* :427-439 */
+        c->relax_exit = 1;
+    c->ICALL++;                                                             /* :441 */
+}
 
 /* ------------------------------------------------------------------------ */
 /* fluxes() — 3dmhdsub.h:194-696                                             */
@@ -479,7 +626,7 @@ void orc_fluxes(orc_ctx* c)
         f_d1_mul(c, SUB, DY, R, FT, TT, VV, 1);
         f_d1_mul(c, SUB, DZ, R, FT, TT, WW, 1);
     }
-    if ((!shr || MUT(22)) && d->flag_lRem) {                                /* :297-302 */
+    if ((!shr || MUT(22)) && (d->RLAX > 0.0 || d->flag_lRem)) {             /* :297-302 */
         hmean(c, c->TTM, TT);
         LOOP(RE, WW1[i] = TT[i] - c->TTM[z]);
     }
@@ -501,6 +648,7 @@ void orc_fluxes(orc_ctx* c)
         f_d2_mul_az(c, ADD, DZ, R, FT, TT, RI, c->RKAPA, tmp);
         f_d1_mul_az(c, ADD, DZ, R, FT, TT, RI, c->DKAPA, tmp);
     }
+    if (d->RLAX > 0.0) relax(c);                                            /* :322-442 */
     if (d->flag_iTCon == 2) {                                               /* :443-461 */
         const double HH = d->HH, CLN = -4.0 * ln2f() / HH / HH;
         const double *EXX = c->EXX, *WYY = c->WYY, *ZEE = c->ZEE;
@@ -886,6 +1034,7 @@ double orc_cfl(orc_ctx* c, double wmin[4], double* umach)
 void orc_stage(orc_ctx* c, int k, double DT)
 {
     const mhd3d_desc* d = &c->d;
+    c->DT = DT;   /* the global DT that fluxes() reads (RLAX) */
     const rng_t R = R_in(c);
     const int mag = d->flag_lMag != 0;
     if (k == 1) {                                                      /* :155-166 */

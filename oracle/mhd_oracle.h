@@ -10,7 +10,8 @@
  *   3dmhdsub.h:31-153   step(): TSTEP/CFL, three Wray RK3 stages
  *   3dmhdsub.h:155-192  step_block_ZETA0 / ZETA / GAM
  *   3dmhdsub.h:194-696  fluxes() incl. the off-default branches iD 1/2, lRem,
- *                       lShr, iTCon 2 (RLAX is not restated: the product refuses it)
+ *                       lShr, iTCon 2 and the RLAX relaxation (:322-442) of a
+ *                       reference run on relax_nPEz z ranks
  *   3dmhdsub.h:698-718  horizontal_mean (one y rank)
  *   3dmhdsub.h:720-873  BCON()
  *   3dmhdutils.h:45-239 operator templates and reductions
@@ -59,6 +60,14 @@ void orc_stage(orc_ctx* c, int k, double dt);
 /* step() (3dmhdsub.h:31-153); fixed_dt!=0 skips TSTEP and uses dt. Returns DT. */
 double orc_step(orc_ctx* c, int fixed_dt, double dt);
 void orc_clock(orc_ctx* c, long long* nIt, double* timt, double* timc);
+/* The globals DT and TIMT that fluxes() reads (for calls outside step()). */
+void orc_set_time(orc_ctx* c, double dt, double timt);
+/* RLAX state (3dmhdsub.h:195-200): out = {TFAC, TTM_MAX, TTM_MIN}, ICALL,
+ * the exit point reached (0 none, 1 "yk not implement" :427-439, 2 "Relax:
+ * check boundary temperature" :344,416) and, when non-null, copies of the
+ * last CORRECT, FCONM and FRADM profiles (nz each).  Returns 0 or -1. */
+int orc_relax_state(orc_ctx* c, double out[3], long long* icall, int* exited, double* correct,
+                    double* fconm, double* fradm);
 
 /* RAN2 (3dmhdphsic.h:523-547).  state = {IDUM, IIY, IIR[97]} (99 ints). */
 double orc_ran2(int* state);

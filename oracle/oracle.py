@@ -57,6 +57,9 @@ def load(path):
     L.orc_step.restype = C.c_double
     L.orc_step.argtypes = [vp, C.c_int, C.c_double]
     L.orc_clock.argtypes = [vp, C.POINTER(C.c_longlong), dp, dp]
+    L.orc_set_time.argtypes = [vp, C.c_double, C.c_double]
+    L.orc_relax_state.restype = C.c_int
+    L.orc_relax_state.argtypes = [vp, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_int), dp, dp, dp]
     L.orc_ran2.restype = C.c_double
     L.orc_ran2.argtypes = [C.POINTER(C.c_int)]
     L.orc_static_profile.restype = C.c_int
@@ -145,6 +148,22 @@ class Oracle:
         tc = C.c_double()
         lib().orc_clock(self.h, C.byref(n), C.byref(t), C.byref(tc))
         return n.value, t.value, tc.value
+
+    def set_time(self, dt, timt):
+        """The globals DT and TIMT that fluxes() reads."""
+        lib().orc_set_time(self.h, float(dt), float(timt))
+
+    def relax_state(self):
+        """RLAX statics and last profiles: dict TFAC, TTM_MAX, TTM_MIN, ICALL,
+        exited, CORRECT, FCONM, FRADM."""
+        out = np.zeros(3)
+        ic = C.c_longlong()
+        ex = C.c_int()
+        nz = self.shape[0]
+        prof = [np.zeros(nz) for _ in range(3)]
+        lib().orc_relax_state(self.h, _dp(out), C.byref(ic), C.byref(ex), *(_dp(p) for p in prof))
+        return {"TFAC": out[0], "TTM_MAX": out[1], "TTM_MIN": out[2], "ICALL": ic.value,
+                "exited": ex.value, "CORRECT": prof[0], "FCONM": prof[1], "FRADM": prof[2]}
 
 
 def static_profile(nPz, THETA, GRAV, zMax):

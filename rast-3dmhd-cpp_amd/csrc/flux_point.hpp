@@ -128,6 +128,7 @@ struct ExtPt {
     double hrad = 0;             // HRAD of the plane (:322-325)
     double ft_old = 0;           // lShr: FT before this call
     double heat_amp = 0, hx = 0, hy = 0, hz = 0;   // iTCon 2 heat loss
+    double corr = 0, rscale = 0;                   // RLAX: CORRECT(z), RLAX * TFAC
 };
 
 // Hydrodynamic part of fluxes(): continuity, pressure and gravity,
@@ -138,10 +139,10 @@ struct ExtPt {
 // density diffusion of W = RO - ROM(z) into FR (iD; compiled only for an
 // accessor that gives RO at the neighbours, A::kNeighbourRO), the lRem conduction of
 // TT - TTM(z), the carried-over FT of lShr (its advection, viscous-heating
-// and pressure-work terms are skipped, :293,477-538) and the heat loss of
-// iTCon 2, each at its place in the reference's statement order.  EXTM
-// (a mask of EXT_* bits) limits the branches compiled in; the flags of e must
-// lie within it.
+// and pressure-work terms are skipped, :293,477-538), the heat loss of
+// iTCon 2 and the RLAX correction, each at its place in the reference's
+// statement order.  EXTM (a mask of EXT_* bits) limits the branches compiled
+// in; the flags of e must lie within it.
 template <bool ROT, int WALL, typename T, bool DEV, typename A, bool EXT = false,
           int EXTM = EXT_INTERIOR>
 __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const PointMetric& m, T& FU,
@@ -266,6 +267,8 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
         FT += ((d2zT * RI) * kap) * tmp;
         FT += ((dzT * RI) * T(m.dkap)) * tmp;
     }
+    // ---- RLAX relaxation :426 (func_base_extend: FT += CORRECT[z] * (RLAX*TFAC))
+    if (EXT && (EXTM & EXT_RLAX) && (e->flags & EXT_RLAX)) FT += e->corr * e->rscale;
     // ---- embedded heat loss, iTCon 2 :443-461: FT -= (((amp*RO)*gx)*gy)*gz
     if (EXT && (EXTM & EXT_HEAT) && (e->flags & EXT_HEAT))
         FT -= (((e->heat_amp * RI) * e->hx) * e->hy) * e->hz;

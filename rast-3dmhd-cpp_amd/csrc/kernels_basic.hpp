@@ -82,7 +82,9 @@ struct DevScalars {
     double nit_stop;       // stop after this many steps in all (0: no budget)
 };
 
-enum : unsigned int { HALT_GUARD = 1u, HALT_STOP = 2u };
+// HALT_EXIT: the reference's program exits at this point (RLAX, 3dmhdsub.h:
+// 344,416,427-439); the kernels of the domain do nothing until a new state.
+enum : unsigned int { HALT_GUARD = 1u, HALT_STOP = 2u, HALT_EXIT = 4u };
 
 // Background profiles of the mixed form (index k * nz + z).
 // FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
@@ -220,7 +222,8 @@ __device__ __forceinline__ PointMetric point_metric(const Metric& m, int xg, int
 // the horizontal means of planes z-1..z+1, the density-diffusion weight
 // wwz[z] (iD 1: 1.0f/ROM[z], iD 2: the host table, 3dmhdsub.h:243-252), the
 // lRem mean-profile term HRAD[z] from TTM (:322-325, same products), the
-// carried-over FT (lShr) and the heat-loss factors (iTCon 2).
+// carried-over FT (lShr), the RLAX correction of the plane and the heat-loss
+// factors (iTCon 2).
 template <int EXTM = EXT_INTERIOR>
 __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int z, int64_t i, ExtPt& e)
 {
@@ -247,6 +250,10 @@ __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int 
         e.hrad = h;
     }
     if ((EXTM & EXT_SHR) && (X.flags & EXT_SHR)) e.ft_old = X.ft[i];
+    if ((EXTM & EXT_RLAX) && (X.flags & EXT_RLAX)) {
+        e.corr = X.relax[z];
+        e.rscale = X.relax[a.g.nz];
+    }
     if ((EXTM & EXT_HEAT) && (X.flags & EXT_HEAT)) {
         e.heat_amp = X.heat_amp;
         e.hx = X.heat_x[x + 1];
@@ -798,39 +805,30 @@ MHD_TU_LOCAL __global__ void set_stop_kernel(DevScalars* s, double nit_stop)
     if (nit_stop > 0.0 && s->nit >= nit_stop) s->halted |= HALT_STOP;
 }
 
-// horizontal_mean (3dmhdsub.h:698-718) of RO (blockIdx.y = 0) and TT (1) on
-// every plane of register U, into out[k * nz + z]: the sum over the plane's
-// interior in the reference's order -- y outer, x inner, one running double --
-// so it rounds exactly as there, divided by float(nPx*nPy).  A sum in that
-// order is one chain of dependent additions, walked by one wave per plane
-// (the planes in parallel).  The wave streams the plane's nPx*nPy values in
-// that order as chunks of 64 (lane l loads value 64c + l; a chunk may span
-// two rows), PM_G chunks per group with the next group's loads in flight,
-// stages each group in LDS and adds it from uniform-address LDS reads,
-// which reach every lane at once: the additions are the only serial work.
-// For RO the z ghost planes take the planes two in, which fluxes() copies
-// there before taking the mean (3dmhdsub.h:230-239).
+// horizontal_mean (3dmhdsub.h:698-718): the sum over a plane's interior in
+// the reference's order -- y outer, x inner, one running double -- so it
+// rounds exactly as there.  A sum in that order is one chain of dependent
+// additions, walked by one wave per plane (the planes in parallel).  The
+// wave streams the plane's nPx*nPy summands in that order as chunks of 64
+// (lane l forms summand 64c + l; a chunk may span two rows), PM_G chunks per
+// group with the next group's loads in flight, stages each group in LDS and
+// adds it from uniform-address LDS reads, which reach every lane at once:
+// the additions are the only serial work.  value(y, x) forms the summand of
+// interior point (x, y); the sum is returned to every lane.
 constexpr int PM_G = 8;
 
-MHD_TU_LOCAL __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt,
-                  double* __restrict__ out)
+template <typename V>
+__device__ __forceinline__ double ordered_plane_sum(const Geom& g, V value, double (*buf)[PM_G * 64])
 {
-    __shared__ __align__(16) double buf[2][PM_G * 64];
-    const int z = blockIdx.x, k = blockIdx.y;
-    if (k == 0 ? !want_ro : !want_tt) return;
-    int zs = z;
-    if (k == 0) zs = z == 0 ? 2 : (z == g.nz - 1 ? g.nz - 3 : z);
-    const double* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
     const int lane = threadIdx.x;
     const int nPx = g.nPx;
     const int64_t N = (int64_t)nPx * g.nPy;
-    // (y, x) of this lane's next value f, advanced by 64 values per chunk
+    // (y, x) of this lane's next summand f, advanced by 64 summands per chunk
     int64_t f = lane;
     int y = lane / nPx, x = lane - (lane / nPx) * nPx;
     const int qy = 64 / nPx, qx = 64 - qy * nPx;
     auto next = [&]() -> double {
-        const double v = f < N ? p[(int64_t)y * g.pitch + x] : 0.0;
+        const double v = f < N ? value(y, x) : 0.0;
         f += 64;
         x += qx;
         y += qy;
@@ -882,7 +880,189 @@ plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt
             for (int j = 0; j < n; ++j) s += b[j];
         }
     }
-    if (lane == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
+    return s;
+}
+
+// Horizontal means of RO (blockIdx.y = 0) and TT (1) on every plane of
+// register U, into out[k * nz + z], divided by float(nPx*nPy) as there.  For
+// RO the z ghost planes take the planes two in, which fluxes() copies there
+// before taking the mean (3dmhdsub.h:230-239).
+MHD_TU_LOCAL __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt,
+                  double* __restrict__ out)
+{
+    __shared__ __align__(16) double buf[2][PM_G * 64];
+    const int z = blockIdx.x, k = blockIdx.y;
+    if (k == 0 ? !want_ro : !want_tt) return;
+    int zs = z;
+    if (k == 0) zs = z == 0 ? 2 : (z == g.nz - 1 ? g.nz - 3 : z);
+    const double* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
+    const double s = ordered_plane_sum(
+        g, [&](int y, int x) -> double { return p[(int64_t)y * g.pitch + x]; }, buf);
+    if (threadIdx.x == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
+}
+
+// FCONM of the RLAX relaxation (3dmhdsub.h:349-353): horizontal_mean of the
+// convective flux WW2 = RW*WW1*(-2.5) + RW*UU*UU*(-0.5) + RW*VV*VV*(-0.5) +
+// RW*WW*WW*(-0.5) on the interior planes 1..nPz (blockIdx.x + 1), with
+// UU = RU * (1/RO) etc. (:268,274-277) and WW1 = TT - TTM(z) (:297-301) --
+// or, under lShr, whose `#if !lShr` block skips that, the pressure RO*TT
+// left there at :267.  Products in the reference's order (func_mul2,
+// func_mul3: ((a*b)*c)*s).  Skipped while the domain is stopped when
+// `stopped` is set (a stage), evaluated regardless otherwise (mhd3d_fluxes).
+MHD_TU_LOCAL __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+conv_flux_mean_kernel(Geom g, const double* __restrict__ U, const double* __restrict__ ttm, int shr,
+                      const DevScalars* __restrict__ ds, int stopped, double* __restrict__ fconm)
+{
+    __shared__ __align__(16) double buf[2][PM_G * 64];
+    if (stopped && ds->halted) return;
+    const int z = blockIdx.x + 1;
+    const int64_t o = (int64_t)z * g.plane;
+    const double* __restrict__ ru = U + o;
+    const double* __restrict__ rv = U + g.fs + o;
+    const double* __restrict__ rw = U + 2 * g.fs + o;
+    const double* __restrict__ ro = U + 3 * g.fs + o;
+    const double* __restrict__ tt = U + 4 * g.fs + o;
+    const double tm = ttm[z];
+    auto conv = [&](int y, int x) -> double {
+        const int64_t i = (int64_t)y * g.pitch + x;
+        const double r = ro[i], t = tt[i], w = rw[i];
+        const double ri = 1.0 / r;
+        const double uu = ru[i] * ri, vv = rv[i] * ri, ww = w * ri;
+        const double w1 = shr ? r * t : t - tm;
+        double v = (w * w1) * -2.5;
+        v += ((w * uu) * uu) * -0.5;
+        v += ((w * vv) * vv) * -0.5;
+        v += ((w * ww) * ww) * -0.5;
+        return v;
+    };
+    const double s = ordered_plane_sum(g, conv, buf);
+    if (threadIdx.x == 0) fconm[z] = s / (double)(float)(g.nPx * g.nPy);
+}
+
+// The statics of fluxes() that the relaxation keeps between calls
+// (3dmhdsub.h:195-198) and the exit point reached, on the device.
+struct RelaxState {
+    double TFAC, TTM_MAX, TTM_MIN;
+    double icall;              // ICALL (an exact integer in a double)
+    unsigned int exited;       // 0; 1: "yk not implement" (:427-439); 2: "Relax: check boundary temperature" (:344,416)
+    unsigned int exit_stage;   // the RK stage (1-3) whose fluxes() call exited; 0 outside a step
+};
+
+// Host-formed constants of the relaxation.
+struct RelaxArgs {
+    double RLAX, TSTART, TOFF, REPR, FTOT, hz, TIMI;
+    int combo;   // 1: iTCon 0 / izCon 1; 2: iTCon 1 / izCon 0; 0: other walls
+    int P, nRz;  // z ranks of the reference run and their planes
+    const double* __restrict__ dzzdz;
+    const double* __restrict__ RKAPA;
+    const double* __restrict__ ZEE;
+};
+
+// FRADM of global plane z (3dmhdsub.h:356-365): the central difference of
+// TTM, one-sided on the two walls of the domain.
+__device__ __forceinline__ double relax_fradm(const RelaxArgs& r, const double* __restrict__ TTM, int z, int nPz)
+{
+    double d;
+    if (z == 1)
+        d = -3.0 * TTM[z] + 4.0 * TTM[z + 1] - TTM[z + 2];
+    else if (z == nPz)
+        d = 3.0 * TTM[z] - 4.0 * TTM[z - 1] + TTM[z - 2];
+    else
+        d = TTM[z + 1] - TTM[z - 1];
+    return d * r.hz * r.dzzdz[z] * r.RKAPA[z] / r.REPR;
+}
+
+// The relaxation's host-side logic of fluxes() (3dmhdsub.h:322-441), for a
+// reference run on P z ranks of nRz planes (rank q's local plane l is global
+// plane q*nRz + l), by one thread: the TFAC update from the wall-plane mean,
+// ALPHA from FCONM and FRADM, the per-rank trapezoid scans of CORRECT over
+// planes 1..nRz+1 (the rank's upper ghost plane with FCONM = 0 -- WW2 is
+// never written on z ghost planes -- and FRADM of the next rank's first
+// plane, 0 on the last rank, which receives none), the end values summed
+// across ranks as :405-421 do, and the exit test.  Writes out[z] = CORRECT
+// (planes 1..nPz) and out[nz] = RLAX*TFAC; endval holds P values.  `stage`
+// (1-3) marks a call inside a step: the domain then stops at once at an
+// exit (the stage's kernels do nothing); with stage 0 (mhd3d_fluxes) the
+// state records the exit and the host stops the domain after the fluxes.
+MHD_TU_LOCAL __global__ void relax_kernel(RelaxArgs r, Geom g, const double* __restrict__ TTM,
+                                          const double* __restrict__ FCONM, DevScalars* ds,
+                                          RelaxState* rs, double* __restrict__ out,
+                                          double* __restrict__ endval, int stage)
+{
+    if (threadIdx.x != 0) return;
+    if (stage && ds->halted) return;
+    const int nPz = g.nPz, P = r.P, nRz = r.nRz;
+    const double DT = ds->dt;
+    const double TIMT = r.TIMI + ds->timc;
+    if ((TIMT > r.TSTART) && (rs->TFAC > (double)1e-4f)) {
+        const double step = DT / (3.0 * r.TOFF);
+        if (r.combo == 1) {
+            if (TTM[nPz] <= rs->TTM_MAX) {
+                rs->TFAC -= step;
+            } else {
+                rs->TTM_MAX = TTM[nPz];
+                rs->TFAC += step;
+                if (2.0 < rs->TFAC) rs->TFAC = 2.0;
+            }
+        } else if (r.combo == 2) {
+            if (TTM[1] >= rs->TTM_MIN) {
+                rs->TFAC -= step;
+            } else {
+                rs->TTM_MIN = TTM[1];
+                rs->TFAC += step;
+                if (2.0 < rs->TFAC) rs->TFAC = 2.0;
+            }
+        } else {
+            rs->exited = 2;
+            rs->exit_stage = stage;
+            if (stage) ds->halted |= HALT_EXIT;
+            return;
+        }
+        // per-rank scans (the rank's sum of the ranks' end values follows)
+        for (int q = 0; q < P; ++q) {
+            const int g0 = q * nRz;
+            double aprev = 0.0, c = 0.0;
+            for (int l = 1; l <= nRz + 1; ++l) {
+                const int z = g0 + l;
+                const double fc = l <= nRz ? FCONM[z] : 0.0;
+                const double fr = (l <= nRz || q < P - 1) ? relax_fradm(r, TTM, z, nPz) : 0.0;
+                double a = 1.0 - (fc + fr) / r.FTOT;
+                a = a / (1.0 + 4.0 * fabs(a));
+                if (l > 1) c = c + 0.5 * (a + aprev) * (r.ZEE[z] - r.ZEE[z - 1]);
+                if (l <= nRz) out[z] = c;
+                aprev = a;
+            }
+            endval[q] = c;
+        }
+        for (int q = 0; q < P; ++q) {
+            double SUM = 0.0;
+            if (r.combo == 1) {
+                for (int k = 0; k < q; ++k) SUM += endval[k];
+            } else {
+                for (int k = P - 1; k >= q; --k) {
+                    SUM -= endval[k];
+                    SUM -= endval[k];
+                }
+            }
+            for (int l = 1; l <= nRz; ++l) out[q * nRz + l] += SUM;
+        }
+    } else {
+        for (int z = 1; z <= nPz; ++z) out[z] = 0.0;
+    }
+    out[g.nz] = r.RLAX * rs->TFAC;
+    if (r.combo != 0 && fmod(TIMT, 25.0) <= DT && fmod(rs->icall, 3.0) == 0.0) {
+        rs->exited = 1;
+        rs->exit_stage = stage;
+        if (stage) ds->halted |= HALT_EXIT;
+    }
+    rs->icall += 1.0;
+}
+
+// Stops the domain at a recorded exit (after an mhd3d_fluxes call).
+MHD_TU_LOCAL __global__ void relax_stop_kernel(DevScalars* ds, const RelaxState* rs)
+{
+    if (rs->exited) ds->halted |= HALT_EXIT;
 }
 
 // Non-finite scan of a register (interior incl. walls): one thread per

@@ -62,13 +62,19 @@ struct mhd3d_ctx {
     int tune_waves[2][2] = {{1, 1}, {2, 2}};
     int tune_sched[2][2] = {{3, 2}, {2, 2}};
     bool fuse_cfl = true;           // TSTEP evaluated inside the stage-3 launch
-    // off-default branches (iD, lRem, lShr, iTCon 2, TP plume): FP64, evaluated
+    // off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX): FP64, evaluated
     // by the point and wall kernels
     ExtArgs xa{};
     bool ext = false;
     bool ext_timedep = false;       // heat loss or plume with TC != 0: factors of TIMT per step
     double* ext_buf = nullptr;      // means [2 nz] | wwz2 [nz] | heat x, y, z | plume x, y
     double* ft = nullptr;           // lShr: the FT register (field layout)
+    // RLAX: FCONM [nz] | CORRECT [nz] + RLAX*TFAC | rank end values [nz], the
+    // relaxation statics, and their constants
+    double* relax_buf = nullptr;
+    RelaxState* rs = nullptr;
+    RelaxArgs ra{};
+    bool relax_exit_seen = false;   // the exit's register has been settled on the host
     int target_blocks = 0;          // work items of one streaming launch (z chunking);
                                     // 0: the resident blocks of the kernel variant
     int64_t bytes = 0;
@@ -113,6 +119,7 @@ static int ext_flags(const mhd3d_desc* d)
     if (d->flag_iTCon == 2) f |= EXT_HEAT;
     if ((d->flag_iTCon == 0 || d->flag_iTCon == 1) && d->TP != 0.0) f |= EXT_PLUME;
     if (d->flag_iD != 0 || d->flag_lRem) f |= EXT_ROCOPY;
+    if (d->RLAX > 0.0) f |= EXT_RLAX;                   // `if (RLAX > 0.0)` (3dmhdsub.h:322)
     return f;
 }
 
@@ -127,17 +134,20 @@ static int validate(const mhd3d_desc* d, std::string& why)
         return MHD3D_EINVAL;
     }
     if (d->flag_ixCon != 0 || d->flag_iyCon != 0) { why = "fluxes: Non-periodic horizontal boundaries (3dmhdsub.h:208)"; return MHD3D_EUNSUPPORTED; }
-    if (d->RLAX != 0.0) {
-        why = "RLAX relaxation is refused: the reference exits in it (\"yk not implement\", 3dmhdsub.h:427-439) "
-              "or rejects the wall combination (:344,416)";
-        return MHD3D_EUNSUPPORTED;
+    if (d->RLAX > 0.0) {
+        if (d->relax_nPEz < 2 || d->nPz % d->relax_nPEz != 0 || d->nPz / d->relax_nPEz < 3) {
+            why = "RLAX > 0 needs relax_nPEz: the z ranks of the reference run (>= 2, dividing nPz, "
+                  "at least 3 planes each; the relaxation scan of 3dmhdsub.h:354-421 runs per rank)";
+            return MHD3D_EINVAL;
+        }
+        if (!d->ZEE) { why = "RLAX > 0 needs the coordinate array ZEE (3dmhdsub.h:387)"; return MHD3D_EINVAL; }
     }
     if (d->flag_iD < 0 || d->flag_iD > 2) { why = "iD must be 0, 1 or 2 (3dmhdsub.h:243-252)"; return MHD3D_EINVAL; }
     if (d->flag_iTCon < 0 || d->flag_iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
     if (d->flag_izCon != 0 && d->flag_izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
     const int ef = ext_flags(d);
     if (ef && d->precision == MHD3D_MIXED) {
-        why = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) are built for the FP64 form only";
+        why = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX) are built for the FP64 form only";
         return MHD3D_EUNSUPPORTED;
     }
     if (((ef & (EXT_DH | EXT_DV)) && d->flag_iD == 2 && !d->ZEE) ||
@@ -273,6 +283,41 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
         ctx->bytes += (int64_t)bytes;
         X.ft = ctx->ft;
     }
+    if (X.flags & EXT_RLAX) {
+        // FCONM [nz] | CORRECT [nz] and RLAX*TFAC | rank end values [nz] | ZEE [nz]
+        const size_t nb = 4 * (size_t)nz + 1;
+        std::vector<double> hb(nb, 0.0);
+        std::memcpy(&hb[3 * (size_t)nz + 1], d->ZEE, nz * sizeof(double));
+        HIPCHK(ctx, hipMalloc(&ctx->relax_buf, nb * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(ctx->relax_buf, hb.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        RelaxState r0{};
+        r0.TFAC = 1.0;          // the statics' initial values (3dmhdsub.h:196-198)
+        r0.TTM_MAX = -1e99;
+        r0.TTM_MIN = 1e99;
+        HIPCHK(ctx, hipMalloc(&ctx->rs, sizeof(RelaxState)));
+        HIPCHK(ctx, hipMemcpy(ctx->rs, &r0, sizeof(RelaxState), hipMemcpyHostToDevice));
+        ctx->bytes += (int64_t)(nb * sizeof(double) + sizeof(RelaxState));
+        X.relax = ctx->relax_buf + nz;
+        RelaxArgs& ra = ctx->ra;
+        ra.RLAX = d->RLAX;
+        ra.TSTART = d->TSTART;
+        ra.TOFF = d->TOFF;
+        ra.REPR = d->REPR;
+        ra.hz = d->hz;
+        ra.TIMI = d->TIMI;
+        if (d->flag_lRem) {
+            const float k = 8.07f * 0.4f;   // FTOT = 8.07f * 0.4f * THETA: a float product first (:379)
+            ra.FTOT = (double)k * d->THETA;
+        } else {
+            ra.FTOT = d->THETA / d->REPR;   // :381
+        }
+        ra.combo = (d->flag_iTCon == 0 && d->flag_izCon == 1) ? 1 : (d->flag_iTCon == 1 && d->flag_izCon == 0) ? 2 : 0;
+        ra.P = d->relax_nPEz;
+        ra.nRz = d->nPz / d->relax_nPEz;
+        ra.dzzdz = ctx->m.dzzdz;
+        ra.RKAPA = ctx->m.RKAPA;
+        ra.ZEE = ctx->relax_buf + 3 * (size_t)nz + 1;
+    }
     step_factors(ctx, d->TIMI);
     return MHD3D_OK;
 }
@@ -371,8 +416,8 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (hipHostMalloc(&ctx->host_stage, (size_t)g.fs * sizeof(double)) != hipSuccess ||
         hipHostMalloc(&ctx->hs, sizeof(DevScalars)) != hipSuccess)
         return cleanup(MHD3D_ENOMEM, "hipHostMalloc failed");
-    ctx->bytes = (int64_t)((nU64 + 1) * reg + (ctx->mixed ? 2 * reg32 : 0) + nmet * sizeof(double) +
-                           sizeof(DevScalars));
+    ctx->bytes += (int64_t)((nU64 + 1) * reg + (ctx->mixed ? 2 * reg32 : 0) + nmet * sizeof(double) +
+                            sizeof(DevScalars));
     const char* km = std::getenv("MHD3D_KERNEL");
     if (km && std::strcmp(km, "point") == 0) ctx->kernel_mode = 1;
     // MHD3D_TUNE_<FP64|MIXED>_<HYDRO|MHD>=waves,sched overrides a variant
@@ -426,6 +471,8 @@ extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
     (void)hipFree(ctx->cfl_parts);
     (void)hipFree(ctx->ext_buf);
     (void)hipFree(ctx->ft);
+    (void)hipFree(ctx->relax_buf);
+    (void)hipFree(ctx->rs);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     if (ctx->hs) (void)hipHostFree(ctx->hs);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -602,6 +649,7 @@ extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
                                    hipMemcpyDeviceToDevice, ctx->stream));
     }
     clear_halt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);   // a new state runs again
+    ctx->relax_exit_seen = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
@@ -649,6 +697,7 @@ extern "C" int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const d
                                    hipMemcpyDeviceToDevice, ctx->stream));
     }
     clear_halt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);   // a new state runs again
+    ctx->relax_exit_seen = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MHD3D_OK;
@@ -783,16 +832,34 @@ static int ensure_cfl_parts(mhd3d_ctx* ctx, int n)
     return MHD3D_OK;
 }
 
-// Horizontal means of the current register's RO and TT, when a branch
-// reads them (iD: ROM, lRem: TTM).
-static int run_means(mhd3d_ctx* ctx, const double* U)
+// What a fluxes() evaluation on register U needs before its point kernels:
+// the horizontal means of RO and TT when a branch reads them (iD: ROM;
+// lRem, RLAX: TTM -- which under lShr is never taken, 3dmhdsub.h:293-302, so
+// it stays zero), then for RLAX the mean convective flux FCONM and the
+// relaxation's scalar pass (TFAC, CORRECT, the exit test).  stage = 1..3
+// inside a step (a stopped domain skips it), 0 for mhd3d_fluxes.
+static int run_prepass(mhd3d_ctx* ctx, const double* U, int stage)
 {
     const int f = ctx->xa.flags;
-    const int want_ro = (f & (EXT_DH | EXT_DV)) ? 1 : 0, want_tt = (f & EXT_REM) ? 1 : 0;
-    if (!want_ro && !want_tt) return MHD3D_OK;
-    plane_mean_kernel<<<dim3(ctx->g.nz, 2, 1), 64, 0, ctx->stream>>>(ctx->g, U, want_ro, want_tt,
-                                                                     ctx->ext_buf);
-    HIPCHK(ctx, hipGetLastError());
+    const int want_ro = (f & (EXT_DH | EXT_DV)) ? 1 : 0;
+    const int want_tt = ((f & (EXT_REM | EXT_RLAX)) && !(f & EXT_SHR)) ? 1 : 0;
+    if (want_ro || want_tt) {
+        plane_mean_kernel<<<dim3(ctx->g.nz, 2, 1), 64, 0, ctx->stream>>>(ctx->g, U, want_ro, want_tt,
+                                                                         ctx->ext_buf);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    if (f & EXT_RLAX) {
+        const int nz = ctx->g.nz;
+        double* fconm = ctx->relax_buf;
+        conv_flux_mean_kernel<<<ctx->g.nPz, 64, 0, ctx->stream>>>(ctx->g, U, ctx->ext_buf + nz,
+                                                                  (f & EXT_SHR) ? 1 : 0, ctx->ds,
+                                                                  stage ? 1 : 0, fconm);
+        HIPCHK(ctx, hipGetLastError());
+        relax_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
+                                              ctx->relax_buf + nz, ctx->relax_buf + 2 * (size_t)nz + 1,
+                                              stage);
+        HIPCHK(ctx, hipGetLastError());
+    }
     return MHD3D_OK;
 }
 
@@ -800,7 +867,7 @@ static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
 {
     StageArgs a = stage_args(ctx, k);
     if (ctx->ext) {
-        int rc = run_means(ctx, a.Uin);
+        int rc = run_prepass(ctx, a.Uin, k);
         if (rc) return rc;
     }
     if (fused_cfl && k == 3 && ctx->kernel_mode == 0) {
@@ -865,6 +932,15 @@ static int collect_timing(mhd3d_ctx* ctx)
     }
     ctx->ev_used = 0;
     return MHD3D_OK;
+}
+
+// MHD3D_EEXIT with the message of the reference's exit_err.
+static int relax_exit_status(mhd3d_ctx* ctx, const RelaxState& rs)
+{
+    return fail(ctx, MHD3D_EEXIT,
+                rs.exited == 2 ? "Relax: check boundary temperature (3dmhdsub.h:344,416: the reference exits)"
+                               : "yk not implement : sub 350 (3dmhdsub.h:427-439: the reference exits at TIMT "
+                                 "within DT after a multiple of 25)");
 }
 
 static int step_core(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
@@ -934,6 +1010,17 @@ static int step_core(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
     int rc = collect_timing(ctx);
     if (rc) return rc;
     if (dt_inout && nsteps > 0) *dt_inout = hs.dt_last;
+    if ((ctx->xa.flags & EXT_RLAX) && (hs.halted & HALT_EXIT)) {
+        RelaxState rs;
+        HIPCHK(ctx, hipMemcpy(&rs, ctx->rs, sizeof(RelaxState), hipMemcpyDeviceToHost));
+        // the state stays the input of the stage whose fluxes() exited:
+        // stage 2 reads the register stage 1 wrote, stages 1 and 3 the
+        // register the step started from (the one the taken steps name)
+        if (!ctx->relax_exit_seen && rs.exit_stage == 2) ctx->cur ^= 1;
+        ctx->relax_exit_seen = true;
+        ctx->dt_ready = false;
+        return relax_exit_status(ctx, rs);
+    }
     if (hs.halted & HALT_GUARD) return fail(ctx, MHD3D_EHALTED, "DT too small, halt (DT guard)");
     if (hs.nonfinite) return fail(ctx, MHD3D_ENONFINITE, "state is not finite after the step");
     return MHD3D_OK;
@@ -1036,6 +1123,12 @@ extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
     HIPCHK(ctx, hipMemcpyAsync(ctx->hs, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->hs->halted) ctx->cur = 1 - ctx->cur;   // the kernels of a stopped domain did nothing
+    if ((ctx->xa.flags & EXT_RLAX) && (ctx->hs->halted & HALT_EXIT)) {
+        RelaxState rs;
+        HIPCHK(ctx, hipMemcpy(&rs, ctx->rs, sizeof(RelaxState), hipMemcpyDeviceToHost));
+        ctx->relax_exit_seen = true;
+        return relax_exit_status(ctx, rs);
+    }
     return collect_timing(ctx);
 }
 
@@ -1060,7 +1153,7 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz);
     const bool mag = ctx->d.flag_lMag, rot = ctx->d.flag_lRot;
     if (ctx->ext) {
-        int rc = run_means(ctx, a.Uin);
+        int rc = run_prepass(ctx, a.Uin, 0);
         if (rc) return rc;
         if (mag && rot) flux_only_kernel<true, true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
         else if (mag) flux_only_kernel<true, false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
@@ -1084,6 +1177,18 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
             for (int y = 0; y < g.nPy; ++y)
                 std::memcpy(dst + ((size_t)z * ny + (y + 1)) * nx + 1,
                             h + (size_t)z * g.plane + (size_t)y * g.pitch, g.nPx * sizeof(double));
+    }
+    if (ctx->xa.flags & EXT_RLAX) {
+        // the reference's fluxes() ends in exit_err at an exit point: the
+        // fluxes are complete, then the domain stops
+        RelaxState rs;
+        HIPCHK(ctx, hipMemcpyAsync(&rs, ctx->rs, sizeof(RelaxState), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if (rs.exited) {
+            relax_stop_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds, ctx->rs);
+            HIPCHK(ctx, hipGetLastError());
+            return relax_exit_status(ctx, rs);
+        }
     }
     return MHD3D_OK;
 }
@@ -1125,6 +1230,27 @@ extern "C" int mhd3d_cfl(mhd3d_ctx* ctx, mhd3d_cfl_info* out)
     for (int k = 0; k < 4; ++k) out->wmin[k] = hs.wmin[k];
     out->umach = hs.umach;
     out->dt = hs.dt;
+    return MHD3D_OK;
+}
+
+extern "C" int mhd3d_relax_state(mhd3d_ctx* ctx, double out[3], int64_t* icall, int* exited)
+{
+    if (!ctx) return MHD3D_EINVAL;
+    RelaxState rs{};
+    rs.TFAC = 1.0;
+    rs.TTM_MAX = -1e99;
+    rs.TTM_MIN = 1e99;
+    if (ctx->rs) {
+        HIPCHK(ctx, hipMemcpyAsync(&rs, ctx->rs, sizeof(RelaxState), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (out) {
+        out[0] = rs.TFAC;
+        out[1] = rs.TTM_MAX;
+        out[2] = rs.TTM_MIN;
+    }
+    if (icall) *icall = (int64_t)rs.icall;
+    if (exited) *exited = rs.exited ? 1 : 0;
     return MHD3D_OK;
 }
 
@@ -1450,7 +1576,7 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
             return bail(MHD3D_EINVAL, "ensemble members must share one precision form");
         if (ext_flags(descs[m]))
             return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) +
-                                                ": the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) "
+                                                ": the off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX) "
                                                 "run on single domains only");
         mhd3d_ctx* c = nullptr;
         int rc = mhd3d_create(descs[m], &c);
@@ -1954,7 +2080,7 @@ extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, cons
     if (rc) { s->err = why; return rc; }
     if (desc->precision != s->precision) { s->err = "request precision differs from the server's"; return MHD3D_EINVAL; }
     if (ext_flags(desc)) {
-        s->err = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume) run on single domains only";
+        s->err = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX) run on single domains only";
         return MHD3D_EUNSUPPORTED;
     }
     ServerReq r;

@@ -98,12 +98,13 @@ enum : int {
     EXT_HEAT = 16,   // iTCon == 2: embedded Gaussian heat loss (:443-461)
     EXT_PLUME = 32,  // TP != 0 with iTCon 0/1: Gaussian top-wall temperature (:750-776)
     EXT_ROCOPY = 64, // iD != 0 or lRem: RO ghost planes read as the planes two in (:230-239)
+    EXT_RLAX = 128,  // RLAX > 0: FT += CORRECT(z) * RLAX * TFAC (:322-442)
 };
 
 // The flags whose branches act on interior points (fluxes() terms); the
 // others (the plume walls, the RO ghost copy) act on the wall planes only,
 // so a run with none of these keeps the default interior kernel.
-constexpr int EXT_INTERIOR = EXT_DH | EXT_DV | EXT_REM | EXT_SHR | EXT_HEAT;
+constexpr int EXT_INTERIOR = EXT_DH | EXT_DV | EXT_REM | EXT_SHR | EXT_HEAT | EXT_RLAX;
 
 // Operands of the off-default branches (device pointers, host-formed scalars).
 struct ExtArgs {
@@ -123,6 +124,9 @@ struct ExtArgs {
     const double* __restrict__ plume_x = nullptr; // TP plume: exp(CLN*(EXX-XP)^2), per ghosted x
     const double* __restrict__ plume_y = nullptr; //   ... per ghosted y
     double tpr = 0.0;                             //   TPR of this step (3dmhdsub.h:752-757)
+    // RLAX: CORRECT per plane [0, nz) and the factor RLAX * TFAC at [nz],
+    // formed on the device before each fluxes() evaluation
+    const double* __restrict__ relax = nullptr;
     double TU = 0.0, DZTU = 0.0, DZTB = 0.0;      // lRem wall temperatures (3dmhdphsic.h:590-592)
     double ORZD = 0.0;                            // 1.0f / double(nPz - 1) (3dmhdsub.h:768)
     double HH = 0.0, TIMI = 0.0;

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 LIB_PATH = os.path.join(PKG, "lib", "libmhd3d.so")
 
-OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENONFINITE, EHALTED = range(7)
+OK, EINVAL, EUNSUPPORTED, EHIP, ENOMEM, ENONFINITE, EHALTED, EEXIT = range(8)
 REFERENCE_DT_MIN = float(np.float32(1.0e-8))   # the reference driver's DT guard (DT < 1.0e-8f)
 FP64, MIXED = 0, 1
 FIELDS = ("RU", "RV", "RW", "RO", "TT", "BX", "BY", "BZ")
@@ -37,6 +37,7 @@ SIGNATURES = {
     "mhd3d_cfl": (C.c_int, [_vp, _vp]),
     "mhd3d_diagnostics": (C.c_int, [_vp, _dp]),
     "mhd3d_clock": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp, _dp]),
+    "mhd3d_relax_state": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "mhd3d_timing_enable": (C.c_int, [_vp, C.c_int]),
     "mhd3d_timing_read": (C.c_int, [_vp, _vp]),
     "mhd3d_synchronize": (C.c_int, [_vp]),

@@ -38,7 +38,7 @@ class MhdDesc(C.Structure):
         ("flag_lRem", C.c_int32), ("flag_lShr", C.c_int32),
         ("flag_ixCon", C.c_int32), ("flag_iyCon", C.c_int32), ("flag_izCon", C.c_int32),
         ("flag_iTCon", C.c_int32), ("flag_iBCon", C.c_int32), ("flag_iD", C.c_int32),
-        ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("precision", C.c_int32), ("relax_nPEz", C.c_int32),
         ("C13", C.c_double), ("C23", C.c_double), ("C43", C.c_double),
         ("GAM1", C.c_double), ("GAM2", C.c_double), ("GAM3", C.c_double),
         ("ZETA1", C.c_double), ("ZETA2", C.c_double),
@@ -52,7 +52,8 @@ class MhdDesc(C.Structure):
         ("OMX", C.c_double), ("OMZ", C.c_double),
         ("RM", C.c_double), ("ORM", C.c_double), ("OBETA", C.c_double),
         ("TP", C.c_double), ("TB", C.c_double), ("TC", C.c_double),
-        ("RLAX", C.c_double), ("TIMI", C.c_double),
+        ("RLAX", C.c_double), ("TSTART", C.c_double), ("TOFF", C.c_double),
+        ("TIMI", C.c_double),
         ("dxxdx", C.POINTER(C.c_double)), ("d2xxdx2", C.POINTER(C.c_double)),
         ("DDX", C.POINTER(C.c_double)),
         ("dyydy", C.POINTER(C.c_double)), ("d2yydy2", C.POINTER(C.c_double)),
@@ -190,6 +191,9 @@ class RunConfig:
     DZTU: float = 0.0
     DZTB: float = 0.0
     kappa: tuple | None = None
+    # RLAX > 0: the z ranks of the reference run whose relaxation scan
+    # (3dmhdsub.h:354-421) is reproduced
+    relax_nPEz: int = 0
 
 
 class Descriptor:
@@ -206,6 +210,7 @@ class Descriptor:
         d.flag_ixCon, d.flag_iyCon, d.flag_izCon = cfg.ixCon, cfg.iyCon, cfg.izCon
         d.flag_iTCon, d.flag_iBCon, d.flag_iD = cfg.iTCon, cfg.iBCon, cfg.iD
         d.precision = cfg.precision
+        d.relax_nPEz = cfg.relax_nPEz
         for k, v in self.consts.items():
             setattr(d, k, v)
         self._arrays = {}
@@ -273,7 +278,8 @@ def set_constants(cfg: RunConfig):
     c["TP"] = TP
     c["TB"] = float(cfg.TB)
     c["TC"] = TC
-    c["RLAX"] = par[59]
+    c["RLAX"] = par[59]                                         # :428-430
+    c["TSTART"], c["TOFF"] = par[57], par[58]
     c["TIMI"] = float(cfg.TIMI)
     c["DH"], c["DV"] = par[55], par[56]                        # :449
     c["zMax"] = zMax

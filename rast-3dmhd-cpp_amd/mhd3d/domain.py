@@ -111,6 +111,16 @@ class Domain:
         self._check(L.lib().mhd3d_clock(self.h, C.byref(n), C.byref(t), C.byref(tc)))
         return n.value, t.value, tc.value
 
+    def relax_state(self):
+        """The RLAX statics (3dmhdsub.h:195-198): dict TFAC, TTM_MAX, TTM_MIN,
+        ICALL and exited (the reference's exit point reached)."""
+        out = (C.c_double * 3)()
+        ic = C.c_int64()
+        ex = C.c_int()
+        self._check(L.lib().mhd3d_relax_state(self.h, out, C.byref(ic), C.byref(ex)))
+        return {"TFAC": out[0], "TTM_MAX": out[1], "TTM_MIN": out[2], "ICALL": ic.value,
+                "exited": ex.value}
+
     def timing(self, on=None):
         if on is not None:
             self._check(L.lib().mhd3d_timing_enable(self.h, int(bool(on))))

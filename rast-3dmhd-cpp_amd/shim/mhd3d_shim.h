@@ -39,7 +39,8 @@ static void gpu_init()
     d.RE = RE; d.ORE = ORE; d.REPR = REPR; d.CV = CV; d.OCV = OCV;
     d.GAMMA = GAMMA; d.THETA = THETA; d.GRAV = GRAV; d.OMX = OMX; d.OMZ = OMZ;
     d.RM = RM; d.ORM = ORM; d.OBETA = OBETA; d.TP = TP; d.TB = TB; d.TC = TC;
-    d.RLAX = RLAX; d.TIMI = TIMI;
+    d.RLAX = RLAX; d.TSTART = TSTART; d.TOFF = TOFF; d.TIMI = TIMI;
+    d.relax_nPEz = nPEz;   // RLAX > 0: the rank layout whose relaxation scan is reproduced
     d.dxxdx = dxxdx; d.d2xxdx2 = d2xxdx2; d.DDX = DDX;
     d.dyydy = dyydy; d.d2yydy2 = d2yydy2; d.DDY = DDY;
     d.dzzdz = dzzdz; d.d2zzdz2 = d2zzdz2; d.DDZ = DDZ;
@@ -61,6 +62,7 @@ static void gpu_init()
 static void gpu_step()
 {
     int rc = mhd3d_step(g_gpu, 1, 0, &DT);
+    // MHD3D_EEXIT: the point where the reference's own fluxes() calls exit_err
     if (rc != MHD3D_OK && rc != MHD3D_ENONFINITE) exit_err(mhd3d_last_error(g_gpu));
     int64_t it;
     mhd3d_clock(g_gpu, &it, &TIMT, &TIMC);

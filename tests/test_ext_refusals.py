@@ -2,9 +2,10 @@
 the descriptor before its first HIP call, so a refused descriptor returns
 without a device).
 
-Refused: RLAX != 0 (the reference's relaxation exits, 3dmhdsub.h:427-439, or
-rejects the wall combination, :344,416), the branches in the mixed form, iD
-outside 0..2 and missing coordinate arrays.
+Refused: the branches (RLAX among them) in the mixed form, iD outside 0..2,
+missing coordinate arrays, and RLAX without the z rank layout of the
+reference run its scan reproduces (relax_nPEz >= 2 dividing nPz, at least 3
+planes per rank, 3dmhdsub.h:354-421).
 """
 import ctypes as C
 import dataclasses
@@ -34,11 +35,14 @@ def _base(**kw):
     return dataclasses.replace(stratified_config(12, 10, 8), **kw)
 
 
-def test_rlax_refused():
+def test_rlax_needs_the_rank_layout():
     cfg = _base()
     par = cfg.par.copy()
     par[59] = 0.1
-    rc, msg = _create(dataclasses.replace(cfg, par=par))
+    for P in (0, 1, 3, 4):                  # nPz = 8: 3 does not divide it, 4 leaves 2 planes a rank
+        rc, msg = _create(dataclasses.replace(cfg, par=par, relax_nPEz=P))
+        assert rc == L.EINVAL and "relax_nPEz" in msg, P
+    rc, msg = _create(dataclasses.replace(cfg, par=par, relax_nPEz=2, precision=1))
     assert rc == L.EUNSUPPORTED and "RLAX" in msg
 
 

@@ -15,6 +15,7 @@ import pytest
 
 import test_oracle_ext as te
 import test_oracle_math as tm
+import test_oracle_relax as tr
 import test_oracle_stretched as ts
 from oracle import oracle as om
 
@@ -41,6 +42,11 @@ MUTANTS = {
     20: "D2z with dzzdz not squared",
     21: "top-wall one-sided D1z(RW) with the metric of plane 2",
     22: "lShr with lRem updating TTM as without lShr",
+    23: "RLAX convective flux of TT instead of TT - TTM",
+    24: "RLAX ALPHA without the 1/(1+4|ALPHA|) limiter",
+    25: "RLAX rank ghost planes with the real FCONM",
+    26: "RLAX iTCon 1 sum subtracting each rank once",
+    27: "RLAX TFAC steps of DT/TOFF",
 }
 
 
@@ -75,6 +81,11 @@ def _checks():
     yield ts.test_wall_d2z_converges_on_the_tanh_grid
     yield lambda: ts.test_cfl_limits_on_the_tanh_grid(25.0)
     yield lambda: ts.test_cfl_limits_on_the_tanh_grid(1e-3)
+    yield lambda: tr.test_correct_is_alpha_times_height_at_rest(3)
+    yield tr.test_convective_flux_mean
+    yield lambda: tr.test_rank_boundaries_take_no_convective_flux(2)
+    yield lambda: tr.test_itcon1_subtracts_each_rank_end_twice(2)
+    yield tr.test_tfac_steps_and_cap
 
 
 def test_clean_oracle_passes_every_check():
