@@ -245,8 +245,13 @@ int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const double* RO_z
  * one precision form for all) advanced together -- one persistent launch per
  * RK3 stage for the planes between the walls and one for the wall planes of
  * every member, each member with its own CFL time step.  A member advances
- * bit-for-bit as the same domain does alone through mhd3d_step.  No
- * counterpart in the reference (its step() advances one domain, 3dmhdsub.h:31).
+ * bit-for-bit as the same domain does alone through mhd3d_step.  FP64
+ * members may take the off-default branches (iD, lRem, lShr, iTCon 2, the
+ * TP plume, RLAX), except the heat loss and the plume with TC != 0, whose
+ * factors of TIMT the host forms before every step (MHD3D_EUNSUPPORTED); a
+ * member that reaches an RLAX exit point stops there and mhd3d_ensemble_step
+ * returns MHD3D_EEXIT.  No counterpart in the reference (its step() advances
+ * one domain, 3dmhdsub.h:31).
  * ------------------------------------------------------------------------ */
 typedef struct mhd3d_ensemble mhd3d_ensemble;
 
@@ -298,6 +303,8 @@ int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* report, int* n
 typedef struct mhd3d_server mhd3d_server;
 
 enum {
+    MHD3D_REQ_EXITED = -3,   /* stopped where the reference's program exits
+                                (the RLAX relaxation, MHD3D_EEXIT)          */
     MHD3D_REQ_NONFINITE = -2,/* evicted: its state held NaN/Inf after a tick
                                 (the reference's driver would keep stepping
                                 it, its minima ignore NaN, 3dmhdutils.h:61-79) */
@@ -324,7 +331,8 @@ double mhd3d_server_now(const mhd3d_server* s);
 int64_t mhd3d_server_bytes_in_use(const mhd3d_server* s);
 int64_t mhd3d_server_cell_steps(const mhd3d_server* s);
 /* desc and its arrays are copied; the initial state is that of
- * mhd3d_init_stratified(TT_z, RO_z, seed, amp, b0, b_amp) */
+ * mhd3d_init_stratified(TT_z, RO_z, seed, amp, b0, b_amp); the branches an
+ * ensemble member may take are accepted (MHD3D_EUNSUPPORTED otherwise) */
 int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, const double* TT_z,
                         const double* RO_z, uint64_t seed, double amp, double b0, double b_amp,
                         int nsteps, int64_t* id);

@@ -25,8 +25,9 @@
 namespace mhd {
 
 // One interior work item: tile (tile_x, tile_y) of member `member` over the
-// planes zb .. ze-1; flags bit 0 = lMag, bit 1 = lRot; `part` = index of its
-// TSTEP partial record in the member's array.
+// planes zb .. ze-1; flags bit 0 = lMag, bit 1 = lRot, bit 3 = the member has
+// off-default branches (ExtArgs); `part` = index of its TSTEP partial record
+// in the member's array.
 struct WorkItem {
     int member, flags, tile_x, tile_y, zb, ze, part, pad;
 };
@@ -87,7 +88,47 @@ __global__ void __launch_bounds__(256, WAVES)
     }
 }
 
-// Persistent wall stage over all members (blocks of 64 x 4 threads).
+// Persistent interior stage over the members whose interior points take
+// off-default branches (FP64): every branch compiled in, as the single-domain
+// kernel that holds them all (one wave per SIMD, one barrier per plane, the
+// deferred update); each point takes its member's branches from its ExtArgs.
+template <int STAGE>
+__global__ void __launch_bounds__(256, 1)
+    ensemble_stage_ext_kernel(const StageArgs* __restrict__ args, const WorkItem* __restrict__ items,
+                              int nitems)
+{
+    __shared__ __align__(16) char lds[StreamLds<double, true, true, true>::bytes];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const StageArgs& a = args[w.member];
+        if (domain_halted(a)) continue;   // uniform over the block
+        CflAcc cfl;
+        switch (w.flags & 3) {
+        case 0:
+            stage_item<double, false, false, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+                                                                                   w.ze, lds, cfl);
+            break;
+        case 1:
+            stage_item<double, true, false, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+                                                                                  w.ze, lds, cfl);
+            break;
+        case 2:
+            stage_item<double, false, true, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+                                                                                  w.ze, lds, cfl);
+            break;
+        default:
+            stage_item<double, true, true, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+                                                                                 w.ze, lds, cfl);
+            break;
+        }
+        if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, w.part);
+        __syncthreads();   // the next item reuses the LDS ring
+    }
+}
+
+// Persistent wall stage over all members (blocks of 64 x 4 threads); a
+// member with off-default branches (flags bit 3, FP64 only) takes the wall
+// code that evaluates them.
 template <typename T, int STAGE>
 __global__ void __launch_bounds__(256)
     ensemble_wall_kernel(const StageArgs* __restrict__ args, const WallItem* __restrict__ items,
@@ -101,11 +142,21 @@ __global__ void __launch_bounds__(256)
         CflAcc cfl;
         if (x < a.g.nPx && y < a.g.nPy) {
             const bool top = w.top != 0;
-            switch (w.flags & 3) {
-            case 0: wall_point<T, false, false, STAGE>(a, x, y, top, cfl); break;
-            case 1: wall_point<T, true, false, STAGE>(a, x, y, top, cfl); break;
-            case 2: wall_point<T, false, true, STAGE>(a, x, y, top, cfl); break;
-            default: wall_point<T, true, true, STAGE>(a, x, y, top, cfl); break;
+            constexpr bool F64 = std::is_same<T, double>::value;
+            if (F64 && (w.flags & 8)) {
+                switch (w.flags & 3) {
+                case 0: wall_point<T, false, false, STAGE, F64>(a, x, y, top, cfl); break;
+                case 1: wall_point<T, true, false, STAGE, F64>(a, x, y, top, cfl); break;
+                case 2: wall_point<T, false, true, STAGE, F64>(a, x, y, top, cfl); break;
+                default: wall_point<T, true, true, STAGE, F64>(a, x, y, top, cfl); break;
+                }
+            } else {
+                switch (w.flags & 3) {
+                case 0: wall_point<T, false, false, STAGE>(a, x, y, top, cfl); break;
+                case 1: wall_point<T, true, false, STAGE>(a, x, y, top, cfl); break;
+                case 2: wall_point<T, false, true, STAGE>(a, x, y, top, cfl); break;
+                default: wall_point<T, true, true, STAGE>(a, x, y, top, cfl); break;
+                }
             }
         }
         if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, w.part);

@@ -75,6 +75,7 @@ struct mhd3d_ctx {
     RelaxState* rs = nullptr;
     RelaxArgs ra{};
     bool relax_exit_seen = false;   // the exit's register has been settled on the host
+    int relax_exit_code = 0;        // RelaxState::exited as last read
     int target_blocks = 0;          // work items of one streaming launch (z chunking);
                                     // 0: the resident blocks of the kernel variant
     int64_t bytes = 0;
@@ -122,6 +123,17 @@ static int ext_flags(const mhd3d_desc* d)
     if (d->RLAX > 0.0) f |= EXT_RLAX;                   // `if (RLAX > 0.0)` (3dmhdsub.h:322)
     return f;
 }
+
+// Branches whose factors of TIMT the host forms before every step (the heat
+// loss and the plume with TC != 0): single domains only, as ensembles and
+// the server advance many steps without the host.
+static int timedep_flags(const mhd3d_desc* d)
+{
+    return d->TC != 0.0 ? (ext_flags(d) & (EXT_HEAT | EXT_PLUME)) : 0;
+}
+static const char* const kTimedepWhy =
+    "the heat loss and the plume with TC != 0 take factors of TIMT formed on the host before every "
+    "step (3dmhdsub.h:446-460,752-757): single domains only";
 
 static int validate(const mhd3d_desc* d, std::string& why)
 {
@@ -832,32 +844,38 @@ static int ensure_cfl_parts(mhd3d_ctx* ctx, int n)
     return MHD3D_OK;
 }
 
+// Whether fluxes() of this domain needs plane means or the relaxation pass.
+static bool needs_prepass(const mhd3d_ctx* ctx)
+{
+    return (ctx->xa.flags & (EXT_DH | EXT_DV | EXT_REM | EXT_RLAX)) != 0;
+}
+
 // What a fluxes() evaluation on register U needs before its point kernels:
 // the horizontal means of RO and TT when a branch reads them (iD: ROM;
 // lRem, RLAX: TTM -- which under lShr is never taken, 3dmhdsub.h:293-302, so
 // it stays zero), then for RLAX the mean convective flux FCONM and the
 // relaxation's scalar pass (TFAC, CORRECT, the exit test).  stage = 1..3
 // inside a step (a stopped domain skips it), 0 for mhd3d_fluxes.
-static int run_prepass(mhd3d_ctx* ctx, const double* U, int stage)
+static int run_prepass(mhd3d_ctx* ctx, const double* U, int stage, hipStream_t stream)
 {
     const int f = ctx->xa.flags;
     const int want_ro = (f & (EXT_DH | EXT_DV)) ? 1 : 0;
     const int want_tt = ((f & (EXT_REM | EXT_RLAX)) && !(f & EXT_SHR)) ? 1 : 0;
     if (want_ro || want_tt) {
-        plane_mean_kernel<<<dim3(ctx->g.nz, 2, 1), 64, 0, ctx->stream>>>(ctx->g, U, want_ro, want_tt,
-                                                                         ctx->ext_buf);
+        plane_mean_kernel<<<dim3(ctx->g.nz, 2, 1), 64, 0, stream>>>(ctx->g, U, want_ro, want_tt,
+                                                                    ctx->ext_buf);
         HIPCHK(ctx, hipGetLastError());
     }
     if (f & EXT_RLAX) {
         const int nz = ctx->g.nz;
         double* fconm = ctx->relax_buf;
-        conv_flux_mean_kernel<<<ctx->g.nPz, 64, 0, ctx->stream>>>(ctx->g, U, ctx->ext_buf + nz,
-                                                                  (f & EXT_SHR) ? 1 : 0, ctx->ds,
-                                                                  stage ? 1 : 0, fconm);
+        conv_flux_mean_kernel<<<ctx->g.nPz, 64, 0, stream>>>(ctx->g, U, ctx->ext_buf + nz,
+                                                             (f & EXT_SHR) ? 1 : 0, ctx->ds,
+                                                             stage ? 1 : 0, fconm);
         HIPCHK(ctx, hipGetLastError());
-        relax_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
-                                              ctx->relax_buf + nz, ctx->relax_buf + 2 * (size_t)nz + 1,
-                                              stage);
+        relax_kernel<<<1, 1, 0, stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
+                                         ctx->relax_buf + nz, ctx->relax_buf + 2 * (size_t)nz + 1,
+                                         stage);
         HIPCHK(ctx, hipGetLastError());
     }
     return MHD3D_OK;
@@ -867,7 +885,7 @@ static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
 {
     StageArgs a = stage_args(ctx, k);
     if (ctx->ext) {
-        int rc = run_prepass(ctx, a.Uin, k);
+        int rc = run_prepass(ctx, a.Uin, k, ctx->stream);
         if (rc) return rc;
     }
     if (fused_cfl && k == 3 && ctx->kernel_mode == 0) {
@@ -934,13 +952,20 @@ static int collect_timing(mhd3d_ctx* ctx)
     return MHD3D_OK;
 }
 
-// MHD3D_EEXIT with the message of the reference's exit_err.
+// The message of the reference's exit_err at a relaxation exit point.
+static std::string relax_exit_text(const mhd3d_ctx* ctx)
+{
+    return ctx->relax_exit_code == 2
+               ? "Relax: check boundary temperature (3dmhdsub.h:344,416: the reference exits)"
+               : "yk not implement : sub 350 (3dmhdsub.h:427-439: the reference exits at TIMT "
+                 "within DT after a multiple of 25)";
+}
+
+// MHD3D_EEXIT for the exit recorded in rs.
 static int relax_exit_status(mhd3d_ctx* ctx, const RelaxState& rs)
 {
-    return fail(ctx, MHD3D_EEXIT,
-                rs.exited == 2 ? "Relax: check boundary temperature (3dmhdsub.h:344,416: the reference exits)"
-                               : "yk not implement : sub 350 (3dmhdsub.h:427-439: the reference exits at TIMT "
-                                 "within DT after a multiple of 25)");
+    ctx->relax_exit_code = (int)rs.exited;
+    return fail(ctx, MHD3D_EEXIT, relax_exit_text(ctx));
 }
 
 static int step_core(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
@@ -1153,7 +1178,7 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz);
     const bool mag = ctx->d.flag_lMag, rot = ctx->d.flag_lRot;
     if (ctx->ext) {
-        int rc = run_prepass(ctx, a.Uin, 0);
+        int rc = run_prepass(ctx, a.Uin, 0, ctx->stream);
         if (rc) return rc;
         if (mag && rot) flux_only_kernel<true, true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
         else if (mag) flux_only_kernel<true, false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
@@ -1340,14 +1365,16 @@ struct mhd3d_ensemble {
     int mirror_cap = 0;
     StageArgs* d_args = nullptr;       // [parity at step start][stage][member]
     WorkItem* d_items = nullptr;
+    WorkItem* d_items_x = nullptr;     // interior items of members with off-default branches
     WallItem* d_walls = nullptr;
     FinishArgs* d_fin = nullptr;
-    int nitems = 0, nwalls = 0;
-    int grid_int = 0, grid_wall = 0;
+    int nitems = 0, nwalls = 0, nitems_x = 0;
+    int grid_int = 0, grid_wall = 0, grid_x = 0;
+    bool prepass = false;              // some member needs plane means / the relaxation pass
     int waves = 1, sched = 0, chunk = 0;   // kernel variant; chunk: planes per item (0 = auto)
     int parity = 0;                    // FP64: register parity of the step tables
     std::vector<int> off;              // member m's state is U[parity ^ off[m]] (FP64)
-    size_t cap_items = 0, cap_walls = 0, cap_fin = 0, cap_args = 0;   // device table capacities
+    size_t cap_items = 0, cap_walls = 0, cap_fin = 0, cap_args = 0, cap_items_x = 0;   // device table capacities
     bool use_graph = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int64_t cells = 0;
@@ -1398,7 +1425,8 @@ static const void* ens_stage_fn(int sched)
 template <typename T>
 static void ens_stage(const mhd3d_ensemble* e, int k, const StageArgs* args)
 {
-    if (e->waves == 1) {
+    if (e->nitems == 0) {
+    } else if (e->waves == 1) {
         if (k == 1) ens_stage_launch<T, 1, 1>(e, args, e->stream, e->sched);
         else if (k == 2) ens_stage_launch<T, 2, 1>(e, args, e->stream, e->sched);
         else ens_stage_launch<T, 3, 1>(e, args, e->stream, e->sched);
@@ -1407,17 +1435,26 @@ static void ens_stage(const mhd3d_ensemble* e, int k, const StageArgs* args)
         else if (k == 2) ens_stage_launch<T, 2, 2>(e, args, e->stream, e->sched);
         else ens_stage_launch<T, 3, 2>(e, args, e->stream, e->sched);
     }
+    if (e->nitems_x > 0) {
+        dim3 blk(STX, STY, 1);
+        if (k == 1) ensemble_stage_ext_kernel<1><<<e->grid_x, blk, 0, e->stream>>>(args, e->d_items_x, e->nitems_x);
+        else if (k == 2) ensemble_stage_ext_kernel<2><<<e->grid_x, blk, 0, e->stream>>>(args, e->d_items_x, e->nitems_x);
+        else ensemble_stage_ext_kernel<3><<<e->grid_x, blk, 0, e->stream>>>(args, e->d_items_x, e->nitems_x);
+    }
     dim3 wblk(64, 4, 1);
     if (k == 1) ensemble_wall_kernel<T, 1><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
     else if (k == 2) ensemble_wall_kernel<T, 2><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
     else ensemble_wall_kernel<T, 3><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
 }
 
-// Resident blocks of the chosen interior variant on the whole device.
-static int ens_resident_blocks(const mhd3d_ensemble* e)
+// Resident blocks of an interior kernel on the whole device: the chosen
+// variant, or (ext) the kernel of the members with off-default branches.
+static int ens_resident_blocks(const mhd3d_ensemble* e, bool ext = false)
 {
     const void* fn;
-    if (e->mixed)
+    if (ext)
+        fn = (const void*)ensemble_stage_ext_kernel<2>;
+    else if (e->mixed)
         fn = mixed_ens_stage_fn(e->waves, e->sched);
     else
         fn = e->waves == 1 ? ens_stage_fn<double, 2, 1>(e->sched) : ens_stage_fn<double, 2, 2>(e->sched);
@@ -1486,14 +1523,16 @@ static int ens_build(mhd3d_ensemble* e)
         ENSCHK(e, hipMalloc(&e->mirror, sizeof(DevScalars) * e->n));
         e->mirror_cap = e->n;
     }
-    std::vector<WorkItem> items;
+    std::vector<WorkItem> items, items_x;
     std::vector<WallItem> walls;
     std::vector<FinishArgs> fin(e->n);
-    std::vector<double> cost;
+    e->prepass = false;
     for (int m = 0; m < e->n; ++m) {
         mhd3d_ctx* c = e->mem[m];
         const Geom& g = c->g;
-        const int flags = (c->d.flag_lMag ? 1 : 0) | (c->d.flag_lRot ? 2 : 0);
+        const int flags = (c->d.flag_lMag ? 1 : 0) | (c->d.flag_lRot ? 2 : 0) | (c->ext ? 8 : 0);
+        const bool ext_interior = ext_variant(c->xa.flags) != EXTV_NONE;
+        if (needs_prepass(c)) e->prepass = true;
         StreamPlan p = stream_plan(g, 256);
         if (e->chunk > 0) {
             const int nint = g.nPz - 2;
@@ -1509,7 +1548,7 @@ static int ens_build(mhd3d_ensemble* e)
                     w.zb = 2 + ch * p.zchunk;
                     w.ze = ch == p.nchunks - 1 ? g.nPz : w.zb + p.zchunk;
                     w.part = part++;
-                    items.push_back(w);
+                    (ext_interior ? items_x : items).push_back(w);
                 }
         for (int top = 1; top >= 0; --top)
             for (int y0 = 0; y0 < g.nPy; y0 += 4)
@@ -1531,19 +1570,25 @@ static int ens_build(mhd3d_ensemble* e)
         fin[m].c = c->c;
     }
     // largest items first (a z chunk of an MHD member costs about twice a hydro one)
-    std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) {
+    auto by_cost = [](const WorkItem& a, const WorkItem& b) {
         const int ca = (a.ze - a.zb) * ((a.flags & 1) ? 2 : 1);
         const int cb = (b.ze - b.zb) * ((b.flags & 1) ? 2 : 1);
         return ca > cb;
-    });
+    };
+    std::stable_sort(items.begin(), items.end(), by_cost);
+    std::stable_sort(items_x.begin(), items_x.end(), by_cost);
     // device tables grow only: a server's ensemble changes membership often
     // and reuses the room it has
     int rc = ens_reserve(e, (void**)&e->d_items, e->cap_items, sizeof(WorkItem), items.size());
+    if (!rc) rc = ens_reserve(e, (void**)&e->d_items_x, e->cap_items_x, sizeof(WorkItem), items_x.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_walls, e->cap_walls, sizeof(WallItem), walls.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_fin, e->cap_fin, sizeof(FinishArgs), fin.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_args, e->cap_args, sizeof(StageArgs), (size_t)6 * e->n);
     if (rc) return rc;
-    ENSCHK(e, hipMemcpy(e->d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
+    if (!items.empty())
+        ENSCHK(e, hipMemcpy(e->d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
+    if (!items_x.empty())
+        ENSCHK(e, hipMemcpy(e->d_items_x, items_x.data(), sizeof(WorkItem) * items_x.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_fin, fin.data(), sizeof(FinishArgs) * fin.size(), hipMemcpyHostToDevice));
     e->off.assign(e->n, 0);
@@ -1551,9 +1596,14 @@ static int ens_build(mhd3d_ensemble* e)
     rc = ens_upload_args(e);
     if (rc) return rc;
     e->nitems = (int)items.size();
+    e->nitems_x = (int)items_x.size();
     e->nwalls = (int)walls.size();
     const int res = ens_resident_blocks(e);
     e->grid_int = e->nitems < res ? e->nitems : res;
+    if (e->nitems_x > 0) {
+        const int rx = ens_resident_blocks(e, true);
+        e->grid_x = e->nitems_x < rx ? e->nitems_x : rx;
+    }
     e->grid_wall = e->nwalls < 2048 ? e->nwalls : 2048;
     return MHD3D_OK;
 }
@@ -1574,10 +1624,8 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
         if (!descs[m]) return bail(MHD3D_EINVAL, "null descriptor");
         if ((descs[m]->precision == MHD3D_MIXED) != e->mixed)
             return bail(MHD3D_EINVAL, "ensemble members must share one precision form");
-        if (ext_flags(descs[m]))
-            return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) +
-                                                ": the off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX) "
-                                                "run on single domains only");
+        if (timedep_flags(descs[m]))
+            return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) + ": " + kTimedepWhy);
         mhd3d_ctx* c = nullptr;
         int rc = mhd3d_create(descs[m], &c);
         if (rc) return bail(rc, "member " + std::to_string(m) + ": " + g_create_error);
@@ -1614,6 +1662,7 @@ extern "C" int mhd3d_ensemble_destroy(mhd3d_ensemble* e)
         for (auto* c : e->mem) mhd3d_destroy(c);
     (void)hipFree(e->mirror);
     (void)hipFree(e->d_items);
+    (void)hipFree(e->d_items_x);
     (void)hipFree(e->d_walls);
     (void)hipFree(e->d_fin);
     (void)hipFree(e->d_args);
@@ -1681,6 +1730,16 @@ static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
 {
     for (int k = 1; k <= 3; ++k) {
         const StageArgs* args = e->d_args + ((size_t)p0 * 3 + (k - 1)) * e->n;
+        if (e->prepass) {
+            // plane means and the relaxation pass of the members whose
+            // fluxes() need them, on the register the stage reads (a
+            // failed launch shows in the caller's hipGetLastError)
+            for (int m = 0; m < e->n; ++m) {
+                mhd3d_ctx* c = e->mem[m];
+                if (needs_prepass(c))
+                    (void)run_prepass(c, c->U[p0 ^ e->off[m] ^ ((k - 1) & 1)], k, e->stream);
+            }
+        }
         if (e->mixed)
             mixed_ens_stage(k, e->waves, e->sched, args, e->d_items, e->nitems, e->grid_int, e->d_walls,
                             e->nwalls, e->grid_wall, e->stream);
@@ -1735,6 +1794,16 @@ static int ens_step_finish(mhd3d_ensemble* e, int nsteps)
         if (e->mixed) continue;
         const int cur0 = e->parity ^ (nsteps & 1) ^ e->off[m];   // the register at the call's start
         c->cur = cur0 ^ (int)(taken & 1);
+        if ((c->xa.flags & EXT_RLAX) && (e->last[m].halted & HALT_EXIT)) {
+            // stopped at the reference's exit point: the state is the input
+            // of the stage that exited (stage 2 reads what stage 1 wrote)
+            RelaxState rs;
+            ENSCHK(e, hipMemcpy(&rs, c->rs, sizeof(RelaxState), hipMemcpyDeviceToHost));
+            // (off[] below keeps the flip for the member's later calls)
+            if (!c->relax_exit_seen && rs.exit_stage == 2) c->cur ^= 1;
+            c->relax_exit_seen = true;
+            c->relax_exit_code = (int)rs.exited;
+        }
         const int off = e->parity ^ c->cur;
         if (off != e->off[m]) {
             e->off[m] = off;
@@ -1742,6 +1811,8 @@ static int ens_step_finish(mhd3d_ensemble* e, int nsteps)
         }
     }
     if (refresh) {
+        // captured prepass launches name the registers they read
+        if (e->prepass) ens_drop_graphs(e);
         int rc = ens_upload_args(e);
         if (rc) return rc;
     }
@@ -1754,6 +1825,9 @@ extern "C" int mhd3d_ensemble_step(mhd3d_ensemble* e, int nsteps)
     int rc = ens_step_enqueue(e, nsteps);
     if (!rc) rc = ens_step_finish(e, nsteps);
     if (rc) return rc;
+    for (int m = 0; m < e->n; ++m)
+        if (e->last[m].halted & HALT_EXIT)
+            return ens_fail(e, MHD3D_EEXIT, "member " + std::to_string(m) + ": " + relax_exit_text(e->mem[m]));
     for (int m = 0; m < e->n; ++m)
         if (e->last[m].nonfinite)
             return ens_fail(e, MHD3D_ENONFINITE, "member " + std::to_string(m) + ": state is not finite after the step");
@@ -2079,26 +2153,30 @@ extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, cons
     int rc = validate(desc, why);
     if (rc) { s->err = why; return rc; }
     if (desc->precision != s->precision) { s->err = "request precision differs from the server's"; return MHD3D_EINVAL; }
-    if (ext_flags(desc)) {
-        s->err = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX) run on single domains only";
+    if (timedep_flags(desc)) {
+        s->err = kTimedepWhy;
         return MHD3D_EUNSUPPORTED;
     }
     ServerReq r;
     r.id = s->next_id++;
     r.d = *desc;
-    r.d.EXX = r.d.WYY = r.d.ZEE = nullptr;   // not read without the branches above
+    // the descriptor's arrays are copied: the run is created at admission
     const int nx = desc->nPx + 2, ny = desc->nPy + 2, nz = desc->nPz + 2;
-    const double* src[11] = {desc->dxxdx, desc->d2xxdx2, desc->DDX, desc->dyydy, desc->d2yydy2, desc->DDY,
-                             desc->dzzdz, desc->d2zzdz2, desc->DDZ, desc->RKAPA, desc->DKAPA};
-    const int len[11] = {nx, nx, nx, ny, ny, ny, nz, nz, nz, nz, nz};
+    constexpr int NA = 14;
+    const double* src[NA] = {desc->dxxdx, desc->d2xxdx2, desc->DDX, desc->dyydy, desc->d2yydy2, desc->DDY,
+                             desc->dzzdz, desc->d2zzdz2, desc->DDZ, desc->RKAPA, desc->DKAPA,
+                             desc->EXX, desc->WYY, desc->ZEE};
+    const int len[NA] = {nx, nx, nx, ny, ny, ny, nz, nz, nz, nz, nz, nx, ny, nz};
     size_t tot = 0;
-    for (int k = 0; k < 11; ++k) tot += len[k];
+    for (int k = 0; k < NA; ++k) tot += src[k] ? len[k] : 0;
     r.met.resize(tot);
     size_t o = 0;
-    const double** dst[11] = {&r.d.dxxdx, &r.d.d2xxdx2, &r.d.DDX, &r.d.dyydy, &r.d.d2yydy2, &r.d.DDY,
-                              &r.d.dzzdz, &r.d.d2zzdz2, &r.d.DDZ, &r.d.RKAPA, &r.d.DKAPA};
-    std::vector<size_t> offs(11);
-    for (int k = 0; k < 11; ++k) {
+    const double** dst[NA] = {&r.d.dxxdx, &r.d.d2xxdx2, &r.d.DDX, &r.d.dyydy, &r.d.d2yydy2, &r.d.DDY,
+                              &r.d.dzzdz, &r.d.d2zzdz2, &r.d.DDZ, &r.d.RKAPA, &r.d.DKAPA,
+                              &r.d.EXX, &r.d.WYY, &r.d.ZEE};
+    std::vector<size_t> offs(NA, SIZE_MAX);
+    for (int k = 0; k < NA; ++k) {
+        if (!src[k]) continue;   // an optional coordinate array the run does not read
         std::memcpy(&r.met[o], src[k], sizeof(double) * len[k]);
         offs[k] = o;
         o += len[k];
@@ -2115,12 +2193,12 @@ extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, cons
     r.t_submit = srv_now(s);
     auto& slot = s->reqs[r.id];
     slot = std::move(r);
-    for (int k = 0; k < 11; ++k) *dst[k] = nullptr;
-    // point the stored descriptor at the stored metric copies
-    const double** sd[11] = {&slot.d.dxxdx, &slot.d.d2xxdx2, &slot.d.DDX, &slot.d.dyydy, &slot.d.d2yydy2,
+    for (int k = 0; k < NA; ++k) *dst[k] = nullptr;
+    // point the stored descriptor at the stored copies
+    const double** sd[NA] = {&slot.d.dxxdx, &slot.d.d2xxdx2, &slot.d.DDX, &slot.d.dyydy, &slot.d.d2yydy2,
                              &slot.d.DDY, &slot.d.dzzdz, &slot.d.d2zzdz2, &slot.d.DDZ, &slot.d.RKAPA,
-                             &slot.d.DKAPA};
-    for (int k = 0; k < 11; ++k) *sd[k] = slot.met.data() + offs[k];
+                             &slot.d.DKAPA, &slot.d.EXX, &slot.d.WYY, &slot.d.ZEE};
+    for (int k = 0; k < NA; ++k) *sd[k] = offs[k] == SIZE_MAX ? nullptr : slot.met.data() + offs[k];
     s->queue.push_back(slot.id);
     *id = slot.id;
     return MHD3D_OK;
@@ -2210,7 +2288,10 @@ extern "C" int mhd3d_server_tick(mhd3d_server* s, int* n_active, int* n_queued)
             const int now_done = (int)hs.nit;
             s->cell_steps += (int64_t)(now_done - r.done) * r.ctx->g.nPx * r.ctx->g.nPy * r.ctx->g.nPz;
             r.done = now_done;
-            if (hs.halted & HALT_GUARD) {   // the reference's driver halts such a run (3dmhd.cpp:292-330)
+            if (hs.halted & HALT_EXIT) {    // the reference's program exits there (RLAX, 3dmhdsub.h:344,416,427-439)
+                r.status = MHD3D_REQ_EXITED;
+                r.t_finish = now;
+            } else if (hs.halted & HALT_GUARD) {   // the reference's driver halts such a run (3dmhd.cpp:292-330)
                 r.status = MHD3D_REQ_FAILED;
                 r.t_finish = now;
             } else if (hs.nonfinite) {      // evicted: its state holds NaN/Inf
@@ -2262,7 +2343,8 @@ extern "C" int mhd3d_server_result(mhd3d_server* s, int64_t id, double* const* f
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) { s->err = "unknown request"; return MHD3D_EINVAL; }
     ServerReq& r = it->second;
-    if (r.status != MHD3D_REQ_DONE && r.status != MHD3D_REQ_FAILED && r.status != MHD3D_REQ_NONFINITE) {
+    if (r.status != MHD3D_REQ_DONE && r.status != MHD3D_REQ_FAILED && r.status != MHD3D_REQ_NONFINITE &&
+        r.status != MHD3D_REQ_EXITED) {
         s->err = "request not finished";
         return MHD3D_EINVAL;
     }

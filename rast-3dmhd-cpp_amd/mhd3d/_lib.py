@@ -85,7 +85,7 @@ SIGNATURES = {
     "mhd3d_server_result": (C.c_int, [_vp, C.c_int64, _dpp]),
 }
 
-REQ_NONFINITE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_RELEASED = -2, -1, 0, 1, 2, 3
+REQ_EXITED, REQ_NONFINITE, REQ_FAILED, REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_RELEASED = -3, -2, -1, 0, 1, 2, 3
 
 
 class RequestInfo(C.Structure):

@@ -111,6 +111,7 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
     results = {}
     failed = []
     evicted = []
+    exited = []
     peak_bytes = 0
     while len(lat) < len(specs):
         now = srv.now() - t0
@@ -126,7 +127,7 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
         peak_bytes = max(peak_bytes, srv.bytes_in_use)
         for rid in list(open_ids):
             info = srv.poll(rid)
-            if info["status"] in (L.REQ_DONE, L.REQ_FAILED, L.REQ_NONFINITE):
+            if info["status"] in (L.REQ_DONE, L.REQ_FAILED, L.REQ_NONFINITE, L.REQ_EXITED):
                 i = ids[rid]
                 lat[i] = info["t_finish"] - t0 - specs[i].arrival
                 results[i] = srv.result(rid, fetch=fetch and info["status"] == L.REQ_DONE)
@@ -135,6 +136,8 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
                     failed.append(i)
                 elif info["status"] == L.REQ_NONFINITE:
                     evicted.append(i)
+                elif info["status"] == L.REQ_EXITED:
+                    exited.append(i)
         if na == 0 and nq == 0 and nxt < len(order):
             wait = specs[order[nxt]].arrival - (srv.now() - t0)
             if wait > 0:
@@ -147,7 +150,8 @@ def serve_trace(specs, precision=0, hbm_budget_bytes=64 << 30, quantum_steps=10,
     return {"p50_s": _percentile(v, 50), "p99_s": _percentile(v, 99), "max_s": max(v),
             "mcell_steps_per_s": cell_steps / (t_end - t_begin) / 1e6, "makespan_s": t_end - t_begin,
             "cell_steps": cell_steps, "peak_device_bytes": peak_bytes, "failed": sorted(failed),
-            "evicted_nonfinite": sorted(evicted), "latencies": v, "results": results}
+            "evicted_nonfinite": sorted(evicted), "exited": sorted(exited), "latencies": v,
+            "results": results}
 
 
 def sequential_trace(specs, precision=0):

@@ -189,9 +189,9 @@ def test_fixed_dt_odd_sizes_lrem_diffusion():
     _same(dom, orc, "lRem+iD fixed dt")
 
 
-def test_ensemble_refuses_branches():
+def test_ensemble_refuses_time_dependent_branches():
     import ctypes as C
-    desc = Descriptor(_cfg((16, 12, 8), "lShr"))
+    desc = Descriptor(_cfg((16, 12, 8), "heatT"))
     arr = (C.c_void_p * 1)(C.cast(desc.ptr(), C.c_void_p))
     h = C.c_void_p()
     rc = L.lib().mhd3d_ensemble_create(arr, 1, C.byref(h))
