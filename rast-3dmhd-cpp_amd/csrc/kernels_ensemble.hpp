@@ -25,9 +25,8 @@
 namespace mhd {
 
 // One interior work item: tile (tile_x, tile_y) of member `member` over the
-// planes zb .. ze-1; flags bit 0 = lMag, bit 1 = lRot, bit 3 = the member has
-// off-default branches (ExtArgs); `part` = index of its TSTEP partial record
-// in the member's array.
+// planes zb .. ze-1; flags bit 0 = lMag, bit 1 = lRot; `part` = index of its
+// TSTEP partial record in the member's array.
 struct WorkItem {
     int member, flags, tile_x, tile_y, zb, ze, part, pad;
 };
@@ -126,10 +125,10 @@ __global__ void __launch_bounds__(256, 1)
     }
 }
 
-// Persistent wall stage over all members (blocks of 64 x 4 threads); a
-// member with off-default branches (flags bit 3, FP64 only) takes the wall
-// code that evaluates them.
-template <typename T, int STAGE>
+// Persistent wall stage over the members (blocks of 64 x 4 threads).  EXT:
+// the members with off-default branches (FP64 only), in a list and a launch
+// of their own so the others keep the smaller register budget.
+template <typename T, int STAGE, bool EXT = false>
 __global__ void __launch_bounds__(256)
     ensemble_wall_kernel(const StageArgs* __restrict__ args, const WallItem* __restrict__ items,
                          int nitems)
@@ -142,21 +141,11 @@ __global__ void __launch_bounds__(256)
         CflAcc cfl;
         if (x < a.g.nPx && y < a.g.nPy) {
             const bool top = w.top != 0;
-            constexpr bool F64 = std::is_same<T, double>::value;
-            if (F64 && (w.flags & 8)) {
-                switch (w.flags & 3) {
-                case 0: wall_point<T, false, false, STAGE, F64>(a, x, y, top, cfl); break;
-                case 1: wall_point<T, true, false, STAGE, F64>(a, x, y, top, cfl); break;
-                case 2: wall_point<T, false, true, STAGE, F64>(a, x, y, top, cfl); break;
-                default: wall_point<T, true, true, STAGE, F64>(a, x, y, top, cfl); break;
-                }
-            } else {
-                switch (w.flags & 3) {
-                case 0: wall_point<T, false, false, STAGE>(a, x, y, top, cfl); break;
-                case 1: wall_point<T, true, false, STAGE>(a, x, y, top, cfl); break;
-                case 2: wall_point<T, false, true, STAGE>(a, x, y, top, cfl); break;
-                default: wall_point<T, true, true, STAGE>(a, x, y, top, cfl); break;
-                }
+            switch (w.flags & 3) {
+            case 0: wall_point<T, false, false, STAGE, EXT>(a, x, y, top, cfl); break;
+            case 1: wall_point<T, true, false, STAGE, EXT>(a, x, y, top, cfl); break;
+            case 2: wall_point<T, false, true, STAGE, EXT>(a, x, y, top, cfl); break;
+            default: wall_point<T, true, true, STAGE, EXT>(a, x, y, top, cfl); break;
             }
         }
         if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, w.part);

@@ -279,6 +279,15 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     else
         U = a.Uin32;
     const double* __restrict__ zsrc = (STAGE == 1) ? a.Uin : a.Z;
+    // the Z operand of stages 2-3 is read once per point and by no other
+    // tile: a nontemporal load keeps its lines from displacing the state's
+    // halo lines in L2 (stage 1 reads it from the state itself)
+    auto zld = [&](int64_t idx) -> double {
+        if constexpr (STAGE > 1)
+            return __builtin_nontemporal_load(zsrc + idx);
+        else
+            return zsrc[idx];
+    };
     const int64_t fs = g.fs;
     SAcc<T, MAG, RO_N> acc;
     T row[3] = {T(0), T(0), T(0)};
@@ -301,7 +310,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     for (int f = 0; f < NF; ++f) zpre[f] = zcur[f] = 0.0;
     if (ZLOAD && !DEFER) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)zb * g.plane + go];
+        for (int f = 0; f < NF; ++f) zpre[f] = zld(f * fs + (int64_t)zb * g.plane + go);
     }
 
     // The column's x and y metric terms (the same on every plane) are loaded
@@ -454,7 +463,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             // previous one): a copy of a value still in flight would wait for
             // every memory operation issued before its load
 #pragma unroll
-            for (int f = 0; f < NF; ++f) zpend[f] = zsrc[f * fs + (int64_t)z * g.plane + go];
+            for (int f = 0; f < NF; ++f) zpend[f] = zld(f * fs + (int64_t)z * g.plane + go);
         }
         if (!DEFER && ZLOAD && z >= zb) {
             // this iteration updates plane z; fetch the operand of plane z+1
@@ -462,7 +471,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             for (int f = 0; f < NF; ++f) zcur[f] = zpre[f];
             if (z + 1 < ze) {
 #pragma unroll
-                for (int f = 0; f < NF; ++f) zpre[f] = zsrc[f * fs + (int64_t)(z + 1) * g.plane + go];
+                for (int f = 0; f < NF; ++f) zpre[f] = zld(f * fs + (int64_t)(z + 1) * g.plane + go);
             }
         }
         if (z >= zb) {
@@ -529,7 +538,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
                 if constexpr (!ZEARLY) {
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
-                        zpend[f] = ZLOAD ? zsrc[f * fs + (int64_t)z * g.plane + go] : uc[f];
+                        zpend[f] = ZLOAD ? zld(f * fs + (int64_t)z * g.plane + go) : uc[f];
                 }
             } else {
                 update(z, o, ZLOAD ? zcur : uc);

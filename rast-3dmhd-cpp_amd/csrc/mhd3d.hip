@@ -460,6 +460,16 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
         else
             fn = desc->flag_lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
         ctx->target_blocks = resident_blocks(fn, STX * STY);
+        if (!ctx->mixed) {
+            // FP64: z chunks of at most 64 planes, in rounds of the resident
+            // blocks (MI355X, FP64 MHD 512x512x256 stage kernel: one chunk
+            // 3.47 ms, two 3.40, four 3.35, eight 3.74; 256x256x128 is best
+            // at its resident count, two chunks of 63); at most 8 rounds,
+            // which bounds the TSTEP partial records
+            const StreamPlan one = stream_plan(g, 1);
+            const int want = one.ntx * one.nty * ((g.nPz - 2 + 63) / 64);
+            ctx->target_blocks = std::max(ctx->target_blocks, std::min(want, 8 * ctx->target_blocks));
+        }
     }
     *out = ctx;
     return MHD3D_OK;
@@ -1367,14 +1377,15 @@ struct mhd3d_ensemble {
     WorkItem* d_items = nullptr;
     WorkItem* d_items_x = nullptr;     // interior items of members with off-default branches
     WallItem* d_walls = nullptr;
+    WallItem* d_walls_x = nullptr;     // wall items of members with off-default branches
     FinishArgs* d_fin = nullptr;
-    int nitems = 0, nwalls = 0, nitems_x = 0;
-    int grid_int = 0, grid_wall = 0, grid_x = 0;
+    int nitems = 0, nwalls = 0, nitems_x = 0, nwalls_x = 0;
+    int grid_int = 0, grid_wall = 0, grid_x = 0, grid_wall_x = 0;
     bool prepass = false;              // some member needs plane means / the relaxation pass
     int waves = 1, sched = 0, chunk = 0;   // kernel variant; chunk: planes per item (0 = auto)
     int parity = 0;                    // FP64: register parity of the step tables
     std::vector<int> off;              // member m's state is U[parity ^ off[m]] (FP64)
-    size_t cap_items = 0, cap_walls = 0, cap_fin = 0, cap_args = 0, cap_items_x = 0;   // device table capacities
+    size_t cap_items = 0, cap_walls = 0, cap_fin = 0, cap_args = 0, cap_items_x = 0, cap_walls_x = 0;   // device table capacities
     bool use_graph = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int64_t cells = 0;
@@ -1442,9 +1453,17 @@ static void ens_stage(const mhd3d_ensemble* e, int k, const StageArgs* args)
         else ensemble_stage_ext_kernel<3><<<e->grid_x, blk, 0, e->stream>>>(args, e->d_items_x, e->nitems_x);
     }
     dim3 wblk(64, 4, 1);
-    if (k == 1) ensemble_wall_kernel<T, 1><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
-    else if (k == 2) ensemble_wall_kernel<T, 2><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
-    else ensemble_wall_kernel<T, 3><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+    if (e->nwalls > 0) {
+        if (k == 1) ensemble_wall_kernel<T, 1><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+        else if (k == 2) ensemble_wall_kernel<T, 2><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+        else ensemble_wall_kernel<T, 3><<<e->grid_wall, wblk, 0, e->stream>>>(args, e->d_walls, e->nwalls);
+    }
+    if (e->nwalls_x > 0) {
+        const WallItem* wx = e->d_walls_x;
+        if (k == 1) ensemble_wall_kernel<double, 1, true><<<e->grid_wall_x, wblk, 0, e->stream>>>(args, wx, e->nwalls_x);
+        else if (k == 2) ensemble_wall_kernel<double, 2, true><<<e->grid_wall_x, wblk, 0, e->stream>>>(args, wx, e->nwalls_x);
+        else ensemble_wall_kernel<double, 3, true><<<e->grid_wall_x, wblk, 0, e->stream>>>(args, wx, e->nwalls_x);
+    }
 }
 
 // Resident blocks of an interior kernel on the whole device: the chosen
@@ -1524,13 +1543,13 @@ static int ens_build(mhd3d_ensemble* e)
         e->mirror_cap = e->n;
     }
     std::vector<WorkItem> items, items_x;
-    std::vector<WallItem> walls;
+    std::vector<WallItem> walls, walls_x;
     std::vector<FinishArgs> fin(e->n);
     e->prepass = false;
     for (int m = 0; m < e->n; ++m) {
         mhd3d_ctx* c = e->mem[m];
         const Geom& g = c->g;
-        const int flags = (c->d.flag_lMag ? 1 : 0) | (c->d.flag_lRot ? 2 : 0) | (c->ext ? 8 : 0);
+        const int flags = (c->d.flag_lMag ? 1 : 0) | (c->d.flag_lRot ? 2 : 0);
         const bool ext_interior = ext_variant(c->xa.flags) != EXTV_NONE;
         if (needs_prepass(c)) e->prepass = true;
         StreamPlan p = stream_plan(g, 256);
@@ -1556,7 +1575,7 @@ static int ens_build(mhd3d_ensemble* e)
                     WallItem w{};
                     w.member = m; w.flags = flags; w.x0 = x0; w.y0 = y0; w.top = top;
                     w.part = part++;
-                    walls.push_back(w);
+                    (c->ext ? walls_x : walls).push_back(w);
                 }
         // room for the fused partials and for the standalone TSTEP pass
         const int nsa = ((g.nPx + 63) / 64) * ((g.nPy + 3) / 4) * (g.nPz < 8 ? g.nPz : 8);
@@ -1582,6 +1601,7 @@ static int ens_build(mhd3d_ensemble* e)
     int rc = ens_reserve(e, (void**)&e->d_items, e->cap_items, sizeof(WorkItem), items.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_items_x, e->cap_items_x, sizeof(WorkItem), items_x.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_walls, e->cap_walls, sizeof(WallItem), walls.size());
+    if (!rc) rc = ens_reserve(e, (void**)&e->d_walls_x, e->cap_walls_x, sizeof(WallItem), walls_x.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_fin, e->cap_fin, sizeof(FinishArgs), fin.size());
     if (!rc) rc = ens_reserve(e, (void**)&e->d_args, e->cap_args, sizeof(StageArgs), (size_t)6 * e->n);
     if (rc) return rc;
@@ -1589,7 +1609,10 @@ static int ens_build(mhd3d_ensemble* e)
         ENSCHK(e, hipMemcpy(e->d_items, items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice));
     if (!items_x.empty())
         ENSCHK(e, hipMemcpy(e->d_items_x, items_x.data(), sizeof(WorkItem) * items_x.size(), hipMemcpyHostToDevice));
-    ENSCHK(e, hipMemcpy(e->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice));
+    if (!walls.empty())
+        ENSCHK(e, hipMemcpy(e->d_walls, walls.data(), sizeof(WallItem) * walls.size(), hipMemcpyHostToDevice));
+    if (!walls_x.empty())
+        ENSCHK(e, hipMemcpy(e->d_walls_x, walls_x.data(), sizeof(WallItem) * walls_x.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_fin, fin.data(), sizeof(FinishArgs) * fin.size(), hipMemcpyHostToDevice));
     e->off.assign(e->n, 0);
     for (int m = 0; m < e->n; ++m) e->off[m] = e->mixed ? 0 : (e->parity ^ e->mem[m]->cur);
@@ -1598,6 +1621,8 @@ static int ens_build(mhd3d_ensemble* e)
     e->nitems = (int)items.size();
     e->nitems_x = (int)items_x.size();
     e->nwalls = (int)walls.size();
+    e->nwalls_x = (int)walls_x.size();
+    e->grid_wall_x = e->nwalls_x < 2048 ? e->nwalls_x : 2048;
     const int res = ens_resident_blocks(e);
     e->grid_int = e->nitems < res ? e->nitems : res;
     if (e->nitems_x > 0) {
@@ -1664,6 +1689,7 @@ extern "C" int mhd3d_ensemble_destroy(mhd3d_ensemble* e)
     (void)hipFree(e->d_items);
     (void)hipFree(e->d_items_x);
     (void)hipFree(e->d_walls);
+    (void)hipFree(e->d_walls_x);
     (void)hipFree(e->d_fin);
     (void)hipFree(e->d_args);
     if (e->stream) (void)hipStreamDestroy(e->stream);
