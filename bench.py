@@ -447,6 +447,22 @@ def main():
                      "vs_batched": round(el / el1, 4),
                      "note": f"{args.steps} calls of mhd3d_step(ctx, 1, 0, &DT)"}
     log(f"drop-in: {out['dropin']}")
+    # the single-domain autotuner over the streaming variants (register budget
+    # x schedule x launch size); the headline above runs the default variant,
+    # which the traffic profile describes
+    w0, s0, _ = dom.variant()
+    trials = dom.autotune(steps=2)
+    ok = [t for t in trials if t[3] > 0]
+    best = min(ok, key=lambda t: t[3]) if ok else None
+    dflt = [t for t in ok if (t[0], t[1], t[2]) == (w0, s0, 64.0)]
+    out["autotune"] = {
+        "candidates": len(trials),
+        "best": None if best is None else {"waves": int(best[0]), "sched": int(best[1]),
+                                           "planes_per_chunk": int(best[2]), "ms_per_step": round(best[3], 4)},
+        "default_ms_per_step": round(dflt[0][3], 4) if dflt else None,
+        "note": "mhd3d_autotune, 2 timed steps per candidate on the headline state; planes_per_chunk 0 = "
+                "one round of the resident blocks"}
+    log(f"autotune: {out['autotune']}")
     # the drop-in boundary hands over host arrays: time one upload and one
     # download of the whole state (host layout, ghosts included)
     if state is not None:

@@ -176,6 +176,20 @@ int mhd3d_step(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout);
  * array: calls that overlap in time (other streams or threads) race. */
 int mhd3d_step_noalloc(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout);
 
+/* Autotuner of a single domain (no counterpart in the reference): times
+ * `steps` CFL steps of each streaming-kernel variant -- register budget
+ * (1 or 2 waves per SIMD) x schedule (4) x launch size (one round of the
+ * resident blocks, or z chunks of at most 64 or 32 planes): 24 candidates --
+ * on the current state, restores the state, the clock and the TSTEP after
+ * each trial and keeps the fastest.  Results do not depend on the variant.
+ * report: NULL or room for 4*24 doubles (waves, sched, planes per chunk
+ * (0 = one round), ms per step; -1 for a trial the DT guard stopped);
+ * *ncand = trials.  MHD3D_EUNSUPPORTED for the point kernel and for the
+ * off-default branches (their state between steps, one variant). */
+int mhd3d_autotune(mhd3d_ctx* ctx, int steps, double* report, int* ncand);
+/* The variant in use: waves per SIMD, schedule bits, blocks per launch. */
+int mhd3d_get_variant(mhd3d_ctx* ctx, int* waves, int* sched, int* blocks);
+
 /* DT guard of the reference's driver (3dmhd.cpp:292-330): after any step
  * with DT < dt_min the run halts -- later steps do nothing until the next
  * set_state.  dt_min = 0 (the default) disables it; MHD3D_REFERENCE_DT_MIN is

@@ -334,6 +334,32 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
     return MHD3D_OK;
 }
 
+// Resident blocks of the streaming stage kernel a context runs with the
+// register budget `w` (waves per SIMD) and schedule `sc`, on the whole device.
+static int variant_resident(const mhd3d_ctx* ctx, int w, int sc)
+{
+    const bool mag = ctx->d.flag_lMag != 0;
+    const void* fn;
+    if (ctx->mixed)
+        fn = mixed_stream_fn(mag, w, sc);
+    else if (ext_variant(ctx->xa.flags) != EXTV_NONE)
+        fn = ext_stream_fn(mag, ext_variant(ctx->xa.flags));
+    else
+        fn = mag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
+    return resident_blocks(fn, STX * STY);
+}
+
+// Blocks of one streaming launch: the resident blocks, or (planes > 0) z
+// chunks of at most `planes` planes in rounds of them -- at most 8 rounds,
+// which bounds the TSTEP partial records.
+static int launch_blocks(const mhd3d_ctx* ctx, int resident, int planes)
+{
+    if (planes <= 0) return resident;
+    const StreamPlan one = stream_plan(ctx->g, 1);
+    const int want = one.ntx * one.nty * ((ctx->g.nPz - 2 + planes - 1) / planes);
+    return std::max(resident, std::min(want, 8 * resident));
+}
+
 extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
 {
     if (!out) return MHD3D_EINVAL;
@@ -451,25 +477,12 @@ extern "C" int mhd3d_create(const mhd3d_desc* desc, mhd3d_ctx** out)
     if (ctx->mixed) ctx->kernel_mode = 0;   // the mixed form has only the streaming kernel
     if (ctx->target_blocks == 0) {
         const int fm = ctx->mixed ? 1 : 0, ph = desc->flag_lMag ? 1 : 0;
-        const int w = ctx->tune_waves[fm][ph], sc = ctx->tune_sched[fm][ph];
-        const void* fn;
-        if (ctx->mixed)
-            fn = mixed_stream_fn(desc->flag_lMag, w, sc);
-        else if (ext_variant(ctx->xa.flags) != EXTV_NONE)
-            fn = ext_stream_fn(desc->flag_lMag, ext_variant(ctx->xa.flags));
-        else
-            fn = desc->flag_lMag ? stream_fn<double, true, true>(w, sc) : stream_fn<double, false, true>(w, sc);
-        ctx->target_blocks = resident_blocks(fn, STX * STY);
-        if (!ctx->mixed) {
-            // FP64: z chunks of at most 64 planes, in rounds of the resident
-            // blocks (MI355X, FP64 MHD 512x512x256 stage kernel: one chunk
-            // 3.47 ms, two 3.40, four 3.35, eight 3.74; 256x256x128 is best
-            // at its resident count, two chunks of 63); at most 8 rounds,
-            // which bounds the TSTEP partial records
-            const StreamPlan one = stream_plan(g, 1);
-            const int want = one.ntx * one.nty * ((g.nPz - 2 + 63) / 64);
-            ctx->target_blocks = std::max(ctx->target_blocks, std::min(want, 8 * ctx->target_blocks));
-        }
+        // FP64: z chunks of at most 64 planes (MI355X, FP64 MHD 512x512x256
+        // stage kernel: one chunk 3.47 ms, two 3.40, four 3.35, eight 3.74;
+        // 256x256x128 is best at its resident count, two chunks of 63); the
+        // mixed form: one round of its resident blocks (its best)
+        ctx->target_blocks = launch_blocks(ctx, variant_resident(ctx, ctx->tune_waves[fm][ph], ctx->tune_sched[fm][ph]),
+                                           ctx->mixed ? 0 : 64);
     }
     *out = ctx;
     return MHD3D_OK;
@@ -1265,6 +1278,112 @@ extern "C" int mhd3d_cfl(mhd3d_ctx* ctx, mhd3d_cfl_info* out)
     for (int k = 0; k < 4; ++k) out->wmin[k] = hs.wmin[k];
     out->umach = hs.umach;
     out->dt = hs.dt;
+    return MHD3D_OK;
+}
+
+// Autotuner of a single domain: times `steps` CFL steps of every streaming
+// variant (register budget x schedule x launch size) on the current state,
+// restoring the state, the clock and the TSTEP of the state after each trial,
+// and keeps the fastest.  The results do not depend on the variant.
+extern "C" int mhd3d_autotune(mhd3d_ctx* ctx, int steps, double* report, int* ncand_out)
+{
+    if (!ctx || steps < 1) return MHD3D_EINVAL;
+    if (ctx->kernel_mode != 0)
+        return fail(ctx, MHD3D_EUNSUPPORTED, "mhd3d_autotune: the point kernel has no variants");
+    if (ctx->ext)
+        return fail(ctx, MHD3D_EUNSUPPORTED,
+                    "mhd3d_autotune: the off-default branches keep state between steps (lShr's FT, RLAX) "
+                    "and run one variant");
+    if (ctx->timing) return fail(ctx, MHD3D_EINVAL, "mhd3d_autotune: disable kernel timing first");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t reg = (size_t)ctx->g.fs * ctx->nf * sizeof(double);
+    double* snap = nullptr;
+    DevScalars* ds_snap = nullptr;
+    if (hipMalloc(&snap, reg) != hipSuccess || hipMalloc(&ds_snap, sizeof(DevScalars)) != hipSuccess) {
+        (void)hipFree(snap);
+        return fail(ctx, MHD3D_ENOMEM, "mhd3d_autotune: snapshot");
+    }
+    const int cur_save = ctx->cur;
+    const bool ready_save = ctx->dt_ready;
+    const int fm = ctx->mixed ? 1 : 0, ph = ctx->d.flag_lMag ? 1 : 0;
+    int best_w = ctx->tune_waves[fm][ph], best_s = ctx->tune_sched[fm][ph], best_t = ctx->target_blocks;
+    int rc = MHD3D_OK;
+    if (hipMemcpyAsync(snap, ctx->U[ctx->cur], reg, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(ds_snap, ctx->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+        rc = fail(ctx, MHD3D_EHIP, "mhd3d_autotune: snapshot copy");
+    auto restore = [&]() -> int {
+        ctx->cur = cur_save;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->U[ctx->cur], snap, reg, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->ds, ds_snap, sizeof(DevScalars), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->dt_ready = ready_save;
+        return MHD3D_OK;
+    };
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    const int wv[2] = {1, 2};
+    const int planes[3] = {0, 64, 32};
+    double best = 1e300;
+    int ncand = 0;
+    for (int iw = 0; iw < 2 && !rc; ++iw)
+        for (int is = 0; is < 4 && !rc; ++is)
+            for (int ip = 0; ip < 3 && !rc; ++ip) {
+                ctx->tune_waves[fm][ph] = wv[iw];
+                ctx->tune_sched[fm][ph] = is;
+                ctx->target_blocks = launch_blocks(ctx, variant_resident(ctx, wv[iw], is), planes[ip]);
+                ctx->dt_ready = false;   // the partial records follow the launch size
+                double dt = 0.0;
+                int r = step_core(ctx, 1, 0, &dt);   // warm-up
+                double per = -1.0;
+                if (r == MHD3D_OK) {
+                    (void)hipEventRecord(t0, ctx->stream);
+                    r = step_core(ctx, steps, 0, &dt);
+                    (void)hipEventRecord(t1, ctx->stream);
+                    (void)hipEventSynchronize(t1);
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, t0, t1);
+                    if (r == MHD3D_OK) per = ms / steps;
+                }
+                // a trial the DT guard stops or whose state turns non-finite
+                // does not count; any other failure ends the search
+                if (r != MHD3D_OK && r != MHD3D_EHALTED && r != MHD3D_ENONFINITE) rc = r;
+                if (report) {
+                    report[4 * ncand + 0] = wv[iw];
+                    report[4 * ncand + 1] = is;
+                    report[4 * ncand + 2] = planes[ip];
+                    report[4 * ncand + 3] = per;
+                }
+                ++ncand;
+                if (per > 0.0 && per < best) {
+                    best = per;
+                    best_w = wv[iw];
+                    best_s = is;
+                    best_t = ctx->target_blocks;
+                }
+                const int rr = restore();
+                if (!rc) rc = rr;
+            }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    ctx->tune_waves[fm][ph] = best_w;
+    ctx->tune_sched[fm][ph] = best_s;
+    ctx->target_blocks = best_t;
+    ctx->dt_ready = false;
+    (void)hipFree(snap);
+    (void)hipFree(ds_snap);
+    if (ncand_out) *ncand_out = ncand;
+    return rc;
+}
+
+// The variant a context runs: waves per SIMD, schedule bits, blocks per launch.
+extern "C" int mhd3d_get_variant(mhd3d_ctx* ctx, int* waves, int* sched, int* blocks)
+{
+    if (!ctx) return MHD3D_EINVAL;
+    const int fm = ctx->mixed ? 1 : 0, ph = ctx->d.flag_lMag ? 1 : 0;
+    if (waves) *waves = ctx->tune_waves[fm][ph];
+    if (sched) *sched = ctx->tune_sched[fm][ph];
+    if (blocks) *blocks = ctx->target_blocks;
     return MHD3D_OK;
 }
 

@@ -38,6 +38,8 @@ SIGNATURES = {
     "mhd3d_diagnostics": (C.c_int, [_vp, _dp]),
     "mhd3d_clock": (C.c_int, [_vp, C.POINTER(C.c_int64), _dp, _dp]),
     "mhd3d_relax_state": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "mhd3d_autotune": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int)]),
+    "mhd3d_get_variant": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mhd3d_timing_enable": (C.c_int, [_vp, C.c_int]),
     "mhd3d_timing_read": (C.c_int, [_vp, _vp]),
     "mhd3d_synchronize": (C.c_int, [_vp]),

@@ -111,6 +111,20 @@ class Domain:
         self._check(L.lib().mhd3d_clock(self.h, C.byref(n), C.byref(t), C.byref(tc)))
         return n.value, t.value, tc.value
 
+    def autotune(self, steps=2):
+        """Pick the fastest streaming variant on the current state (state and
+        clock unchanged).  Returns the trials as (waves, sched, planes, ms)."""
+        rep = (C.c_double * (4 * 24))()
+        n = C.c_int()
+        self._check(L.lib().mhd3d_autotune(self.h, int(steps), rep, C.byref(n)))
+        return [tuple(rep[4 * i: 4 * i + 4]) for i in range(n.value)]
+
+    def variant(self):
+        """(waves per SIMD, schedule bits, blocks per launch) in use."""
+        w, s, b = C.c_int(), C.c_int(), C.c_int()
+        self._check(L.lib().mhd3d_get_variant(self.h, C.byref(w), C.byref(s), C.byref(b)))
+        return w.value, s.value, b.value
+
     def relax_state(self):
         """The RLAX statics (3dmhdsub.h:195-198): dict TFAC, TTM_MAX, TTM_MIN,
         ICALL and exited (the reference's exit point reached)."""
