@@ -41,4 +41,10 @@ const void* ext_stream_fn(bool mag, int variant);
 void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, hipStream_t s,
                int target_blocks);
 
+// The mixed form with off-default branches: every branch compiled in, FP64
+// fluxes from the FP32 copies of the stage values (stage 1 from the FP64
+// state), the copies written as in the mixed form.
+const void* ext_stream_fn_s32(bool mag);
+void ext_stage_s32(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s, int target_blocks);
+
 }  // namespace mhd

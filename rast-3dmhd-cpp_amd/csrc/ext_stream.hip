@@ -53,7 +53,39 @@ const void* fn_mask(bool mag)
                : (const void*)stage_stream_kernel<double, false, true, 2, 1, true, true, true, M>;
 }
 
+// The mixed form with branches: FP64 fluxes with every branch from the FP32
+// copies of the stage values (S32).
+template <bool MAG, bool ROT, int STAGE>
+void launch_s32(const StageArgs& a, hipStream_t s, int target_blocks)
+{
+    const StreamPlan p = stream_plan(a.g, target_blocks);
+    stage_stream_kernel<double, MAG, ROT, STAGE, 1, true, true, true, EXT_INTERIOR, true>
+        <<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
+}
+
+template <int STAGE>
+void stage_s32(bool mag, bool rot, const StageArgs& a, hipStream_t s, int tb)
+{
+    if (mag && rot) launch_s32<true, true, STAGE>(a, s, tb);
+    else if (mag) launch_s32<true, false, STAGE>(a, s, tb);
+    else if (rot) launch_s32<false, true, STAGE>(a, s, tb);
+    else launch_s32<false, false, STAGE>(a, s, tb);
+}
+
 }  // namespace
+
+const void* ext_stream_fn_s32(bool mag)
+{
+    return mag ? (const void*)stage_stream_kernel<double, true, true, 2, 1, true, true, true, EXT_INTERIOR, true>
+               : (const void*)stage_stream_kernel<double, false, true, 2, 1, true, true, true, EXT_INTERIOR, true>;
+}
+
+void ext_stage_s32(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s, int target_blocks)
+{
+    if (stage == 1) stage_s32<1>(mag, rot, a, s, target_blocks);
+    else if (stage == 2) stage_s32<2>(mag, rot, a, s, target_blocks);
+    else stage_s32<3>(mag, rot, a, s, target_blocks);
+}
 
 const void* ext_stream_fn(bool mag, int variant)
 {

@@ -22,23 +22,59 @@
 
 namespace mhd {
 
+// Background profiles of the mixed form (index k * nz + z).
+// FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
+// their D1z and their D2z (one-sided on the wall planes for BX0 and BY0).
+enum { BG_R0 = 0, BG_T0 = 1, BG_DZT0 = 2, BG_D2ZT0 = 3, BG_HB = 4, BG_B0 = 5, BG_DZB = 8,
+       BG_D2ZB = 11, NBGF = 14 };
+// FP64: RO0, TT0, BX0 BY0 BZ0.
+enum { BG64_R0 = 0, BG64_T0 = 1, BG64_B0 = 2, NBG64 = 5 };
+
+// Index of the FP64 background of field f (RO, TT, BX, BY, BZ), or -1.
+__host__ __device__ __forceinline__ int bg64_slot(int f)
+{
+    return f == 3 ? BG64_R0 : (f == 4 ? BG64_T0 : (f >= 5 ? BG64_B0 + (f - 5) : -1));
+}
+
+// A value every thread of a workgroup reads at the same address (a plane's
+// metric term or background value, the plane's TSTEP record), read through
+// the constant address space: the compiler then issues a scalar load, which
+// waits on its own counter instead of the in-order vector-memory counter
+// the stencil loads of the next plane are counted on.  Only for memory no
+// kernel writes while it runs.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p)
+{
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // State accessor that reads the register straight from global memory and
 // forms 1/RO, u, v, w and P on the fly (3dmhdsub.h:267-276).  TI is the type
 // the register is stored in, T the type the fluxes are evaluated in.
 // GHOST_COPY (the off-default kernels): with nz_ghost_copy != 0 (= nz), RO
 // on a z ghost plane reads the plane two in, as fluxes() copies it there
 // under iD != 0 or lRem before forming P and 1/RO (3dmhdsub.h:230-239).
-template <typename T, typename TI = T, bool GHOST_COPY = false>
+// BG: the register holds the mixed form's FP32 copy (RO, TT, B as
+// deviations from the FP64 background bg64 of each plane, in the layout of
+// StageArgs::bg64) and the accessor adds the background back.
+template <typename T, typename TI = T, bool GHOST_COPY = false, bool BG = false>
 struct GAcc {
     static constexpr bool kNeighbourRO = true;   // ro() at any offset
     const TI* __restrict__ U;
     int64_t fs, plane;
     int pitch, nPx, nPy, x, y, z;
     int nz_ghost_copy = 0;
+    const double* __restrict__ bg64 = nullptr;   // BG only
+    int nzb = 0;                                 // BG only: planes of bg64
     __device__ __forceinline__ T ld(int f, int dx, int dy, int dz) const
     {
         const int xx = wrapi(x + dx, nPx), yy = wrapi(y + dy, nPy);
-        return T(U[f * fs + (int64_t)(z + dz) * plane + (int64_t)yy * pitch + xx]);
+        const TI v = U[f * fs + (int64_t)(z + dz) * plane + (int64_t)yy * pitch + xx];
+        if constexpr (BG) {
+            const int k = bg64_slot(f);
+            return k < 0 ? T(v) : T(double(v) + uniform_load(bg64 + k * nzb + (z + dz)));
+        }
+        return T(v);
     }
     __device__ __forceinline__ T ru(int a, int b, int c) const { return ld(0, a, b, c); }
     __device__ __forceinline__ T rv(int a, int b, int c) const { return ld(1, a, b, c); }
@@ -86,19 +122,6 @@ struct DevScalars {
 // 344,416,427-439); the kernels of the domain do nothing until a new state.
 enum : unsigned int { HALT_GUARD = 1u, HALT_STOP = 2u, HALT_EXIT = 4u };
 
-// Background profiles of the mixed form (index k * nz + z).
-// FP32: RO0, TT0, D1z(TT0), D2z(TT0), RO0*GRAV - D1z(P0), then BX0 BY0 BZ0,
-// their D1z and their D2z (one-sided on the wall planes for BX0 and BY0).
-enum { BG_R0 = 0, BG_T0 = 1, BG_DZT0 = 2, BG_D2ZT0 = 3, BG_HB = 4, BG_B0 = 5, BG_DZB = 8,
-       BG_D2ZB = 11, NBGF = 14 };
-// FP64: RO0, TT0, BX0 BY0 BZ0.
-enum { BG64_R0 = 0, BG64_T0 = 1, BG64_B0 = 2, NBG64 = 5 };
-
-// Index of the FP64 background of field f (RO, TT, BX, BY, BZ), or -1.
-__host__ __device__ __forceinline__ int bg64_slot(int f)
-{
-    return f == 3 ? BG64_R0 : (f == 4 ? BG64_T0 : (f >= 5 ? BG64_B0 + (f - 5) : -1));
-}
 
 // Global-memory accessor of the mixed form's wall kernel: reads the FP64
 // state (TI = double, stage 1; deviations are formed in FP64, then rounded)
@@ -185,17 +208,6 @@ struct StageArgs {
 // Forward: whether the DT guard has stopped the domain (defined below).
 __device__ __forceinline__ bool domain_halted(const StageArgs& a);
 
-// A value every thread of a workgroup reads at the same address (a plane's
-// metric term or background value, the plane's TSTEP record), read through
-// the constant address space: the compiler then issues a scalar load, which
-// waits on its own counter instead of the in-order vector-memory counter
-// the stencil loads of the next plane are counted on.  Only for memory no
-// kernel writes while it runs.
-template <typename T>
-__device__ __forceinline__ T uniform_load(const T* p)
-{
-    return *(const __attribute__((address_space(4))) T*)p;
-}
 
 // Background subtracted from field f of plane z when the FP32 copy is written.
 __device__ __forceinline__ double bg_sub(const StageArgs& a, int f, int z)
@@ -509,11 +521,12 @@ __device__ __forceinline__ void bcon_point(const Phys& c, const Metric& m, const
 // at stages 2-3; the update, BCON and the Z register in FP64; BCON reads the
 // new inward planes from the FP32 copy at stages 1-2 and from the FP64 state
 // at stage 3).
-template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false>
+template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false, bool S32 = false>
 __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, bool top, CflAcc& cfl)
 {
     constexpr bool F64 = std::is_same<T, double>::value;
-    static_assert(F64 || !EXT, "the off-default branches are FP64 only");
+    static_assert(F64 || !EXT, "the off-default branches are evaluated in FP64");
+    static_assert(F64 || !S32, "S32: FP64 fluxes from the mixed form's FP32 copies");
     constexpr int NF = MAG ? NF_MAG : NF_HYDRO;
     const int z = top ? 1 : a.g.nPz;
     const int zi1 = top ? z + 1 : z - 1;   // first plane inward
@@ -525,18 +538,29 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
     using TO = typename std::conditional<F64 || STAGE == 1, double, T>::type;
     FluxOutT<TO> o;
     if constexpr (F64 || STAGE == 1) {
-        GAcc<double, double, EXT> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
-        ExtPt e;
-        if constexpr (EXT) {
-            ext_point(a, x, y, z, (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x, e);
-            if (a.x.flags & EXT_ROCOPY) acc.nz_ghost_copy = a.g.nz;
-        }
-        if (top)
-            flux_point<MAG, ROT, 1, double, false, EXT>(acc, a.c, pm, o, &e);
-        else
-            flux_point<MAG, ROT, 2, double, false, EXT>(acc, a.c, pm, o, &e);
-        if constexpr (EXT) {
-            if (a.x.flags & EXT_SHR) a.x.ft[(int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x] = o.FT;
+        auto eval = [&](auto& acc) {
+            ExtPt e;
+            if constexpr (EXT) {
+                ext_point(a, x, y, z, (int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x, e);
+                if (a.x.flags & EXT_ROCOPY) acc.nz_ghost_copy = a.g.nz;
+            }
+            if (top)
+                flux_point<MAG, ROT, 1, double, false, EXT>(acc, a.c, pm, o, &e);
+            else
+                flux_point<MAG, ROT, 2, double, false, EXT>(acc, a.c, pm, o, &e);
+            if constexpr (EXT) {
+                if (a.x.flags & EXT_SHR) a.x.ft[(int64_t)z * a.g.plane + (int64_t)y * a.g.pitch + x] = o.FT;
+            }
+        };
+        if constexpr (S32 && STAGE > 1) {
+            // the FP32 copy of the stage value, its FP64 background added back
+            GAcc<double, float, EXT, true> acc{a.Uin32, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+            acc.bg64 = a.bg64;
+            acc.nzb = a.g.nz;
+            eval(acc);
+        } else {
+            GAcc<double, double, EXT> acc{a.Uin, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z};
+            eval(acc);
         }
     } else {
         GAccDev<float> acc{a.Uin32, a.g.fs, a.g.plane, a.g.pitch, a.g.nPx, a.g.nPy, x, y, z,
@@ -561,7 +585,7 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
         un[f] = zin + cg * fsel(o, f);
     }
     // BCON (3dmhdsub.h:727-872) on the updated planes inward
-    const bool new64 = F64 || STAGE == 3;
+    const bool new64 = (F64 && !S32) || STAGE == 3;
     auto Un = [&](int f, int64_t j, int zj) -> double {
         if (new64) return a.Uout[f * fs + j];
         return double(a.Uout32[f * fs + j]) + bg_sub(a, f, zj);
@@ -592,7 +616,7 @@ __device__ __forceinline__ void wall_point(const StageArgs& a, int x, int y, boo
 // for three waves per SIMD (FP64) or two (the mixed form's deviation
 // stencils), so the register allocator keeps the occupancy; the off-default
 // variant needs the registers of one wave.
-template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false>
+template <typename T, bool MAG, bool ROT, int STAGE, bool EXT = false, bool S32 = false>
 __global__ void __launch_bounds__(256, EXT ? 1 : (sizeof(T) == 4 ? 2 : 3)) stage_wall_kernel(StageArgs a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -602,7 +626,7 @@ __global__ void __launch_bounds__(256, EXT ? 1 : (sizeof(T) == 4 ? 2 : 3)) stage
     // every thread reaches the block reduction below; threads outside the
     // domain only contribute the identity
     CflAcc cfl;
-    if (x < a.g.nPx && y < a.g.nPy) wall_point<T, MAG, ROT, STAGE, EXT>(a, x, y, top, cfl);
+    if (x < a.g.nPx && y < a.g.nPy) wall_point<T, MAG, ROT, STAGE, EXT, S32>(a, x, y, top, cfl);
     if (STAGE == 3 && a.cfl_parts) {
         const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + b);
@@ -886,19 +910,30 @@ __device__ __forceinline__ double ordered_plane_sum(const Geom& g, V value, doub
 // Horizontal means of RO (blockIdx.y = 0) and TT (1) on every plane of
 // register U, into out[k * nz + z], divided by float(nPx*nPy) as there.  For
 // RO the z ghost planes take the planes two in, which fluxes() copies there
-// before taking the mean (3dmhdsub.h:230-239).
-MHD_TU_LOCAL __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt,
-                  double* __restrict__ out)
+// before taking the mean (3dmhdsub.h:230-239).  TI = float: U is the mixed
+// form's FP32 copy, whose values are deviations from bg64 (added back).
+template <typename TI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+plane_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64, int want_ro,
+                  int want_tt, double* __restrict__ out)
 {
     __shared__ __align__(16) double buf[2][PM_G * 64];
     const int z = blockIdx.x, k = blockIdx.y;
     if (k == 0 ? !want_ro : !want_tt) return;
     int zs = z;
     if (k == 0) zs = z == 0 ? 2 : (z == g.nz - 1 ? g.nz - 3 : z);
-    const double* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
+    const TI* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
+    double b = 0.0;
+    if constexpr (std::is_same<TI, float>::value) b = bg64[(k == 0 ? BG64_R0 : BG64_T0) * g.nz + zs];
     const double s = ordered_plane_sum(
-        g, [&](int y, int x) -> double { return p[(int64_t)y * g.pitch + x]; }, buf);
+        g,
+        [&](int y, int x) -> double {
+            if constexpr (std::is_same<TI, float>::value)
+                return double(p[(int64_t)y * g.pitch + x]) + b;
+            else
+                return p[(int64_t)y * g.pitch + x];
+        },
+        buf);
     if (threadIdx.x == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
 }
 
@@ -910,25 +945,38 @@ plane_mean_kernel(Geom g, const double* __restrict__ U, int want_ro, int want_tt
 // left there at :267.  Products in the reference's order (func_mul2,
 // func_mul3: ((a*b)*c)*s).  Skipped while the domain is stopped when
 // `stopped` is set (a stage), evaluated regardless otherwise (mhd3d_fluxes).
-MHD_TU_LOCAL __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-conv_flux_mean_kernel(Geom g, const double* __restrict__ U, const double* __restrict__ ttm, int shr,
-                      const DevScalars* __restrict__ ds, int stopped, double* __restrict__ fconm)
+template <typename TI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+conv_flux_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64,
+                      const double* __restrict__ ttm, int shr, const DevScalars* __restrict__ ds,
+                      int stopped, double* __restrict__ fconm)
 {
     __shared__ __align__(16) double buf[2][PM_G * 64];
     if (stopped && ds->halted) return;
     const int z = blockIdx.x + 1;
     const int64_t o = (int64_t)z * g.plane;
-    const double* __restrict__ ru = U + o;
-    const double* __restrict__ rv = U + g.fs + o;
-    const double* __restrict__ rw = U + 2 * g.fs + o;
-    const double* __restrict__ ro = U + 3 * g.fs + o;
-    const double* __restrict__ tt = U + 4 * g.fs + o;
+    const TI* __restrict__ ru = U + o;
+    const TI* __restrict__ rv = U + g.fs + o;
+    const TI* __restrict__ rw = U + 2 * g.fs + o;
+    const TI* __restrict__ ro = U + 3 * g.fs + o;
+    const TI* __restrict__ tt = U + 4 * g.fs + o;
     const double tm = ttm[z];
+    // TI = float: the mixed form's FP32 copy, RO and TT as deviations from bg64
+    [[maybe_unused]] double r0 = 0.0, t0 = 0.0;
+    if constexpr (std::is_same<TI, float>::value) {
+        r0 = bg64[BG64_R0 * g.nz + z];
+        t0 = bg64[BG64_T0 * g.nz + z];
+    }
     auto conv = [&](int y, int x) -> double {
         const int64_t i = (int64_t)y * g.pitch + x;
-        const double r = ro[i], t = tt[i], w = rw[i];
+        double r = double(ro[i]), t = double(tt[i]);
+        if constexpr (std::is_same<TI, float>::value) {
+            r += r0;
+            t += t0;
+        }
+        const double w = double(rw[i]);
         const double ri = 1.0 / r;
-        const double uu = ru[i] * ri, vv = rv[i] * ri, ww = w * ri;
+        const double uu = double(ru[i]) * ri, vv = double(rv[i]) * ri, ww = w * ri;
         const double w1 = shr ? r * t : t - tm;
         double v = (w * w1) * -2.5;
         v += ((w * uu) * uu) * -0.5;

@@ -93,7 +93,8 @@ struct PrepPt {
 // RO, TT, BX, BY, BZ (or, when the input is the FP64 state, its deviations
 // from the FP64 background rounded to FP32), with the full density RO0 + r
 // and P - P0 = (RO0*t + r*TT0) + r*t formed from the FP32 background of
-// plane z.
+// plane z.  FP64 values from an FP32 copy (the mixed form with the
+// off-default branches): the deviations plus the FP64 background.
 template <typename T, bool MAG, typename TI>
 __device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const StageArgs& a, int z)
 {
@@ -106,12 +107,25 @@ __device__ __forceinline__ PrepPt<T> prep_point(const RawPt<TI, MAG>& raw, const
         q.bx = T(raw.bx - uniform_load(a.bg64 + (BG64_B0 + 0) * nz + z));
         q.by = T(raw.by - uniform_load(a.bg64 + (BG64_B0 + 1) * nz + z));
         q.bz = T(raw.bz - uniform_load(a.bg64 + (BG64_B0 + 2) * nz + z));
+    } else if constexpr (std::is_same<T, double>::value && std::is_same<TI, float>::value && MAG) {
+        const int nz = a.g.nz;
+        q.bx = double(raw.bx) + uniform_load(a.bg64 + (BG64_B0 + 0) * nz + z);
+        q.by = double(raw.by) + uniform_load(a.bg64 + (BG64_B0 + 1) * nz + z);
+        q.bz = double(raw.bz) + uniform_load(a.bg64 + (BG64_B0 + 2) * nz + z);
     } else {
         q.bx = T(raw.bx);
         q.by = T(raw.by);
         q.bz = T(raw.bz);
     }
-    if constexpr (std::is_same<T, double>::value) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<TI, float>::value) {
+        // FP64 fluxes from the mixed form's FP32 copy: the FP64 background
+        // of plane z added back to the stored deviations
+        const int nz = a.g.nz;
+        q.ro = double(raw.ro) + uniform_load(a.bg64 + BG64_R0 * nz + z);
+        q.tt = double(raw.tt) + uniform_load(a.bg64 + BG64_T0 * nz + z);
+        q.p = q.ro * q.tt;
+        q.rdev = T(0);
+    } else if constexpr (std::is_same<T, double>::value) {
         q.ro = raw.ro;
         q.tt = raw.tt;
         q.p = raw.ro * raw.tt;
@@ -221,8 +235,11 @@ struct StreamLds {
 // one-sided stencils from global memory would raise this kernel's register
 // budget).  lds: StreamLds::bytes of shared memory.  Stage 3 accumulates the
 // CFL terms of every point it updates into cfl.
+// S32 (with T = double): the mixed form with the off-default branches --
+// fluxes evaluated in FP64 (with the branches) from the FP32 copies of the
+// stage values, which stages 1-2 write as the mixed form does.
 template <typename T, bool MAG, bool ROT, int STAGE, bool ONE_BARRIER, bool DEFER, bool EXT = false,
-          int EXTM = EXT_INTERIOR>
+          int EXTM = EXT_INTERIOR, bool S32 = false>
 __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int tile_y, int zb,
                                            int ze, char* lds, CflAcc& cfl)
 {
@@ -243,7 +260,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     // the mixed form also the FP64 state for stage 1
     constexpr bool ZLOAD = STAGE > 1 || !F64;
     // storage type of the stencil input: the FP64 state, or the FP32 copy
-    using TI = typename std::conditional<F64 || STAGE == 1, double, float>::type;
+    using TI = typename std::conditional<(F64 && !S32) || STAGE == 1, double, float>::type;
     T (*s_uvw)[NR][SHP] = reinterpret_cast<T (*)[NR][SHP]>(lds);
     T (*s_st)[NS][SHP] = reinterpret_cast<T (*)[NS][SHP]>(lds + sizeof(T) * NRING * NR * SHP);
 
@@ -337,7 +354,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         for (int f = 0; f < NF; ++f) {
             const double F = fsel(o, f);
             un[f] = zin[f] + cg * F;
-            if (F64 || STAGE == 3)
+            if ((F64 && !S32) || STAGE == 3)
                 stream_store(&a.Uout[f * fs + i], un[f]);
             else
                 stream_store(&a.Uout32[f * fs + i], float(un[f] - bg_sub(a, f, zq)));
@@ -564,7 +581,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
 // XCDs round-robin) so halo re-reads hit its L2.  Chunk k covers the planes
 // 2 + k*zchunk .. (the last chunk runs to nPz-1).
 template <typename T, bool MAG, bool ROT, int STAGE, int WAVES, bool ONE_BARRIER, bool DEFER,
-          bool EXT = false, int EXTM = EXT_INTERIOR>
+          bool EXT = false, int EXTM = EXT_INTERIOR, bool S32 = false>
 __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, int zchunk, int nch,
                                                                   int ntx, int nty, int xcd_remap)
 {
@@ -581,7 +598,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     const int zb = 2 + chunk * zchunk;
     const int ze = chunk == nch - 1 ? a.g.nPz : zb + zchunk;
     CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
-    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER, EXT, EXTM>(a, tile_x, tile_y, zb, ze, lds, cfl);
+    stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER, EXT, EXTM, S32>(a, tile_x, tile_y, zb, ze, lds, cfl);
     if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
 }
 

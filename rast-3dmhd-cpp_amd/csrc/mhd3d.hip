@@ -131,6 +131,8 @@ static int timedep_flags(const mhd3d_desc* d)
 {
     return d->TC != 0.0 ? (ext_flags(d) & (EXT_HEAT | EXT_PLUME)) : 0;
 }
+static const char* const kMixedBranchWhy =
+    "the off-default branches in the mixed form run on single domains only";
 static const char* const kTimedepWhy =
     "the heat loss and the plume with TC != 0 take factors of TIMT formed on the host before every "
     "step (3dmhdsub.h:446-460,752-757): single domains only";
@@ -158,10 +160,6 @@ static int validate(const mhd3d_desc* d, std::string& why)
     if (d->flag_iTCon < 0 || d->flag_iTCon > 4) { why = "invalid iTCon"; return MHD3D_EINVAL; }
     if (d->flag_izCon != 0 && d->flag_izCon != 1) { why = "BCON:  Invalid izCon (3dmhdsub.h:863)"; return MHD3D_EINVAL; }
     const int ef = ext_flags(d);
-    if (ef && d->precision == MHD3D_MIXED) {
-        why = "the off-default branches (iD, lRem, lShr, iTCon 2, TP plume, RLAX) are built for the FP64 form only";
-        return MHD3D_EUNSUPPORTED;
-    }
     if (((ef & (EXT_DH | EXT_DV)) && d->flag_iD == 2 && !d->ZEE) ||
         ((ef & EXT_HEAT) && (!d->EXX || !d->WYY || !d->ZEE)) || ((ef & EXT_PLUME) && (!d->EXX || !d->WYY))) {
         why = "coordinate array (EXX, WYY or ZEE) missing";
@@ -340,7 +338,9 @@ static int variant_resident(const mhd3d_ctx* ctx, int w, int sc)
 {
     const bool mag = ctx->d.flag_lMag != 0;
     const void* fn;
-    if (ctx->mixed)
+    if (ctx->mixed && ctx->ext)
+        fn = ext_stream_fn_s32(mag);
+    else if (ctx->mixed)
         fn = mixed_stream_fn(mag, w, sc);
     else if (ext_variant(ctx->xa.flags) != EXTV_NONE)
         fn = ext_stream_fn(mag, ext_variant(ctx->xa.flags));
@@ -796,7 +796,9 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     const int fm = ctx->mixed ? 1 : 0, ph = MAG ? 1 : 0;
-    if (ctx->mixed) {
+    if (ctx->mixed && ctx->ext) {
+        ext_stage_s32(MAG, ROT, STAGE, a, ctx->stream, ctx->target_blocks);
+    } else if (ctx->mixed) {
         mixed_stage(MAG, ROT, STAGE, a, ctx->stream, ctx->tune_waves[fm][ph], ctx->tune_sched[fm][ph],
                     ctx->target_blocks);
     } else if (ctx->kernel_mode == 0 && ext_variant(ctx->xa.flags) != EXTV_NONE) {
@@ -819,7 +821,9 @@ static void launch_wall_t(mhd3d_ctx* ctx, const StageArgs& a)
     const Geom& g = ctx->g;
     dim3 blk(64, 4, 1);
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, 2);
-    if (ctx->mixed)
+    if (ctx->mixed && ctx->ext)
+        stage_wall_kernel<double, MAG, ROT, STAGE, true, true><<<grd, blk, 0, ctx->stream>>>(a);
+    else if (ctx->mixed)
         mixed_wall(MAG, ROT, STAGE, a, grd, blk, ctx->stream);
     else if (ctx->ext)
         stage_wall_kernel<double, MAG, ROT, STAGE, true><<<grd, blk, 0, ctx->stream>>>(a);
@@ -879,22 +883,23 @@ static bool needs_prepass(const mhd3d_ctx* ctx)
 // it stays zero), then for RLAX the mean convective flux FCONM and the
 // relaxation's scalar pass (TFAC, CORRECT, the exit test).  stage = 1..3
 // inside a step (a stopped domain skips it), 0 for mhd3d_fluxes.
-static int run_prepass(mhd3d_ctx* ctx, const double* U, int stage, hipStream_t stream)
+template <typename TI>
+static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t stream)
 {
     const int f = ctx->xa.flags;
     const int want_ro = (f & (EXT_DH | EXT_DV)) ? 1 : 0;
     const int want_tt = ((f & (EXT_REM | EXT_RLAX)) && !(f & EXT_SHR)) ? 1 : 0;
     if (want_ro || want_tt) {
-        plane_mean_kernel<<<dim3(ctx->g.nz, 2, 1), 64, 0, stream>>>(ctx->g, U, want_ro, want_tt,
-                                                                    ctx->ext_buf);
+        plane_mean_kernel<TI><<<dim3(ctx->g.nz, 2, 1), 64, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro,
+                                                                        want_tt, ctx->ext_buf);
         HIPCHK(ctx, hipGetLastError());
     }
     if (f & EXT_RLAX) {
         const int nz = ctx->g.nz;
         double* fconm = ctx->relax_buf;
-        conv_flux_mean_kernel<<<ctx->g.nPz, 64, 0, stream>>>(ctx->g, U, ctx->ext_buf + nz,
-                                                             (f & EXT_SHR) ? 1 : 0, ctx->ds,
-                                                             stage ? 1 : 0, fconm);
+        conv_flux_mean_kernel<TI><<<ctx->g.nPz, 64, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
+                                                                 (f & EXT_SHR) ? 1 : 0, ctx->ds,
+                                                                 stage ? 1 : 0, fconm);
         HIPCHK(ctx, hipGetLastError());
         relax_kernel<<<1, 1, 0, stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
                                          ctx->relax_buf + nz, ctx->relax_buf + 2 * (size_t)nz + 1,
@@ -904,11 +909,20 @@ static int run_prepass(mhd3d_ctx* ctx, const double* U, int stage, hipStream_t s
     return MHD3D_OK;
 }
 
+// The prepass on the input of stage k's fluxes (k = 0: mhd3d_fluxes, on
+// the state): the FP64 register, or in the mixed form's stages 2-3 the FP32
+// copy of the stage value.
+static int run_prepass(mhd3d_ctx* ctx, const StageArgs& a, int k, hipStream_t stream)
+{
+    if (ctx->mixed && k >= 2) return run_prepass_t<float>(ctx, a.Uin32, k, stream);
+    return run_prepass_t<double>(ctx, a.Uin, k, stream);
+}
+
 static int run_stage(mhd3d_ctx* ctx, int k, bool fused_cfl = false)
 {
     StageArgs a = stage_args(ctx, k);
     if (ctx->ext) {
-        int rc = run_prepass(ctx, a.Uin, k, ctx->stream);
+        int rc = run_prepass(ctx, a, k, ctx->stream);
         if (rc) return rc;
     }
     if (fused_cfl && k == 3 && ctx->kernel_mode == 0) {
@@ -1201,7 +1215,7 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
     dim3 grd((g.nPx + 63) / 64, (g.nPy + 3) / 4, g.nPz);
     const bool mag = ctx->d.flag_lMag, rot = ctx->d.flag_lRot;
     if (ctx->ext) {
-        int rc = run_prepass(ctx, a.Uin, 0, ctx->stream);
+        int rc = run_prepass(ctx, a, 0, ctx->stream);
         if (rc) return rc;
         if (mag && rot) flux_only_kernel<true, true, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
         else if (mag) flux_only_kernel<true, false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
@@ -1770,6 +1784,8 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
             return bail(MHD3D_EINVAL, "ensemble members must share one precision form");
         if (timedep_flags(descs[m]))
             return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) + ": " + kTimedepWhy);
+        if (e->mixed && ext_flags(descs[m]))
+            return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) + ": " + kMixedBranchWhy);
         mhd3d_ctx* c = nullptr;
         int rc = mhd3d_create(descs[m], &c);
         if (rc) return bail(rc, "member " + std::to_string(m) + ": " + g_create_error);
@@ -1882,7 +1898,7 @@ static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
             for (int m = 0; m < e->n; ++m) {
                 mhd3d_ctx* c = e->mem[m];
                 if (needs_prepass(c))
-                    (void)run_prepass(c, c->U[p0 ^ e->off[m] ^ ((k - 1) & 1)], k, e->stream);
+                    (void)run_prepass_t<double>(c, c->U[p0 ^ e->off[m] ^ ((k - 1) & 1)], k, e->stream);
             }
         }
         if (e->mixed)
@@ -2300,6 +2316,10 @@ extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, cons
     if (desc->precision != s->precision) { s->err = "request precision differs from the server's"; return MHD3D_EINVAL; }
     if (timedep_flags(desc)) {
         s->err = kTimedepWhy;
+        return MHD3D_EUNSUPPORTED;
+    }
+    if (desc->precision == MHD3D_MIXED && ext_flags(desc)) {
+        s->err = kMixedBranchWhy;
         return MHD3D_EUNSUPPORTED;
     }
     ServerReq r;

@@ -2,10 +2,11 @@
 the descriptor before its first HIP call, so a refused descriptor returns
 without a device).
 
-Refused: the branches (RLAX among them) in the mixed form, iD outside 0..2,
-missing coordinate arrays, and RLAX without the z rank layout of the
-reference run its scan reproduces (relax_nPEz >= 2 dividing nPz, at least 3
-planes per rank, 3dmhdsub.h:354-421).
+Refused: iD outside 0..2, missing coordinate arrays, RLAX without the z
+rank layout of the reference run its scan reproduces (relax_nPEz >= 2
+dividing nPz, at least 3 planes per rank, 3dmhdsub.h:354-421), and the
+branches in the mixed form of an ensemble (single mixed domains take them;
+ensembles check their members before any device call).
 """
 import ctypes as C
 import dataclasses
@@ -42,18 +43,20 @@ def test_rlax_needs_the_rank_layout():
     for P in (0, 1, 3, 4):                  # nPz = 8: 3 does not divide it, 4 leaves 2 planes a rank
         rc, msg = _create(dataclasses.replace(cfg, par=par, relax_nPEz=P))
         assert rc == L.EINVAL and "relax_nPEz" in msg, P
-    rc, msg = _create(dataclasses.replace(cfg, par=par, relax_nPEz=2, precision=1))
-    assert rc == L.EUNSUPPORTED and "RLAX" in msg
 
 
-def test_mixed_form_refuses_branches():
+def test_mixed_ensemble_refuses_branches():
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("libmhd3d.so not built")
     cfg = _base(precision=1, iD=1)
     par = cfg.par.copy()
     par[55] = 0.5
-    rc, msg = _create(dataclasses.replace(cfg, par=par))
-    assert rc == L.EUNSUPPORTED and "FP64" in msg
-    rc, _ = _create(_base(precision=1, lShr=True))
-    assert rc == L.EUNSUPPORTED
+    desc = Descriptor(dataclasses.replace(cfg, par=par))
+    arr = (C.c_void_p * 1)(C.cast(desc.ptr(), C.c_void_p))
+    h = C.c_void_p()
+    rc = L.lib().mhd3d_ensemble_create(arr, 1, C.byref(h))
+    msg = (L.lib().mhd3d_ensemble_last_error(None) or b"").decode()
+    assert rc == L.EUNSUPPORTED and "mixed" in msg and not h.value
 
 
 def test_bad_iD_and_missing_coordinates():

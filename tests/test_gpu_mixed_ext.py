@@ -1,0 +1,77 @@
+"""The mixed form with the off-default branches, on the GPU, against the
+oracle (FP64).
+
+With a branch switched on, the mixed form keeps its storage -- FP32 copies of
+the stage values (deviations from the FP64 background of each plane), FP64
+RK registers -- and evaluates the fluxes in FP64 from those copies with the
+background added back, so the branches run through the FP64 code.  Bar:
+after 10 CFL steps every field within 1e-5 of the oracle
+(max|g - c| / max|c| over the interior incl. walls; the FP32 rounding of the
+stage values gives ~1e-7), for iD 1 and 2, lRem, lShr (hydro and MHD), lShr
+with lRem, the iTCon 2 heat loss with its time factor, the TP plume, and
+RLAX (both wall settings, with lRem and MHD); RLAX's statics follow the
+oracle's (the call count exactly, TFAC to 1e-12).  Under lShr the walls
+drive a shear flow (|RV| ~ 10, stored in FP32 as it is: only RO, TT and B
+are stored as deviations from a background) next to |RU| ~ 1e-3, so RU's
+error relative to its own maximum is bounded by 5e-5 there (measured
+1.2e-5).
+"""
+import dataclasses
+
+import pytest
+
+import test_gpu_ext as tx
+import test_gpu_relax as tr
+from gpu_common import relerr
+from mhd3d.desc import Descriptor
+from mhd3d.domain import Domain
+from mhd3d.initial import stratified_state
+from oracle import oracle as om
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+TOL_SHEAR = 5e-5
+
+CASES = ["iD1", "iD2_mhd", "lRem1", "lShr", "lShr_mhd", "lShrRem", "heatT", "plume0"]
+RLAX_CASES = [("it0", 2), ("it1_rem_mhd", 4)]
+
+
+def _run(cfg, seed, what, tol=TOL):
+    d64 = Descriptor(cfg)
+    dmx = Descriptor(dataclasses.replace(cfg, precision=1))
+    st = stratified_state(d64, seed=seed)
+    orc = om.Oracle(d64)
+    orc.set_state(st)
+    orc.communicate()
+    for _ in range(10):
+        orc.step()
+    dom = Domain(dmx)
+    dom.set_state(st)
+    dom.step(10)
+    worst = 0.0
+    for name, g in dom.state().items():
+        e = relerr(g, orc.get(name))
+        worst = max(worst, e)
+        assert e <= tol, (what, name, e)
+    print(f"  {what}: worst field error {worst:.2e}", flush=True)
+    return dom, orc
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_branch_in_mixed_form(case):
+    print(f"mixed branch {case}", flush=True)
+    dom, orc = _run(tx._cfg((24, 20, 12), case), 7, case, TOL_SHEAR if case.startswith("lShr") else TOL)
+    dom.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("case,P", RLAX_CASES)
+def test_rlax_in_mixed_form(case, P):
+    print(f"mixed rlax {case} P={P}", flush=True)
+    dom, orc = _run(tr._cfg((24, 20, 12), case, P=P), 7, (case, P))
+    g, c = dom.relax_state(), orc.relax_state()
+    assert g["ICALL"] == c["ICALL"] == 30
+    assert abs(g["TFAC"] - c["TFAC"]) <= 1e-12 * abs(c["TFAC"])
+    dom.close()
+    orc.close()
