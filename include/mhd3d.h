@@ -54,7 +54,8 @@ enum {
 enum {
     MHD3D_FP64 = 0,   /* plain FP64, same operation order as the reference   */
     MHD3D_MIXED = 1   /* FP32 stencils from FP32 copies, FP64 RK registers,
-                         FP64 CFL and diagnostics                             */
+                         FP64 CFL and diagnostics; with an off-default branch
+                         the fluxes are FP64 from the same FP32 copies      */
 };
 
 /*
