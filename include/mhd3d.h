@@ -260,12 +260,12 @@ int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const double* RO_z
  * one precision form for all) advanced together -- one persistent launch per
  * RK3 stage for the planes between the walls and one for the wall planes of
  * every member, each member with its own CFL time step.  A member advances
- * bit-for-bit as the same domain does alone through mhd3d_step.  FP64
- * members may take the off-default branches (iD, lRem, lShr, iTCon 2, the
- * TP plume, RLAX), except the heat loss and the plume with TC != 0, whose
- * factors of TIMT the host forms before every step (MHD3D_EUNSUPPORTED); a
- * member that reaches an RLAX exit point stops there and mhd3d_ensemble_step
- * returns MHD3D_EEXIT.  No counterpart in the reference (its step() advances
+ * bit-for-bit as the same domain does alone through mhd3d_step.  Members of
+ * either form may take the off-default branches (iD, lRem, lShr, iTCon 2,
+ * the TP plume, RLAX), except the heat loss and the plume with TC != 0,
+ * whose factors of TIMT the host forms before every step
+ * (MHD3D_EUNSUPPORTED); a member that reaches an RLAX exit point stops there
+ * and mhd3d_ensemble_step returns MHD3D_EEXIT.  No counterpart in the reference (its step() advances
  * one domain, 3dmhdsub.h:31).
  * ------------------------------------------------------------------------ */
 typedef struct mhd3d_ensemble mhd3d_ensemble;

@@ -1,5 +1,5 @@
-// Launches of the off-default interior stage kernels (FP64 only), defined in
-// ext_stream.hip.  Each variant compiles in one set of branches (EXTM mask),
+// Launches of the off-default interior stage kernels (FP64 fluxes), defined
+// in ext_stream.hip.  Each variant compiles in one set of branches (EXTM mask),
 // so a run with one off-default switch does not carry the registers of the
 // others; a run whose interior switches fit no single set takes the variant
 // with all of them.  Every variant has one wave per SIMD, one barrier per
@@ -46,5 +46,17 @@ void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, h
 // state), the copies written as in the mixed form.
 const void* ext_stream_fn_s32(bool mag);
 void ext_stage_s32(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s, int target_blocks);
+
+struct WorkItem;
+struct WallItem;
+
+// Mixed-form ensemble members with off-default branches: the persistent
+// interior launch over `items` (grid_int blocks, ensemble_stage_ext_kernel
+// with S32) and the persistent wall launch over `walls` (grid_wall blocks);
+// an empty list launches nothing.  The host stub of the interior stage-2
+// kernel is for occupancy queries.
+const void* ext_ens_fn_s32();
+void ext_ens_stage_s32(int stage, const StageArgs* args, const WorkItem* items, int nitems, int grid_int,
+                       const WallItem* walls, int nwalls, int grid_wall, hipStream_t s);
 
 }  // namespace mhd

@@ -1,5 +1,6 @@
-// The off-default interior stage kernels (FP64): the streaming kernel with
-// the branches of one ExtVariant compiled in.  Kept in a translation unit of
+// The off-default interior stage kernels (FP64 fluxes): the streaming kernel
+// with the branches of one ExtVariant compiled in, and those of the mixed
+// form (single domains and ensemble members).  Kept in a translation unit of
 // its own so the 60 instantiations build in parallel with the main one; the
 // arithmetic is that of mhd3d.hip's kernels (no contraction).
 #define MHD3D_SECONDARY_TU 1
@@ -8,6 +9,7 @@
 
 #include "ext_launch.hpp"
 #include "kernels_basic.hpp"
+#include "kernels_ensemble.hpp"
 #include "kernels_stream.hpp"
 
 namespace mhd {
@@ -85,6 +87,28 @@ void ext_stage_s32(bool mag, bool rot, int stage, const StageArgs& a, hipStream_
     if (stage == 1) stage_s32<1>(mag, rot, a, s, target_blocks);
     else if (stage == 2) stage_s32<2>(mag, rot, a, s, target_blocks);
     else stage_s32<3>(mag, rot, a, s, target_blocks);
+}
+
+const void* ext_ens_fn_s32()
+{
+    return (const void*)ensemble_stage_ext_kernel<2, true>;
+}
+
+void ext_ens_stage_s32(int stage, const StageArgs* args, const WorkItem* items, int nitems, int grid_int,
+                       const WallItem* walls, int nwalls, int grid_wall, hipStream_t s)
+{
+    if (nitems > 0) {
+        const dim3 blk(STX, STY, 1);
+        if (stage == 1) ensemble_stage_ext_kernel<1, true><<<grid_int, blk, 0, s>>>(args, items, nitems);
+        else if (stage == 2) ensemble_stage_ext_kernel<2, true><<<grid_int, blk, 0, s>>>(args, items, nitems);
+        else ensemble_stage_ext_kernel<3, true><<<grid_int, blk, 0, s>>>(args, items, nitems);
+    }
+    if (nwalls > 0) {
+        const dim3 wblk(64, 4, 1);
+        if (stage == 1) ensemble_wall_kernel<double, 1, true, true><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
+        else if (stage == 2) ensemble_wall_kernel<double, 2, true, true><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
+        else ensemble_wall_kernel<double, 3, true, true><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
+    }
 }
 
 const void* ext_stream_fn(bool mag, int variant)

@@ -88,10 +88,12 @@ __global__ void __launch_bounds__(256, WAVES)
 }
 
 // Persistent interior stage over the members whose interior points take
-// off-default branches (FP64): every branch compiled in, as the single-domain
-// kernel that holds them all (one wave per SIMD, one barrier per plane, the
-// deferred update); each point takes its member's branches from its ExtArgs.
-template <int STAGE>
+// off-default branches (FP64 fluxes): every branch compiled in, as the
+// single-domain kernel that holds them all (one wave per SIMD, one barrier per
+// plane, the deferred update); each point takes its member's branches from its
+// ExtArgs.  S32: members of the mixed form (fluxes from the FP32 copies of the
+// stage values with the background added back, as their runs alone).
+template <int STAGE, bool S32 = false>
 __global__ void __launch_bounds__(256, 1)
     ensemble_stage_ext_kernel(const StageArgs* __restrict__ args, const WorkItem* __restrict__ items,
                               int nitems)
@@ -104,19 +106,19 @@ __global__ void __launch_bounds__(256, 1)
         CflAcc cfl;
         switch (w.flags & 3) {
         case 0:
-            stage_item<double, false, false, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+            stage_item<double, false, false, STAGE, true, true, true, EXT_INTERIOR, S32>(a, w.tile_x, w.tile_y, w.zb,
                                                                                    w.ze, lds, cfl);
             break;
         case 1:
-            stage_item<double, true, false, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+            stage_item<double, true, false, STAGE, true, true, true, EXT_INTERIOR, S32>(a, w.tile_x, w.tile_y, w.zb,
                                                                                   w.ze, lds, cfl);
             break;
         case 2:
-            stage_item<double, false, true, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+            stage_item<double, false, true, STAGE, true, true, true, EXT_INTERIOR, S32>(a, w.tile_x, w.tile_y, w.zb,
                                                                                   w.ze, lds, cfl);
             break;
         default:
-            stage_item<double, true, true, STAGE, true, true, true, EXT_INTERIOR>(a, w.tile_x, w.tile_y, w.zb,
+            stage_item<double, true, true, STAGE, true, true, true, EXT_INTERIOR, S32>(a, w.tile_x, w.tile_y, w.zb,
                                                                                  w.ze, lds, cfl);
             break;
         }
@@ -126,9 +128,10 @@ __global__ void __launch_bounds__(256, 1)
 }
 
 // Persistent wall stage over the members (blocks of 64 x 4 threads).  EXT:
-// the members with off-default branches (FP64 only), in a list and a launch
-// of their own so the others keep the smaller register budget.
-template <typename T, int STAGE, bool EXT = false>
+// the members with off-default branches (FP64 fluxes; S32: of the mixed
+// form), in a list and a launch of their own so the others keep the smaller
+// register budget.
+template <typename T, int STAGE, bool EXT = false, bool S32 = false>
 __global__ void __launch_bounds__(256)
     ensemble_wall_kernel(const StageArgs* __restrict__ args, const WallItem* __restrict__ items,
                          int nitems)
@@ -142,10 +145,10 @@ __global__ void __launch_bounds__(256)
         if (x < a.g.nPx && y < a.g.nPy) {
             const bool top = w.top != 0;
             switch (w.flags & 3) {
-            case 0: wall_point<T, false, false, STAGE, EXT>(a, x, y, top, cfl); break;
-            case 1: wall_point<T, true, false, STAGE, EXT>(a, x, y, top, cfl); break;
-            case 2: wall_point<T, false, true, STAGE, EXT>(a, x, y, top, cfl); break;
-            default: wall_point<T, true, true, STAGE, EXT>(a, x, y, top, cfl); break;
+            case 0: wall_point<T, false, false, STAGE, EXT, S32>(a, x, y, top, cfl); break;
+            case 1: wall_point<T, true, false, STAGE, EXT, S32>(a, x, y, top, cfl); break;
+            case 2: wall_point<T, false, true, STAGE, EXT, S32>(a, x, y, top, cfl); break;
+            default: wall_point<T, true, true, STAGE, EXT, S32>(a, x, y, top, cfl); break;
             }
         }
         if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, w.part);

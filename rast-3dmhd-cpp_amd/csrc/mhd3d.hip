@@ -131,8 +131,6 @@ static int timedep_flags(const mhd3d_desc* d)
 {
     return d->TC != 0.0 ? (ext_flags(d) & (EXT_HEAT | EXT_PLUME)) : 0;
 }
-static const char* const kMixedBranchWhy =
-    "the off-default branches in the mixed form run on single domains only";
 static const char* const kTimedepWhy =
     "the heat loss and the plume with TC != 0 take factors of TIMT formed on the host before every "
     "step (3dmhdsub.h:446-460,752-757): single domains only";
@@ -1077,8 +1075,10 @@ static int step_core(mhd3d_ctx* ctx, int nsteps, int fixed_dt, double* dt_inout)
         HIPCHK(ctx, hipMemcpy(&rs, ctx->rs, sizeof(RelaxState), hipMemcpyDeviceToHost));
         // the state stays the input of the stage whose fluxes() exited:
         // stage 2 reads the register stage 1 wrote, stages 1 and 3 the
-        // register the step started from (the one the taken steps name)
-        if (!ctx->relax_exit_seen && rs.exit_stage == 2) ctx->cur ^= 1;
+        // register the step started from (the one the taken steps name).
+        // The mixed form has one FP64 register, which holds the step's
+        // start: its stage values live only in the FP32 copies.
+        if (!ctx->mixed && !ctx->relax_exit_seen && rs.exit_stage == 2) ctx->cur ^= 1;
         ctx->relax_exit_seen = true;
         ctx->dt_ready = false;
         return relax_exit_status(ctx, rs);
@@ -1605,7 +1605,7 @@ static int ens_resident_blocks(const mhd3d_ensemble* e, bool ext = false)
 {
     const void* fn;
     if (ext)
-        fn = (const void*)ensemble_stage_ext_kernel<2>;
+        fn = e->mixed ? ext_ens_fn_s32() : (const void*)ensemble_stage_ext_kernel<2>;
     else if (e->mixed)
         fn = mixed_ens_stage_fn(e->waves, e->sched);
     else
@@ -1683,7 +1683,9 @@ static int ens_build(mhd3d_ensemble* e)
         mhd3d_ctx* c = e->mem[m];
         const Geom& g = c->g;
         const int flags = (c->d.flag_lMag ? 1 : 0) | (c->d.flag_lRot ? 2 : 0);
-        const bool ext_interior = ext_variant(c->xa.flags) != EXTV_NONE;
+        // (a mixed-form member with any branch evaluates every flux in FP64,
+        // as alone)
+        const bool ext_interior = e->mixed ? c->ext : ext_variant(c->xa.flags) != EXTV_NONE;
         if (needs_prepass(c)) e->prepass = true;
         StreamPlan p = stream_plan(g, 256);
         if (e->chunk > 0) {
@@ -1784,8 +1786,6 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
             return bail(MHD3D_EINVAL, "ensemble members must share one precision form");
         if (timedep_flags(descs[m]))
             return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) + ": " + kTimedepWhy);
-        if (e->mixed && ext_flags(descs[m]))
-            return bail(MHD3D_EUNSUPPORTED, "member " + std::to_string(m) + ": " + kMixedBranchWhy);
         mhd3d_ctx* c = nullptr;
         int rc = mhd3d_create(descs[m], &c);
         if (rc) return bail(rc, "member " + std::to_string(m) + ": " + g_create_error);
@@ -1895,16 +1895,24 @@ static void ens_enqueue_step(mhd3d_ensemble* e, int p0)
             // plane means and the relaxation pass of the members whose
             // fluxes() need them, on the register the stage reads (a
             // failed launch shows in the caller's hipGetLastError)
+            // (the mixed form: stage 1 reads the FP64 state, stages 2-3 the
+            // FP32 copies of the stage values)
             for (int m = 0; m < e->n; ++m) {
                 mhd3d_ctx* c = e->mem[m];
-                if (needs_prepass(c))
-                    (void)run_prepass_t<double>(c, c->U[p0 ^ e->off[m] ^ ((k - 1) & 1)], k, e->stream);
+                if (!needs_prepass(c)) continue;
+                if (e->mixed && k >= 2) (void)run_prepass_t<float>(c, c->U32[k - 2], k, e->stream);
+                else if (e->mixed) (void)run_prepass_t<double>(c, c->U[0], k, e->stream);
+                else (void)run_prepass_t<double>(c, c->U[p0 ^ e->off[m] ^ ((k - 1) & 1)], k, e->stream);
             }
         }
-        if (e->mixed)
+        if (e->mixed) {
             mixed_ens_stage(k, e->waves, e->sched, args, e->d_items, e->nitems, e->grid_int, e->d_walls,
                             e->nwalls, e->grid_wall, e->stream);
-        else ens_stage<double>(e, k, args);
+            ext_ens_stage_s32(k, args, e->d_items_x, e->nitems_x, e->grid_x, e->d_walls_x, e->nwalls_x,
+                              e->grid_wall_x, e->stream);
+        } else {
+            ens_stage<double>(e, k, args);
+        }
     }
     ensemble_finish_kernel<<<e->n, 256, 0, e->stream>>>(e->d_fin);
 }
@@ -1952,19 +1960,22 @@ static int ens_step_finish(mhd3d_ensemble* e, int nsteps)
         mhd3d_ctx* c = e->mem[m];
         const int64_t taken = (int64_t)(e->last[m].nit - e->last[m].nit_call0);
         c->dt_ready = taken == nsteps;
-        if (e->mixed) continue;
-        const int cur0 = e->parity ^ (nsteps & 1) ^ e->off[m];   // the register at the call's start
-        c->cur = cur0 ^ (int)(taken & 1);
+        if (!e->mixed) {
+            const int cur0 = e->parity ^ (nsteps & 1) ^ e->off[m];   // the register at the call's start
+            c->cur = cur0 ^ (int)(taken & 1);
+        }
         if ((c->xa.flags & EXT_RLAX) && (e->last[m].halted & HALT_EXIT)) {
             // stopped at the reference's exit point: the state is the input
-            // of the stage that exited (stage 2 reads what stage 1 wrote)
+            // of the stage that exited (stage 2 reads what stage 1 wrote;
+            // the mixed form keeps the step's start in its one FP64 register)
             RelaxState rs;
             ENSCHK(e, hipMemcpy(&rs, c->rs, sizeof(RelaxState), hipMemcpyDeviceToHost));
             // (off[] below keeps the flip for the member's later calls)
-            if (!c->relax_exit_seen && rs.exit_stage == 2) c->cur ^= 1;
+            if (!e->mixed && !c->relax_exit_seen && rs.exit_stage == 2) c->cur ^= 1;
             c->relax_exit_seen = true;
             c->relax_exit_code = (int)rs.exited;
         }
+        if (e->mixed) continue;
         const int off = e->parity ^ c->cur;
         if (off != e->off[m]) {
             e->off[m] = off;
@@ -2092,9 +2103,19 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
     const int p_save = e->parity;
     std::vector<char> ready_save(e->n);
     std::vector<int> cur_save(e->n);
+    // RLAX members: the relaxation's statics (TFAC, the temperature range,
+    // the call count) and the host's record of an exit
+    std::vector<RelaxState> rs_save(e->n);
+    std::vector<char> exit_seen_save(e->n);
+    std::vector<int> exit_code_save(e->n);
     for (int m = 0; m < e->n; ++m) {
-        ready_save[m] = e->mem[m]->dt_ready;
-        cur_save[m] = e->mem[m]->cur;
+        mhd3d_ctx* c = e->mem[m];
+        ready_save[m] = c->dt_ready;
+        cur_save[m] = c->cur;
+        exit_seen_save[m] = c->relax_exit_seen;
+        exit_code_save[m] = c->relax_exit_code;
+        if (c->xa.flags & EXT_RLAX)
+            ENSCHK(e, hipMemcpy(&rs_save[m], c->rs, sizeof(RelaxState), hipMemcpyDeviceToHost));
     }
     DevScalars* ds_snap = nullptr;
     auto cleanup = [&]() {
@@ -2124,7 +2145,14 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
         for (int m = 0; m < e->n; ++m)
             ENSCHK(e, hipMemcpyAsync(e->mem[m]->ds, ds_snap + m, sizeof(DevScalars), hipMemcpyDeviceToDevice, e->stream));
         ENSCHK(e, hipStreamSynchronize(e->stream));
-        for (int m = 0; m < e->n; ++m) e->mem[m]->dt_ready = ready_save[m] != 0;
+        for (int m = 0; m < e->n; ++m) {
+            mhd3d_ctx* c = e->mem[m];
+            c->dt_ready = ready_save[m] != 0;
+            c->relax_exit_seen = exit_seen_save[m] != 0;
+            c->relax_exit_code = exit_code_save[m];
+            if (c->xa.flags & EXT_RLAX)
+                ENSCHK(e, hipMemcpy(c->rs, &rs_save[m], sizeof(RelaxState), hipMemcpyHostToDevice));
+        }
         return MHD3D_OK;
     };
     const int wv[2] = {1, 2};
@@ -2146,10 +2174,14 @@ extern "C" int mhd3d_ensemble_autotune(mhd3d_ensemble* e, int steps, double* rep
                 e->chunk = chs[ic];
                 rc = ens_build(e);
                 if (rc) break;
+                // a member that reaches an RLAX exit point stops (the trial
+                // goes on with the others; the restore below undoes it)
                 rc = mhd3d_ensemble_step(e, 1);   // warm-up (and the TSTEP pass if needed)
+                if (rc == MHD3D_EEXIT) rc = MHD3D_OK;
                 if (rc) break;
                 (void)hipEventRecord(t0, e->stream);
                 rc = mhd3d_ensemble_step(e, steps);
+                if (rc == MHD3D_EEXIT) rc = MHD3D_OK;
                 (void)hipEventRecord(t1, e->stream);
                 (void)hipEventSynchronize(t1);
                 float ms = 0.f;
@@ -2316,10 +2348,6 @@ extern "C" int mhd3d_server_submit(mhd3d_server* s, const mhd3d_desc* desc, cons
     if (desc->precision != s->precision) { s->err = "request precision differs from the server's"; return MHD3D_EINVAL; }
     if (timedep_flags(desc)) {
         s->err = kTimedepWhy;
-        return MHD3D_EUNSUPPORTED;
-    }
-    if (desc->precision == MHD3D_MIXED && ext_flags(desc)) {
-        s->err = kMixedBranchWhy;
         return MHD3D_EUNSUPPORTED;
     }
     ServerReq r;

@@ -47,7 +47,8 @@ void mixed_ens_stage(int stage, int waves, int sched, const StageArgs* args, con
                      int nitems, int grid_int, const WallItem* walls, int nwalls, int grid_wall,
                      hipStream_t s)
 {
-    if (waves == 1) {
+    if (nitems == 0) {
+    } else if (waves == 1) {
         if (stage == 1) ens_launch<1, 1>(sched, args, items, nitems, grid_int, s);
         else if (stage == 2) ens_launch<2, 1>(sched, args, items, nitems, grid_int, s);
         else ens_launch<3, 1>(sched, args, items, nitems, grid_int, s);
@@ -57,7 +58,8 @@ void mixed_ens_stage(int stage, int waves, int sched, const StageArgs* args, con
         else ens_launch<3, 2>(sched, args, items, nitems, grid_int, s);
     }
     const dim3 wblk(64, 4, 1);
-    if (stage == 1) ensemble_wall_kernel<float, 1><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
+    if (nwalls == 0) {
+    } else if (stage == 1) ensemble_wall_kernel<float, 1><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
     else if (stage == 2) ensemble_wall_kernel<float, 2><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
     else ensemble_wall_kernel<float, 3><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
 }

@@ -28,7 +28,8 @@ void mixed_wall(bool mag, bool rot, int stage, const StageArgs& a, dim3 grid, di
 const void* mixed_ens_stage_fn(int waves, int sched);
 
 // One ensemble stage: the persistent interior launch over `items` (grid_int
-// blocks) and the persistent wall launch over `walls` (grid_wall blocks).
+// blocks) and the persistent wall launch over `walls` (grid_wall blocks); an
+// empty list launches nothing.
 void mixed_ens_stage(int stage, int waves, int sched, const StageArgs* args, const WorkItem* items,
                      int nitems, int grid_int, const WallItem* walls, int nwalls, int grid_wall,
                      hipStream_t s);

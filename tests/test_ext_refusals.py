@@ -4,8 +4,9 @@ without a device).
 
 Refused: iD outside 0..2, missing coordinate arrays, RLAX without the z
 rank layout of the reference run its scan reproduces (relax_nPEz >= 2
-dividing nPz, at least 3 planes per rank, 3dmhdsub.h:354-421), and the
-branches in the mixed form of an ensemble (single mixed domains take them;
+dividing nPz, at least 3 planes per rank, 3dmhdsub.h:354-421), and in
+ensembles of either form the heat loss and the plume with TC != 0, whose
+factors of TIMT the host forms before every step (3dmhdsub.h:446-460,752-757;
 ensembles check their members before any device call).
 """
 import ctypes as C
@@ -45,18 +46,20 @@ def test_rlax_needs_the_rank_layout():
         assert rc == L.EINVAL and "relax_nPEz" in msg, P
 
 
-def test_mixed_ensemble_refuses_branches():
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ensemble_refuses_time_dependent_branches(precision):
     if not os.path.exists(L.LIB_PATH):
         pytest.skip("libmhd3d.so not built")
-    cfg = _base(precision=1, iD=1)
+    cfg = _base(precision=precision, iTCon=2, TIMI=0.001)
     par = cfg.par.copy()
-    par[55] = 0.5
+    par[11], par[12], par[13], par[14], par[15] = 0.5, 0.002, 1.2, 0.9, 0.6   # heat loss, TC != 0
+    par[52], par[54] = 0.01, 0.8
     desc = Descriptor(dataclasses.replace(cfg, par=par))
     arr = (C.c_void_p * 1)(C.cast(desc.ptr(), C.c_void_p))
     h = C.c_void_p()
     rc = L.lib().mhd3d_ensemble_create(arr, 1, C.byref(h))
     msg = (L.lib().mhd3d_ensemble_last_error(None) or b"").decode()
-    assert rc == L.EUNSUPPORTED and "mixed" in msg and not h.value
+    assert rc == L.EUNSUPPORTED and "TIMT" in msg and not h.value
 
 
 def test_bad_iD_and_missing_coordinates():

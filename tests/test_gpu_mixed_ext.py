@@ -10,7 +10,10 @@ after 10 CFL steps every field within 1e-5 of the oracle
 stage values gives ~1e-7), for iD 1 and 2, lRem, lShr (hydro and MHD), lShr
 with lRem, the iTCon 2 heat loss with its time factor, the TP plume, and
 RLAX (both wall settings, with lRem and MHD); RLAX's statics follow the
-oracle's (the call count exactly, TFAC to 1e-12).  Under lShr the walls
+oracle's (the call count exactly, TFAC to 1e-12).  At an RLAX exit point in a
+stage 1 or a stage 2 the run stops with MHD3D_EEXIT and keeps the state of
+the step's start, its one FP64 register (the stage values live only in the
+FP32 copies): bit-identical to the same run advanced up to that step.  Under lShr the walls
 drive a shear flow (|RV| ~ 10, stored in FP32 as it is: only RO, TT and B
 are stored as deviations from a background) next to |RU| ~ 1e-3, so RU's
 error relative to its own maximum is bounded by 5e-5 there (measured
@@ -18,11 +21,13 @@ error relative to its own maximum is bounded by 5e-5 there (measured
 """
 import dataclasses
 
+import numpy as np
 import pytest
 
 import test_gpu_ext as tx
 import test_gpu_relax as tr
 from gpu_common import relerr
+from mhd3d import _lib as L
 from mhd3d.desc import Descriptor
 from mhd3d.domain import Domain
 from mhd3d.initial import stratified_state
@@ -75,3 +80,29 @@ def test_rlax_in_mixed_form(case, P):
     assert abs(g["TFAC"] - c["TFAC"]) <= 1e-12 * abs(c["TFAC"])
     dom.close()
     orc.close()
+
+
+@pytest.mark.parametrize("shift", [0, 2])
+def test_rlax_exit_in_mixed_form(shift):
+    """shift 0: the exit falls on the sixth step's stage 1; shift 2 (two
+    fluxes() calls first): on its stage 2."""
+    print(f"mixed rlax exit shift={shift}", flush=True)
+    cfg = dataclasses.replace(tr._cfg((24, 20, 12), "it0_mhd", TIMI=25.0 - 4.5 * tr.DT), precision=1)
+    desc = Descriptor(cfg)
+    st = stratified_state(desc, seed=5)
+    dom, ref = Domain(desc), Domain(desc)
+    for d in (dom, ref):
+        d.set_state(st)
+        for _ in range(shift):
+            d.fluxes()
+    ref.step(5, dt=tr.DT)
+    with pytest.raises(L.MhdError) as ei:
+        dom.step(10, dt=tr.DT)
+    assert ei.value.code == L.EEXIT and dom.clock()[0] == 5
+    g = dom.relax_state()
+    assert g["exited"] == 1 and g["ICALL"] == shift + 5 * 3 + (2 if shift else 1)
+    want = ref.state()
+    for k, v in dom.state().items():
+        assert np.array_equal(v.view(np.int64), want[k].view(np.int64)), (shift, k)
+    dom.close()
+    ref.close()
