@@ -225,10 +225,12 @@ int mhd3d_diagnostics(mhd3d_ctx* ctx, double out[3]);
  * *icall = ICALL (fluxes() calls so far), *exited = 1 after the reference's
  * exit point (then every step call returns MHD3D_EEXIT and the state is that
  * at the start of the step in which the exit fell, or, for an exit in a
- * later stage, that stage's input), else 0.  A context starts as a fresh
- * process does: TFAC = 1, TTM_MAX = -1e99, TTM_MIN = 1e99, ICALL = 0; every
- * fluxes() evaluation (each RK stage, mhd3d_fluxes) advances it as the
- * reference's call does. */
+ * later stage of the FP64 form, that stage's input), else 0.  A context
+ * starts as a fresh process does: TFAC = 1, TTM_MAX = -1e99, TTM_MIN = 1e99,
+ * ICALL = 0; every fluxes() evaluation (each RK stage, mhd3d_fluxes)
+ * advances it as the reference's call does.  mhd3d_set_state and
+ * mhd3d_init_stratified clear the exit (the domain runs again); the other
+ * statics persist, as a process's function statics do. */
 int mhd3d_relax_state(mhd3d_ctx* ctx, double out[3], int64_t* icall, int* exited);
 
 /* Simulation clock: nIt, TIMT, TIMC (3dmhdsub.h:150-152). */
@@ -276,7 +278,7 @@ const char* mhd3d_ensemble_last_error(const mhd3d_ensemble* e);   /* e == NULL: 
 int mhd3d_ensemble_size(const mhd3d_ensemble* e);
 int64_t mhd3d_ensemble_cells(const mhd3d_ensemble* e);
 /* fields as for mhd3d_set_state / mhd3d_get_state, for member `member`;
- * set_state also resets that member's clock */
+ * set_state also resets that member's clock (its DT guard stays) */
 int mhd3d_ensemble_set_state(mhd3d_ensemble* e, int member, const double* const* fields);
 int mhd3d_ensemble_get_state(mhd3d_ensemble* e, int member, double* const* fields);
 /* nsteps RK3 steps of every member, DT from each member's CFL (TSTEP) */

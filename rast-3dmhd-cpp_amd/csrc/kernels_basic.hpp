@@ -1113,6 +1113,15 @@ MHD_TU_LOCAL __global__ void relax_stop_kernel(DevScalars* ds, const RelaxState*
     if (rs->exited) ds->halted |= HALT_EXIT;
 }
 
+// A new state runs again: the exit record is cleared (the other statics --
+// TFAC, the temperature range, the call count -- persist, as the function
+// statics of a process do).
+MHD_TU_LOCAL __global__ void relax_clear_exit_kernel(RelaxState* rs)
+{
+    rs->exited = 0u;
+    rs->exit_stage = 0u;
+}
+
 // Non-finite scan of a register (interior incl. walls): one thread per
 // (x, y) column, looping over z.
 MHD_TU_LOCAL __global__ void __launch_bounds__(256) nonfinite_kernel(Geom g, int nf,

@@ -682,6 +682,7 @@ extern "C" int mhd3d_set_state(mhd3d_ctx* ctx, const double* const* fields)
                                    hipMemcpyDeviceToDevice, ctx->stream));
     }
     clear_halt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);   // a new state runs again
+    if (ctx->rs) relax_clear_exit_kernel<<<1, 1, 0, ctx->stream>>>(ctx->rs);
     ctx->relax_exit_seen = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -730,6 +731,7 @@ extern "C" int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const d
                                    hipMemcpyDeviceToDevice, ctx->stream));
     }
     clear_halt_kernel<<<1, 1, 0, ctx->stream>>>(ctx->ds);   // a new state runs again
+    if (ctx->rs) relax_clear_exit_kernel<<<1, 1, 0, ctx->stream>>>(ctx->rs);
     ctx->relax_exit_seen = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1847,8 +1849,11 @@ extern "C" int mhd3d_ensemble_set_state(mhd3d_ensemble* e, int member, const dou
     int rc = mhd3d_set_state(c, fields);
     if (rc) return ens_fail(e, rc, c->err);
     // a fresh state needs its TSTEP (set_state cleared the member's
-    // dt_ready); reset the member's clock
+    // dt_ready); reset the member's clock, keep its DT guard
+    DevScalars was{};
+    ENSCHK(e, hipMemcpy(&was, c->ds, sizeof(was), hipMemcpyDeviceToHost));
     DevScalars z{};
+    z.dt_min = was.dt_min;
     ENSCHK(e, hipMemcpy(c->ds, &z, sizeof(z), hipMemcpyHostToDevice));
     return MHD3D_OK;
 }

@@ -10,7 +10,8 @@ oracle's.  The exit points: a restart 4.5 DT before TIMT = 25 stops with
 MHD3D_EEXIT in the sixth step's first fluxes() call (the state is that
 after five steps), and -- with the call count shifted by mhd3d_fluxes --
 in the second stage (the state is then the stage-1 output); other walls
-stop as soon as the relaxation starts.  FP64 form: bit identity, 1e-12
+stop as soon as the relaxation starts; set_state clears an exit and the
+domain runs on with the statics it has.  FP64 form: bit identity, 1e-12
 stated as the tolerance.
 """
 import dataclasses
@@ -190,6 +191,29 @@ def test_fluxes_call_at_the_exit_point():
     orc.set_time(0.0, 0.0)
     orc.fluxes()
     assert orc.relax_state()["exited"] == 1 and dom.relax_state()["exited"] == 1
+
+
+def test_set_state_after_an_exit_runs_again():
+    """set_state clears the exit record and keeps the other statics: after a
+    fresh run's first fluxes() call exits, the next call (ICALL 1, no exit
+    point) returns the reference's second call bit for bit."""
+    print("relax set_state after an exit", flush=True)
+    desc, dom, orc = _pair(_cfg((16, 12, 12), "it0", TIMI=0.0, TSTART=0.0), seed=9)
+    st = dom.state()
+    with pytest.raises(L.MhdError) as ei:
+        dom.fluxes()
+    assert ei.value.code == L.EEXIT and dom.relax_state()["exited"] == 1
+    orc.set_time(0.0, 0.0)
+    orc.fluxes()
+    dom.set_state(st)
+    assert dom.relax_state()["exited"] == 0
+    fg = dom.fluxes()
+    orc.fluxes()
+    for name in L.FLUXES[: dom.nf]:
+        c = orc.get(name)
+        assert bit_identical_fraction(fg[name], c) == 1.0, (name, relerr(fg[name], c))
+    g, c = dom.relax_state(), orc.relax_state()
+    assert g["ICALL"] == c["ICALL"] == 2 and g["TFAC"] == c["TFAC"]
 
 
 def test_other_walls():

@@ -147,6 +147,24 @@ def test_dt_guard_in_an_ensemble_stops_one_member():
         assert _same(ens.state(i), alone.state()), i
 
 
+def test_ensemble_guard_survives_set_state():
+    """set_state resets a member's clock and keeps its DT guard: a guard set
+    before the states are loaded still halts member 0 after its first
+    step."""
+    from mhd3d.ensemble import Ensemble
+    from mhd3d.initial import stratified_state
+    descs = [Descriptor(stratified_config(24, 16, 9)), Descriptor(stratified_config(32, 24, 12, lMag=True))]
+    states = [stratified_state(d, seed=9 + k) for k, d in enumerate(descs)]
+    ens = Ensemble(descs)
+    ens.set_dt_guard(1.0, member=0)
+    for i, st in enumerate(states):
+        ens.set_state(i, st)
+    ens.step(3)
+    assert ens.halted(0) and not ens.halted(1)
+    assert ens.clock(0)[0] == 1 and ens.clock(1)[0] == 3
+    ens.close()
+
+
 def _spec(i, g, steps, lMag=False):
     return RunSpec(index=i, nPx=g[0], nPy=g[1], nPz=g[2], Ra=1e5, Pr=1.0, Pm=3.0, lMag=lMag,
                    lRot=True, seed=700 + i, steps=steps, arrival=0.0)
