@@ -34,6 +34,8 @@ def _cfgs(precision=0):
     out.append(tr._cfg((24, 20, 12), "it0_mhd", P=3))
     out.append(tr._cfg((20, 16, 12), "it1_rem", P=2))
     out.append(tr._cfg((24, 20, 12), "it0", TIMI=25.0 - 1e-9))      # exits in its second step
+    out.append(tx._cfg((17, 9, 11), "lRem1"))                          # partial tiles
+    out.append(tx._cfg((1, 40, 10), "iD1"))                            # one column in x
     out.append(stratified_config(32, 24, 10, Ra=5e4))
     out.append(stratified_config(28, 20, 12, lMag=True, lRot=True, Ra=1e5))
     return [dataclasses.replace(c, precision=precision) for c in out]

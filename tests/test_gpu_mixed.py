@@ -72,10 +72,13 @@ def test_mixed_vs_oracle_10_steps(case):
         assert e <= TOL_ORACLE_10, (case, name, e)
 
 
-@pytest.mark.parametrize("case", ["hydro", "mhd_rot"])
-def test_mixed_fixed_dt_odd_sizes(case):
-    desc, _, orc, st = make_pair((33, 17, 7), case, seed=19)
-    cfg = stratified_config(33, 17, 7, **CASES[case], precision=1)
+@pytest.mark.parametrize("case,n", [("hydro", (33, 17, 7)), ("mhd_rot", (33, 17, 7)), ("hydro", (1, 24, 12)),
+                                    ("mhd_rot", (1, 20, 9))])
+def test_mixed_fixed_dt_odd_sizes(case, n):
+    """Odd sizes and one column in x (nPx = 1: every x difference vanishes)."""
+    print(f"mixed odd sizes {case} {n}", flush=True)
+    desc, _, orc, st = make_pair(n, case, seed=19)
+    cfg = stratified_config(*n, **CASES[case], precision=1)
     dmx = Domain(Descriptor(cfg))
     dmx.set_state(st)
     dt = 1e-4

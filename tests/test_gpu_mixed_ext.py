@@ -8,7 +8,8 @@ background added back, so the branches run through the FP64 code.  Bar:
 after 10 CFL steps every field within 1e-5 of the oracle
 (max|g - c| / max|c| over the interior incl. walls; the FP32 rounding of the
 stage values gives ~1e-7), for iD 1 and 2, lRem, lShr (hydro and MHD), lShr
-with lRem, the iTCon 2 heat loss with its time factor, the TP plume, and
+with lRem (also at one column in x and at odd sizes with partial tiles),
+the iTCon 2 heat loss with its time factor, the TP plume, and
 RLAX (both wall settings, with lRem and MHD); RLAX's statics follow the
 oracle's (the call count exactly, TFAC to 1e-12).  At an RLAX exit point in a
 stage 1 or a stage 2 the run stops with MHD3D_EEXIT and keeps the state of
@@ -67,6 +68,15 @@ def _run(cfg, seed, what, tol=TOL):
 def test_branch_in_mixed_form(case):
     print(f"mixed branch {case}", flush=True)
     dom, orc = _run(tx._cfg((24, 20, 12), case), 7, case, TOL_SHEAR if case.startswith("lShr") else TOL)
+    dom.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("case,n", [("iD1", (1, 40, 10)), ("lRem1", (17, 9, 11)), ("lShr_mhd", (33, 7, 9))])
+def test_branch_in_mixed_form_odd_sizes(case, n):
+    """One column in x, and odd sizes that leave partial tiles in x and y."""
+    print(f"mixed branch {case} {n}", flush=True)
+    dom, orc = _run(tx._cfg(n, case), 13, (case, n), TOL_SHEAR if case.startswith("lShr") else TOL)
     dom.close()
     orc.close()
 
