@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mhd3d.h"
@@ -50,6 +51,7 @@ struct mhd3d_ctx {
     int cfl_parts_cap = 0;
     bool dt_ready = false;          // ds->dt holds TSTEP of the current state
     double* host_stage = nullptr;   // pinned, one field
+    hipEvent_t slab_ev[4] = {};     // last copy through each slab of host_stage
     hipStream_t stream = nullptr;
     int cur = 0;
     int device = 0;
@@ -506,6 +508,8 @@ extern "C" int mhd3d_destroy(mhd3d_ctx* ctx)
     (void)hipFree(ctx->ft);
     (void)hipFree(ctx->relax_buf);
     (void)hipFree(ctx->rs);
+    for (hipEvent_t& ev : ctx->slab_ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     if (ctx->hs) (void)hipHostFree(ctx->hs);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -533,38 +537,150 @@ extern "C" int mhd3d_geometry(const mhd3d_ctx* ctx, int32_t out[4])
 // ---------------------------------------------------------------------------
 // host <-> device field transfer
 
+// A field moves between the host array ([nz][ny][nx], ghosts included) and
+// the device layout ([nz][nPy][pitch]) through the pinned staging buffer in
+// kSlabs slabs of z planes: a team of host threads repacks one slab while
+// the copy of another is in flight, and each slab's event orders the reuse
+// of its part of the buffer.
+constexpr int kSlabs = 4;
+
+// fn(begin, end) over parts of [0, n) on up to 16 host threads (the CPU
+// share of one GPU on a shared host), at least `grain` items a thread.
+template <typename F>
+static void host_parallel(int64_t n, int64_t grain, const F& fn)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    int64_t T = hw ? std::min<unsigned>(hw, 16u) : 4;
+    T = std::min<int64_t>(T, (n + grain - 1) / grain);
+    if (T <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> team;
+    int64_t started = 1;
+    try {
+        for (; started < T; ++started) {
+            const int64_t t = started;
+            team.emplace_back([&fn, n, T, t] { fn(n * t / T, n * (t + 1) / T); });
+        }
+    } catch (...) {
+        // fewer threads than asked: this thread takes the parts left over
+    }
+    fn((int64_t)0, n / T);
+    for (int64_t t = started; t < T; ++t) fn(n * t / T, n * (t + 1) / T);
+    for (std::thread& th : team) th.join();
+}
+
+static int slab_events(mhd3d_ctx* ctx)
+{
+    for (hipEvent_t& ev : ctx->slab_ev)
+        if (!ev) HIPCHK(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    return MHD3D_OK;
+}
+
+// Planes [z0, z1) of slab s.
+static void slab_planes(const Geom& g, int s, int& z0, int& z1)
+{
+    const int zs = (g.nz + kSlabs - 1) / kSlabs;
+    z0 = std::min(g.nz, s * zs);
+    z1 = std::min(g.nz, z0 + zs);
+}
+
+// The host array src into the device field dst (interior x, y of every
+// plane; the stream orders the copies, the caller synchronizes it).
 static int upload_field(mhd3d_ctx* ctx, double* dst, const double* src)
 {
     const Geom& g = ctx->g;
     const int nx = g.nPx + 2, ny = g.nPy + 2;
     double* h = ctx->host_stage;
-    std::memset(h, 0, (size_t)g.fs * sizeof(double));
-    for (int z = 0; z < g.nz; ++z)
-        for (int y = 0; y < g.nPy; ++y)
-            std::memcpy(h + (size_t)z * g.plane + (size_t)y * g.pitch,
-                        src + ((size_t)z * ny + (y + 1)) * nx + 1, g.nPx * sizeof(double));
-    HIPCHK(ctx, hipMemcpyAsync(dst, h, (size_t)g.fs * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = slab_events(ctx);
+    if (rc) return rc;
+    for (int s = 0; s < kSlabs; ++s) {
+        int z0, z1;
+        slab_planes(g, s, z0, z1);
+        if (z0 >= z1) break;
+        HIPCHK(ctx, hipEventSynchronize(ctx->slab_ev[s]));   // the slab's last copy is done
+        host_parallel(z1 - z0, 1, [&](int64_t p0, int64_t p1) {
+            for (int64_t z = z0 + p0; z < z0 + p1; ++z)
+                for (int y = 0; y < g.nPy; ++y) {
+                    double* d = h + (size_t)z * g.plane + (size_t)y * g.pitch;
+                    std::memcpy(d, src + ((size_t)z * ny + (y + 1)) * nx + 1, g.nPx * sizeof(double));
+                    if (g.pitch > g.nPx) std::memset(d + g.nPx, 0, (g.pitch - g.nPx) * sizeof(double));
+                }
+        });
+        const size_t off = (size_t)z0 * g.plane;
+        HIPCHK(ctx, hipMemcpyAsync(dst + off, h + off, (size_t)(z1 - z0) * g.plane * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->slab_ev[s], ctx->stream));
+    }
     return MHD3D_OK;
 }
 
-static int download_field(mhd3d_ctx* ctx, double* dst, const double* src)
+// nf device fields from src (field stride fs) into the host arrays dst[f]:
+// the slab copies run ahead of the unpacking by one buffer.  periodic: the
+// x and y ghosts take the periodic copies (3dmhdmpi.h:65-68,118-121,135-142)
+// and every plane is written (the state); otherwise ghosts and the z ghost
+// planes are zero (fluxes).
+static int download_fields(mhd3d_ctx* ctx, double* const* dst, const double* src, int nf, bool periodic)
 {
     const Geom& g = ctx->g;
     const int nx = g.nPx + 2, ny = g.nPy + 2;
     double* h = ctx->host_stage;
-    HIPCHK(ctx, hipMemcpyAsync(h, src, (size_t)g.fs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    for (int z = 0; z < g.nz; ++z) {
-        for (int y = 0; y < g.nPy; ++y) {
-            double* row = dst + ((size_t)z * ny + (y + 1)) * nx;
-            std::memcpy(row + 1, h + (size_t)z * g.plane + (size_t)y * g.pitch, g.nPx * sizeof(double));
-            row[0] = row[nx - 2];            // 3dmhdmpi.h:118-121
-            row[nx - 1] = row[1];
+    int rc = slab_events(ctx);
+    if (rc) return rc;
+    int ns = 0;   // slabs with planes
+    for (int s = 0; s < kSlabs; ++s) {
+        int z0, z1;
+        slab_planes(g, s, z0, z1);
+        if (z0 < z1) ns = s + 1;
+    }
+    const int jobs = nf * ns;
+    auto issue = [&](int j) -> int {
+        const int f = j / ns, s = j % ns;
+        int z0, z1;
+        slab_planes(g, s, z0, z1);
+        const size_t off = (size_t)z0 * g.plane;
+        HIPCHK(ctx, hipMemcpyAsync(h + off, src + (size_t)f * g.fs + off, (size_t)(z1 - z0) * g.plane * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->slab_ev[s], ctx->stream));
+        return MHD3D_OK;
+    };
+    for (int j = 0; j < std::min(ns, jobs); ++j) {
+        rc = issue(j);
+        if (rc) return rc;
+    }
+    for (int j = 0; j < jobs; ++j) {
+        const int f = j / ns, s = j % ns;
+        int z0, z1;
+        slab_planes(g, s, z0, z1);
+        HIPCHK(ctx, hipEventSynchronize(ctx->slab_ev[s]));
+        double* out = dst[f];
+        host_parallel(z1 - z0, 1, [&](int64_t p0, int64_t p1) {
+            for (int64_t z = z0 + p0; z < z0 + p1; ++z) {
+                double* pl = out + (size_t)z * ny * nx;
+                if (!periodic && (z == 0 || z == g.nz - 1)) {
+                    std::memset(pl, 0, (size_t)ny * nx * sizeof(double));
+                    continue;
+                }
+                for (int y = 0; y < g.nPy; ++y) {
+                    double* row = pl + (size_t)(y + 1) * nx;
+                    std::memcpy(row + 1, h + (size_t)z * g.plane + (size_t)y * g.pitch, g.nPx * sizeof(double));
+                    row[0] = periodic ? row[nx - 2] : 0.0;
+                    row[nx - 1] = periodic ? row[1] : 0.0;
+                }
+                if (periodic) {
+                    std::memcpy(pl, pl + (size_t)(ny - 2) * nx, nx * sizeof(double));
+                    std::memcpy(pl + (size_t)(ny - 1) * nx, pl + (size_t)nx, nx * sizeof(double));
+                } else {
+                    std::memset(pl, 0, nx * sizeof(double));
+                    std::memset(pl + (size_t)(ny - 1) * nx, 0, nx * sizeof(double));
+                }
+            }
+        });
+        if (j + ns < jobs) {
+            rc = issue(j + ns);
+            if (rc) return rc;
         }
-        double* pl = dst + (size_t)z * ny * nx;
-        std::memcpy(pl, pl + (size_t)(ny - 2) * nx, nx * sizeof(double));   // :65-68,135-142
-        std::memcpy(pl + (size_t)(ny - 1) * nx, pl + (size_t)nx, nx * sizeof(double));
     }
     return MHD3D_OK;
 }
@@ -587,17 +703,20 @@ static int set_background(mhd3d_ctx* ctx, const double* const* fields)
     // means[k][z], k = RO, TT, BX, BY, BZ
     std::vector<double> mean[5];
     const int src_field[5] = {3, 4, 5, 6, 7};
-    for (int k = 0; k < 5; ++k) {
-        mean[k].assign(nz, 0.0);
-        if (k >= 2 && !mag) continue;
-        const double* F = fields[src_field[k]];
-        for (int z = 0; z < nz; ++z) {
+    const int nk = mag ? 5 : 2;
+    for (int k = 0; k < 5; ++k) mean[k].assign(nz, 0.0);
+    // one plane's sum on one thread, in a fixed order (the same doubles
+    // whatever the number of threads)
+    host_parallel((int64_t)nk * nz, 1, [&](int64_t j0, int64_t j1) {
+        for (int64_t j = j0; j < j1; ++j) {
+            const int k = (int)(j / nz), z = (int)(j % nz);
+            const double* F = fields[src_field[k]];
             double sum = 0.0;
             for (int y = 1; y <= g.nPy; ++y)
                 for (int x = 1; x <= g.nPx; ++x) sum += F[((size_t)z * ny + y) * nx + x];
             mean[k][z] = sum / ((double)g.nPx * g.nPy);
         }
-    }
+    });
     return set_background_profiles(ctx, mean);
 }
 
@@ -741,12 +860,9 @@ extern "C" int mhd3d_init_stratified(mhd3d_ctx* ctx, const double* TT_z, const d
 extern "C" int mhd3d_get_state(mhd3d_ctx* ctx, double* const* fields)
 {
     if (!ctx || !fields) return MHD3D_EINVAL;
-    for (int f = 0; f < ctx->nf; ++f) {
+    for (int f = 0; f < ctx->nf; ++f)
         if (!fields[f]) return fail(ctx, MHD3D_EINVAL, "null field pointer");
-        int rc = download_field(ctx, fields[f], ctx->U[ctx->cur] + (size_t)f * ctx->g.fs);
-        if (rc) return rc;
-    }
-    return MHD3D_OK;
+    return download_fields(ctx, fields, ctx->U[ctx->cur], ctx->nf, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1199,6 +1315,8 @@ extern "C" int mhd3d_stage(mhd3d_ctx* ctx, int stage, double dt)
 extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
 {
     if (!ctx || !flux) return MHD3D_EINVAL;
+    for (int f = 0; f < ctx->nf; ++f)
+        if (!flux[f]) return fail(ctx, MHD3D_EINVAL, "null flux pointer");
     const Geom& g = ctx->g;
     const size_t reg = (size_t)g.fs * ctx->nf * sizeof(double);
     if (!ctx->F) {
@@ -1228,19 +1346,9 @@ extern "C" int mhd3d_fluxes(mhd3d_ctx* ctx, double* const* flux)
     else if (rot) flux_only_kernel<false, true><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
     else flux_only_kernel<false, false><<<grd, blk, 0, ctx->stream>>>(a, ctx->F);
     HIPCHK(ctx, hipGetLastError());
-    const int nx = g.nPx + 2, ny = g.nPy + 2;
-    for (int f = 0; f < ctx->nf; ++f) {
-        if (!flux[f]) return fail(ctx, MHD3D_EINVAL, "null flux pointer");
-        double* h = ctx->host_stage;
-        HIPCHK(ctx, hipMemcpyAsync(h, ctx->F + (size_t)f * g.fs, (size_t)g.fs * sizeof(double),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        double* dst = flux[f];
-        std::memset(dst, 0, (size_t)g.nz * ny * nx * sizeof(double));
-        for (int z = 1; z <= g.nPz; ++z)
-            for (int y = 0; y < g.nPy; ++y)
-                std::memcpy(dst + ((size_t)z * ny + (y + 1)) * nx + 1,
-                            h + (size_t)z * g.plane + (size_t)y * g.pitch, g.nPx * sizeof(double));
+    {
+        int rc = download_fields(ctx, flux, ctx->F, ctx->nf, false);
+        if (rc) return rc;
     }
     if (ctx->xa.flags & EXT_RLAX) {
         // the reference's fluxes() ends in exit_err at an exit point: the
