@@ -95,8 +95,8 @@ typedef struct mhd3d_desc {
     double RM, ORM, OBETA;
     double TP, TB, TC;
     /* heat-flux relaxation (3dmhdsub.h:322-442): RLAX = par[59], TSTART =
-     * par[57], TOFF = par[58] (3dmhd.cpp:428-430); FP64 form, single domains,
-     * with ZEE.  The reference exits in the first fluxes() call of a step
+     * par[57], TOFF = par[58] (3dmhd.cpp:428-430); with ZEE and
+     * relax_nPEz.  The reference exits in the first fluxes() call of a step
      * that starts within DT after a multiple of 25 in TIMT (:427-439), so a
      * run can only use it from a restart; MHD3D_EEXIT reports that point. */
     double RLAX, TSTART, TOFF;
