@@ -1,7 +1,8 @@
 """The off-default branches on the GPU against the oracle, bit for bit.
 
 iD 1/2 (density diffusion, incl. the plane means it needs), lRem (the
-conduction of TT - TTM, its walls and CFL), lShr (the carried-over FT, the
+conduction of TT - TTM, its walls and CFL; also together with iD 1, both
+plane means in one pass), lShr (the carried-over FT, the
 moving walls), iTCon 2 (heat loss with and without its time factor) and the
 TP plume of iTCon 0/1 (3dmhdsub.h:229-326, 443-461, 720-873): 10 CFL steps,
 single stages, fluxes() and BCON through the C ABI, compared over the
@@ -37,7 +38,11 @@ def _cfg(n, case):
     cfg = stratified_config(*n, lMag=lMag, lRot=lMag, Ra=3e4)
     par = cfg.par.copy()
     kw = {}
-    if case.startswith("iD1"):
+    if case.startswith("iDrem"):                 # density diffusion and lRem: both plane means
+        kw = dict(iD=1, lRem=True, iTCon=1, izCon=1, TU=1.0, DZTU=-0.02, DZTB=0.03,
+                  kappa=_kappa(n[2] + 2))
+        par[55], par[56] = 0.5, 0.4
+    elif case.startswith("iD1"):
         kw = dict(iD=1)
         par[55], par[56] = 0.5, 0.4
     elif case.startswith("iD2"):
@@ -73,7 +78,7 @@ def _cfg(n, case):
 
 
 CASES = ["iD1", "iD2_mhd", "iDh_mhd", "lRem1", "lRem0", "lShr", "lShr_mhd", "lShrRem", "lShrRem_mhd",
-         "heatT", "heat0_mhd", "plume0", "plume1"]
+         "heatT", "heat0_mhd", "plume0", "plume1", "iDrem", "iDrem_mhd"]
 
 
 def _pair(n, case, seed=7):

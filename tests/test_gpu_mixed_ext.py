@@ -39,7 +39,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 TOL_SHEAR = 5e-5
 
-CASES = ["iD1", "iD2_mhd", "lRem1", "lShr", "lShr_mhd", "lShrRem", "heatT", "plume0"]
+CASES = ["iD1", "iD2_mhd", "lRem1", "lShr", "lShr_mhd", "lShrRem", "heatT", "plume0", "iDrem_mhd"]
 RLAX_CASES = [("it0", 2), ("it1_rem_mhd", 4)]
 
 

@@ -3,7 +3,8 @@
 3dmhdsub.h:322-442 for a reference run on relax_nPEz z ranks: the mean
 convective flux FCONM (one ordered plane sum per plane), the radiative flux
 FRADM, the per-rank CORRECT scans, the TFAC update and the FT term, with
-iTCon 0 / izCon 1 and iTCon 1 / izCon 0, hydro and MHD, with lRem and lShr;
+iTCon 0 / izCon 1 and iTCon 1 / izCon 0, hydro and MHD, with lRem, lShr and
+iD 1;
 10 CFL steps from a restart (TIMT past TSTART), single stages and fluxes()
 through the C ABI; the statics TFAC, TTM_MAX/MIN and ICALL equal the
 oracle's.  The exit points: a restart 4.5 DT before TIMT = 25 stops with
@@ -51,6 +52,9 @@ def _cfg(n, case, TIMI=3.7, TSTART=1.0, P=2):
         kw.update(lRem=True, TU=1.0, DZTU=-0.02, DZTB=0.03, kappa=_kappa(n[2] + 2))
     if "shr" in case:
         kw.update(lShr=True)
+    if "id" in case:                             # with density diffusion: RO and TT means
+        kw.update(iD=1)
+        par[55], par[56] = 0.5, 0.4
     return dataclasses.replace(cfg, par=par, TIMI=TIMI, relax_nPEz=P, **kw)
 
 
@@ -80,7 +84,7 @@ def _same_statics(dom, orc, what):
 
 
 CASES = [("it0", 2), ("it0", 4), ("it1", 2), ("it0_mhd", 3), ("it1_mhd", 2), ("it0_rem", 2),
-         ("it0_shr", 2), ("it1_rem_mhd", 4)]
+         ("it0_shr", 2), ("it1_rem_mhd", 4), ("it0_id_mhd", 2)]
 
 
 @pytest.mark.parametrize("case,P", CASES)
