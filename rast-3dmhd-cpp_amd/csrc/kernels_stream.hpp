@@ -306,6 +306,15 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             return zsrc[idx];
     };
     const int64_t fs = g.fs;
+    // Mixed form, stage 1: the update's Z-type operand is the FP64 state at
+    // the point, the very values the thread loaded for the stencil one
+    // iteration before.  They wait in a thread-private LDS ring of two planes
+    // instead of being read from HBM again (most of them have left L2 by
+    // then: that read doubled the stage's fetched bytes).  Only with the
+    // deferred update and two barriers, whose LDS leaves room for it.
+    constexpr bool ZLDS = !F64 && STAGE == 1 && DEFER && !ONE_BARRIER;
+    constexpr int ZRN = ZLDS ? 2 * NF * STX * STY : 1;
+    __shared__ double zring[ZRN];
     SAcc<T, MAG, RO_N> acc;
     T row[3] = {T(0), T(0), T(0)};
     T rinv[3] = {T(0), T(0), T(0)};
@@ -404,6 +413,19 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
         // -- derive plane L and publish it
         {
             const PrepPt<T> q = prep_point<T, MAG, TI>(own, a, L);
+            if constexpr (ZLDS) {
+                double* zr = &zring[(L & 1) * NF * STX * STY + t];
+                zr[0 * STX * STY] = own.ru;
+                zr[1 * STX * STY] = own.rv;
+                zr[2 * STX * STY] = own.rw;
+                zr[3 * STX * STY] = own.ro;
+                zr[4 * STX * STY] = own.tt;
+                if constexpr (MAG) {
+                    zr[5 * STX * STY] = own.bx;
+                    zr[6 * STX * STY] = own.by;
+                    zr[7 * STX * STY] = own.bz;
+                }
+            }
             const T ri = T(1) / q.ro;
             const T u = q.ru * ri, v = q.rv * ri, w = q.rw * ri;
             T* su = &s_uvw[L % NRING][0][0];
@@ -552,7 +574,12 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             if constexpr (DEFER) {
                 fpend = o;
                 zp = z;
-                if constexpr (!ZEARLY) {
+                if constexpr (ZLDS) {
+                    // plane z's values, stored by this thread when it derived plane z
+                    const double* zr = &zring[(z & 1) * NF * STX * STY + t];
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) zpend[f] = zr[f * STX * STY];
+                } else if constexpr (!ZEARLY) {
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
                         zpend[f] = ZLOAD ? zld(f * fs + (int64_t)z * g.plane + go) : uc[f];
