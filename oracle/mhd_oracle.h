@@ -25,8 +25,12 @@
  * What pins each function: the published iteration-9 answers
  * (tests/test_oracle_known_answers.py), closed forms on uniform and
  * stretched grids (tests/test_oracle_math.py, test_oracle_stretched.py,
- * test_oracle_ext.py), and planted mistakes those checks must catch
- * (tests/test_oracle_mutants.py, the ORC_MUTANT switches of mhd_oracle.c).
+ * test_oracle_ext.py, test_oracle_relax.py), and planted mistakes those
+ * checks must catch (tests/test_oracle_mutants.py, the ORC_MUTANT switches
+ * of mhd_oracle.c).  The lMag block is pinned by the continuum identities
+ * its fluxes must satisfy, not to the last bit: no published answer covers
+ * it, so the order of its terms within one output follows the reference's
+ * source by reading alone.
  */
 #ifndef MHD_ORACLE_H_
 #define MHD_ORACLE_H_
