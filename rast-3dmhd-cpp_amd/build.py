@@ -11,7 +11,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = [os.path.join(CSRC, f)
-           for f in ("mhd3d.hip", "mixed.hip", "mixed_ensemble.hip", "ext_stream.hip", "dumpio.cpp",
+           for f in ("mhd3d.hip", "mixed.hip", "mixed_ensemble.hip", "ext_stream.hip", "ext_mixed.hip", "dumpio.cpp",
                      "static_state.cpp")]
 OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "lib", "obj")

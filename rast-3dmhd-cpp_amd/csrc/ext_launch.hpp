@@ -1,5 +1,5 @@
 // Launches of the off-default interior stage kernels (FP64 fluxes), defined
-// in ext_stream.hip.  Each variant compiles in one set of branches (EXTM mask),
+// in ext_stream.hip (FP64 form) and ext_mixed.hip (mixed form).  Each variant compiles in one set of branches (EXTM mask),
 // so a run with one off-default switch does not carry the registers of the
 // others; a run whose interior switches fit no single set takes the variant
 // with all of them.  Every variant has one wave per SIMD, one barrier per
@@ -41,11 +41,13 @@ const void* ext_stream_fn(bool mag, int variant);
 void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, hipStream_t s,
                int target_blocks);
 
-// The mixed form with off-default branches: every branch compiled in, FP64
-// fluxes from the FP32 copies of the stage values (stage 1 from the FP64
-// state), the copies written as in the mixed form.
-const void* ext_stream_fn_s32(bool mag);
-void ext_stage_s32(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s, int target_blocks);
+// The mixed form with off-default branches (ext_mixed.hip): the branch set
+// of the variant compiled in, FP64 fluxes from the FP32 copies of the stage
+// values (stage 1 from the FP64 state), the copies written as in the mixed
+// form.
+const void* ext_stream_fn_s32(bool mag, int variant);
+void ext_stage_s32(bool mag, bool rot, int stage, int variant, const StageArgs& a, hipStream_t s,
+                   int target_blocks);
 
 struct WorkItem;
 struct WallItem;

@@ -1,15 +1,14 @@
-// The off-default interior stage kernels (FP64 fluxes): the streaming kernel
-// with the branches of one ExtVariant compiled in, and those of the mixed
-// form (single domains and ensemble members).  Kept in a translation unit of
-// its own so the 60 instantiations build in parallel with the main one; the
-// arithmetic is that of mhd3d.hip's kernels (no contraction).
+// The off-default interior stage kernels of the FP64 form: the streaming
+// kernel with the branches of one ExtVariant compiled in.  Kept in a
+// translation unit of its own so the 60 instantiations build in parallel
+// with the main one; the arithmetic is that of mhd3d.hip's kernels (no
+// contraction).
 #define MHD3D_SECONDARY_TU 1
 
 #include <hip/hip_runtime.h>
 
 #include "ext_launch.hpp"
 #include "kernels_basic.hpp"
-#include "kernels_ensemble.hpp"
 #include "kernels_stream.hpp"
 
 namespace mhd {
@@ -55,61 +54,7 @@ const void* fn_mask(bool mag)
                : (const void*)stage_stream_kernel<double, false, true, 2, 1, true, true, true, M>;
 }
 
-// The mixed form with branches: FP64 fluxes with every branch from the FP32
-// copies of the stage values (S32).
-template <bool MAG, bool ROT, int STAGE>
-void launch_s32(const StageArgs& a, hipStream_t s, int target_blocks)
-{
-    const StreamPlan p = stream_plan(a.g, target_blocks);
-    stage_stream_kernel<double, MAG, ROT, STAGE, 1, true, true, true, EXT_INTERIOR, true>
-        <<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
-}
-
-template <int STAGE>
-void stage_s32(bool mag, bool rot, const StageArgs& a, hipStream_t s, int tb)
-{
-    if (mag && rot) launch_s32<true, true, STAGE>(a, s, tb);
-    else if (mag) launch_s32<true, false, STAGE>(a, s, tb);
-    else if (rot) launch_s32<false, true, STAGE>(a, s, tb);
-    else launch_s32<false, false, STAGE>(a, s, tb);
-}
-
 }  // namespace
-
-const void* ext_stream_fn_s32(bool mag)
-{
-    return mag ? (const void*)stage_stream_kernel<double, true, true, 2, 1, true, true, true, EXT_INTERIOR, true>
-               : (const void*)stage_stream_kernel<double, false, true, 2, 1, true, true, true, EXT_INTERIOR, true>;
-}
-
-void ext_stage_s32(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s, int target_blocks)
-{
-    if (stage == 1) stage_s32<1>(mag, rot, a, s, target_blocks);
-    else if (stage == 2) stage_s32<2>(mag, rot, a, s, target_blocks);
-    else stage_s32<3>(mag, rot, a, s, target_blocks);
-}
-
-const void* ext_ens_fn_s32()
-{
-    return (const void*)ensemble_stage_ext_kernel<2, true>;
-}
-
-void ext_ens_stage_s32(int stage, const StageArgs* args, const WorkItem* items, int nitems, int grid_int,
-                       const WallItem* walls, int nwalls, int grid_wall, hipStream_t s)
-{
-    if (nitems > 0) {
-        const dim3 blk(STX, STY, 1);
-        if (stage == 1) ensemble_stage_ext_kernel<1, true><<<grid_int, blk, 0, s>>>(args, items, nitems);
-        else if (stage == 2) ensemble_stage_ext_kernel<2, true><<<grid_int, blk, 0, s>>>(args, items, nitems);
-        else ensemble_stage_ext_kernel<3, true><<<grid_int, blk, 0, s>>>(args, items, nitems);
-    }
-    if (nwalls > 0) {
-        const dim3 wblk(64, 4, 1);
-        if (stage == 1) ensemble_wall_kernel<double, 1, true, true><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
-        else if (stage == 2) ensemble_wall_kernel<double, 2, true, true><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
-        else ensemble_wall_kernel<double, 3, true, true><<<grid_wall, wblk, 0, s>>>(args, walls, nwalls);
-    }
-}
 
 const void* ext_stream_fn(bool mag, int variant)
 {

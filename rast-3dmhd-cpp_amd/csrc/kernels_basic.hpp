@@ -907,17 +907,54 @@ __device__ __forceinline__ double ordered_plane_sum(const Geom& g, V value, doub
     return s;
 }
 
+// Plane sum of the mixed form: each of the block's PT_T threads sums a fixed
+// strided subset of the plane in FP64, in order, then a fixed pairwise tree
+// combines the partial sums.  The same doubles on every run, alone or in an
+// ensemble, in microseconds instead of one dependent chain of nPx*nPy
+// additions; not the reference's sequential order, which the FP64 form keeps
+// for bit identity (the mixed form is held to 1e-5 per field, not to the
+// bit, and a tree of partial sums is the more accurate of the two).
+constexpr int PT_T = 256;
+
+template <typename V>
+__device__ __forceinline__ double tree_plane_sum(const Geom& g, V value, double* red)
+{
+    const int t = threadIdx.x;
+    const int nPx = g.nPx;
+    const int64_t N = (int64_t)nPx * g.nPy;
+    int y = t / nPx, x = t - (t / nPx) * nPx;
+    const int qy = PT_T / nPx, qx = PT_T - qy * nPx;
+    double s = 0.0;
+    for (int64_t f = t; f < N; f += PT_T) {
+        s += value(y, x);
+        x += qx;
+        y += qy;
+        if (x >= nPx) {
+            x -= nPx;
+            ++y;
+        }
+    }
+    red[t] = s;
+    __syncthreads();
+    for (int w = PT_T / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
 // Horizontal means of RO (blockIdx.y = 0) and TT (1) on every plane of
 // register U, into out[k * nz + z], divided by float(nPx*nPy) as there.  For
 // RO the z ghost planes take the planes two in, which fluxes() copies there
 // before taking the mean (3dmhdsub.h:230-239).  TI = float: U is the mixed
 // form's FP32 copy, whose values are deviations from bg64 (added back).
-template <typename TI>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-plane_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64, int want_ro,
-                  int want_tt, double* __restrict__ out)
+// SUM(g, value) sums the plane: the reference's order (FP64 form) or the
+// tree of the mixed form.
+template <typename TI, typename SUM>
+__device__ __forceinline__ void plane_mean_body(const Geom& g, const TI* __restrict__ U,
+                                                const double* __restrict__ bg64, int want_ro,
+                                                int want_tt, double* __restrict__ out, SUM sum)
 {
-    __shared__ __align__(16) double buf[2][PM_G * 64];
     const int z = blockIdx.x, k = blockIdx.y;
     if (k == 0 ? !want_ro : !want_tt) return;
     int zs = z;
@@ -925,16 +962,35 @@ plane_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ b
     const TI* __restrict__ p = U + (int64_t)(3 + k) * g.fs + (int64_t)zs * g.plane;
     double b = 0.0;
     if constexpr (std::is_same<TI, float>::value) b = bg64[(k == 0 ? BG64_R0 : BG64_T0) * g.nz + zs];
-    const double s = ordered_plane_sum(
-        g,
-        [&](int y, int x) -> double {
-            if constexpr (std::is_same<TI, float>::value)
-                return double(p[(int64_t)y * g.pitch + x]) + b;
-            else
-                return p[(int64_t)y * g.pitch + x];
-        },
-        buf);
+    const double s = sum(g, [&](int y, int x) -> double {
+        if constexpr (std::is_same<TI, float>::value)
+            return double(p[(int64_t)y * g.pitch + x]) + b;
+        else
+            return p[(int64_t)y * g.pitch + x];
+    });
     if (threadIdx.x == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
+}
+
+// One wave per plane, the reference's summation order.
+template <typename TI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+plane_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64, int want_ro,
+                  int want_tt, double* __restrict__ out)
+{
+    __shared__ __align__(16) double buf[2][PM_G * 64];
+    plane_mean_body(g, U, bg64, want_ro, want_tt, out,
+                    [&](const Geom& gg, auto value) { return ordered_plane_sum(gg, value, buf); });
+}
+
+// PT_T threads per plane, the tree of the mixed form.
+template <typename TI>
+__global__ void __launch_bounds__(PT_T)
+plane_mean_tree_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64, int want_ro,
+                       int want_tt, double* __restrict__ out)
+{
+    __shared__ double red[PT_T];
+    plane_mean_body(g, U, bg64, want_ro, want_tt, out,
+                    [&](const Geom& gg, auto value) { return tree_plane_sum(gg, value, red); });
 }
 
 // FCONM of the RLAX relaxation (3dmhdsub.h:349-353): horizontal_mean of the
@@ -945,13 +1001,14 @@ plane_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ b
 // left there at :267.  Products in the reference's order (func_mul2,
 // func_mul3: ((a*b)*c)*s).  Skipped while the domain is stopped when
 // `stopped` is set (a stage), evaluated regardless otherwise (mhd3d_fluxes).
-template <typename TI>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-conv_flux_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64,
-                      const double* __restrict__ ttm, int shr, const DevScalars* __restrict__ ds,
-                      int stopped, double* __restrict__ fconm)
+// SUM as for plane_mean_body.
+template <typename TI, typename SUM>
+__device__ __forceinline__ void conv_flux_mean_body(const Geom& g, const TI* __restrict__ U,
+                                                    const double* __restrict__ bg64,
+                                                    const double* __restrict__ ttm, int shr,
+                                                    const DevScalars* __restrict__ ds, int stopped,
+                                                    double* __restrict__ fconm, SUM sum)
 {
-    __shared__ __align__(16) double buf[2][PM_G * 64];
     if (stopped && ds->halted) return;
     const int z = blockIdx.x + 1;
     const int64_t o = (int64_t)z * g.plane;
@@ -967,7 +1024,7 @@ conv_flux_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict
         r0 = bg64[BG64_R0 * g.nz + z];
         t0 = bg64[BG64_T0 * g.nz + z];
     }
-    auto conv = [&](int y, int x) -> double {
+    const double s = sum(g, [&](int y, int x) -> double {
         const int64_t i = (int64_t)y * g.pitch + x;
         double r = double(ro[i]), t = double(tt[i]);
         if constexpr (std::is_same<TI, float>::value) {
@@ -983,9 +1040,30 @@ conv_flux_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict
         v += ((w * vv) * vv) * -0.5;
         v += ((w * ww) * ww) * -0.5;
         return v;
-    };
-    const double s = ordered_plane_sum(g, conv, buf);
+    });
     if (threadIdx.x == 0) fconm[z] = s / (double)(float)(g.nPx * g.nPy);
+}
+
+template <typename TI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+conv_flux_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64,
+                      const double* __restrict__ ttm, int shr, const DevScalars* __restrict__ ds,
+                      int stopped, double* __restrict__ fconm)
+{
+    __shared__ __align__(16) double buf[2][PM_G * 64];
+    conv_flux_mean_body(g, U, bg64, ttm, shr, ds, stopped, fconm,
+                        [&](const Geom& gg, auto value) { return ordered_plane_sum(gg, value, buf); });
+}
+
+template <typename TI>
+__global__ void __launch_bounds__(PT_T)
+conv_flux_mean_tree_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64,
+                           const double* __restrict__ ttm, int shr, const DevScalars* __restrict__ ds,
+                           int stopped, double* __restrict__ fconm)
+{
+    __shared__ double red[PT_T];
+    conv_flux_mean_body(g, U, bg64, ttm, shr, ds, stopped, fconm,
+                        [&](const Geom& gg, auto value) { return tree_plane_sum(gg, value, red); });
 }
 
 // The statics of fluxes() that the relaxation keeps between calls

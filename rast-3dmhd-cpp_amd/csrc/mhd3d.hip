@@ -339,7 +339,7 @@ static int variant_resident(const mhd3d_ctx* ctx, int w, int sc)
     const bool mag = ctx->d.flag_lMag != 0;
     const void* fn;
     if (ctx->mixed && ctx->ext)
-        fn = ext_stream_fn_s32(mag);
+        fn = ext_stream_fn_s32(mag, ext_variant(ctx->xa.flags));
     else if (ctx->mixed)
         fn = mixed_stream_fn(mag, w, sc);
     else if (ext_variant(ctx->xa.flags) != EXTV_NONE)
@@ -913,7 +913,7 @@ static void launch_stage_t(mhd3d_ctx* ctx, const StageArgs& a)
     dim3 blk(64, 4, 1);
     const int fm = ctx->mixed ? 1 : 0, ph = MAG ? 1 : 0;
     if (ctx->mixed && ctx->ext) {
-        ext_stage_s32(MAG, ROT, STAGE, a, ctx->stream, ctx->target_blocks);
+        ext_stage_s32(MAG, ROT, STAGE, ext_variant(ctx->xa.flags), a, ctx->stream, ctx->target_blocks);
     } else if (ctx->mixed) {
         mixed_stage(MAG, ROT, STAGE, a, ctx->stream, ctx->tune_waves[fm][ph], ctx->tune_sched[fm][ph],
                     ctx->target_blocks);
@@ -999,23 +999,36 @@ static bool needs_prepass(const mhd3d_ctx* ctx)
 // it stays zero), then for RLAX the mean convective flux FCONM and the
 // relaxation's scalar pass (TFAC, CORRECT, the exit test).  stage = 1..3
 // inside a step (a stopped domain skips it), 0 for mhd3d_fluxes.
+// The plane sums follow the reference's order (one dependent chain per
+// plane) in the FP64 form and in mhd3d_fluxes; inside the mixed form's
+// steps a fixed-order tree of partial sums (tree_plane_sum).
 template <typename TI>
 static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t stream)
 {
     const int f = ctx->xa.flags;
     const int want_ro = (f & (EXT_DH | EXT_DV)) ? 1 : 0;
     const int want_tt = ((f & (EXT_REM | EXT_RLAX)) && !(f & EXT_SHR)) ? 1 : 0;
+    const bool tree = ctx->mixed && stage > 0;
     if (want_ro || want_tt) {
-        plane_mean_kernel<TI><<<dim3(ctx->g.nz, 2, 1), 64, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro,
-                                                                        want_tt, ctx->ext_buf);
+        const dim3 grd(ctx->g.nz, 2, 1);
+        if (tree)
+            plane_mean_tree_kernel<TI><<<grd, PT_T, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro, want_tt,
+                                                                 ctx->ext_buf);
+        else
+            plane_mean_kernel<TI><<<grd, 64, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro, want_tt,
+                                                          ctx->ext_buf);
         HIPCHK(ctx, hipGetLastError());
     }
     if (f & EXT_RLAX) {
         const int nz = ctx->g.nz;
         double* fconm = ctx->relax_buf;
-        conv_flux_mean_kernel<TI><<<ctx->g.nPz, 64, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
-                                                                 (f & EXT_SHR) ? 1 : 0, ctx->ds,
-                                                                 stage ? 1 : 0, fconm);
+        const int shr = (f & EXT_SHR) ? 1 : 0;
+        if (tree)
+            conv_flux_mean_tree_kernel<TI><<<ctx->g.nPz, PT_T, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
+                                                                            shr, ctx->ds, 1, fconm);
+        else
+            conv_flux_mean_kernel<TI><<<ctx->g.nPz, 64, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
+                                                                     shr, ctx->ds, stage ? 1 : 0, fconm);
         HIPCHK(ctx, hipGetLastError());
         relax_kernel<<<1, 1, 0, stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
                                          ctx->relax_buf + nz, ctx->relax_buf + 2 * (size_t)nz + 1,

@@ -4,7 +4,9 @@ oracle (FP64).
 With a branch switched on, the mixed form keeps its storage -- FP32 copies of
 the stage values (deviations from the FP64 background of each plane), FP64
 RK registers -- and evaluates the fluxes in FP64 from those copies with the
-background added back, so the branches run through the FP64 code.  Bar:
+background added back, so the branches run through the FP64 code; the plane
+means inside a step are fixed-order trees of partial sums instead of the
+reference's one chain (the same doubles alone and in an ensemble).  Bar:
 after 10 CFL steps every field within 1e-5 of the oracle
 (max|g - c| / max|c| over the interior incl. walls; the FP32 rounding of the
 stage values gives ~1e-7), for iD 1 and 2, lRem, lShr (hydro and MHD), lShr
