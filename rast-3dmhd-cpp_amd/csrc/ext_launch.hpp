@@ -19,6 +19,7 @@ enum ExtVariant : int {
     EXTV_SHR = 3,    // lShr carried-over FT
     EXTV_HEAT = 4,   // iTCon 2 heat loss
     EXTV_ALL = 5,    // any combination
+    EXTV_RLAX = 6,   // the RLAX correction of FT
 };
 
 // The narrowest variant whose branch set holds the interior flags.
@@ -30,6 +31,7 @@ inline int ext_variant(int flags)
     if (f == EXT_REM) return EXTV_REM;
     if (f == EXT_SHR) return EXTV_SHR;
     if (f == EXT_HEAT) return EXTV_HEAT;
+    if (f == EXT_RLAX) return EXTV_RLAX;
     return EXTV_ALL;
 }
 

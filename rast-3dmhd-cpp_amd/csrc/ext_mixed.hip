@@ -21,7 +21,7 @@ namespace {
 
 // interior branch sets by ExtVariant (EXTV_NONE: a run whose branches act on
 // the walls only still evaluates its fluxes from the FP32 copies in FP64)
-constexpr int kMaskS32[6] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR};
+constexpr int kMaskS32[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR, EXT_RLAX};
 
 template <bool MAG, bool ROT, int STAGE, int M>
 void launch_s32(const StageArgs& a, hipStream_t s, int target_blocks)
@@ -70,6 +70,7 @@ const void* ext_stream_fn_s32(bool mag, int variant)
     case EXTV_REM: return fn_s32<kMaskS32[EXTV_REM]>(mag);
     case EXTV_SHR: return fn_s32<kMaskS32[EXTV_SHR]>(mag);
     case EXTV_HEAT: return fn_s32<kMaskS32[EXTV_HEAT]>(mag);
+    case EXTV_RLAX: return fn_s32<kMaskS32[EXTV_RLAX]>(mag);
     default: return fn_s32<kMaskS32[EXTV_ALL]>(mag);
     }
 }
@@ -83,6 +84,7 @@ void ext_stage_s32(bool mag, bool rot, int stage, int variant, const StageArgs& 
     case EXTV_REM: stage_s32<kMaskS32[EXTV_REM]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_SHR: stage_s32<kMaskS32[EXTV_SHR]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_HEAT: stage_s32<kMaskS32[EXTV_HEAT]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_RLAX: stage_s32<kMaskS32[EXTV_RLAX]>(mag, rot, stage, a, s, target_blocks); break;
     default: stage_s32<kMaskS32[EXTV_ALL]>(mag, rot, stage, a, s, target_blocks); break;
     }
 }

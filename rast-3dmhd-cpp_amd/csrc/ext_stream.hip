@@ -15,7 +15,7 @@ namespace mhd {
 
 namespace {
 
-constexpr int kMask[6] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR};
+constexpr int kMask[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR, EXT_RLAX};
 
 template <bool MAG, bool ROT, int STAGE, int M>
 void launch_m(const StageArgs& a, hipStream_t s, int target_blocks)
@@ -63,6 +63,7 @@ const void* ext_stream_fn(bool mag, int variant)
     case EXTV_REM: return fn_mask<kMask[EXTV_REM]>(mag);
     case EXTV_SHR: return fn_mask<kMask[EXTV_SHR]>(mag);
     case EXTV_HEAT: return fn_mask<kMask[EXTV_HEAT]>(mag);
+    case EXTV_RLAX: return fn_mask<kMask[EXTV_RLAX]>(mag);
     default: return fn_mask<kMask[EXTV_ALL]>(mag);
     }
 }
@@ -75,6 +76,7 @@ void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, h
     case EXTV_REM: stage_mask<kMask[EXTV_REM]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_SHR: stage_mask<kMask[EXTV_SHR]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_HEAT: stage_mask<kMask[EXTV_HEAT]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_RLAX: stage_mask<kMask[EXTV_RLAX]>(mag, rot, stage, a, s, target_blocks); break;
     default: stage_mask<kMask[EXTV_ALL]>(mag, rot, stage, a, s, target_blocks); break;
     }
 }
