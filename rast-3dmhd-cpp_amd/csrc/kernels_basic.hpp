@@ -943,6 +943,333 @@ __device__ __forceinline__ double tree_plane_sum(const Geom& g, V value, double*
     return red[0];
 }
 
+// The sequential sum of ordered_plane_sum -- the same double, bit for bit --
+// by a block of PW_T threads.  While the running sum s stays inside one
+// binade, |s| in (2^e, 2^(e+1)), every value it takes is a multiple of that
+// binade's grid u = 2^(e-52): s = M*u with the integer M in (2^52, 2^53).
+// Adding v then gives M' = M + RNE(v/u) (the exact sum's nearest grid
+// point) unless v/u is a halfway case k + 1/2, where the tie goes to the
+// even one of M + k and M + k + 1.  So the running value is M plus a prefix
+// sum of integer increments, with one bit of sequential state, the parity of
+// M: a halfway summand leaves M even, any other flips the parity by its
+// increment's.  A round takes the next PW_C summands, PW_R consecutive ones
+// per thread, forms their increments, runs the parity through the block
+// (within a wave by ballots, across waves from each wave's published map
+// and its totals for either incoming parity), and accepts the summands in
+// order up to the first one with |v| >= 2^e or whose prefix P takes M + P
+// out of the open binade; that one is added in FP64 (s' = u*(M + P) + v, the
+// chain's own rounding) and the next round starts after it.  The increments
+// and their sums are integers held in doubles: the accepted prefixes lie in
+// an interval of width 2^52 and |increment| <= 2^52 + 1, so every partial
+// sum formed before the first rejected summand is exact, and so is that
+// summand's own range test (rounding is monotone and the bounds are
+// doubles); sums after it may round and are never used.  A plane of
+// positive values (RO, TT) needs one FP64 addition per binade its sum
+// crosses: ~11 for 512 x 512 summands.  The first PW_K0 summands, a zero or
+// non-normal running sum, and planes that keep leaving the binade (sums of
+// both signs near zero) are walked as the plain chain, a resident chunk at
+// a time.
+//
+// The plane streams through LDS as chunks of PW_C summands in a ring of
+// PW_SLOTS chunks: at a round starting in chunk a, chunks a .. a+2 are
+// resident; the round loads chunk a+3 into registers first and stores it
+// after its own work, in the slot of chunk a-1.  The chunk after the
+// plane's last is stored as zeros, summands that add nothing, so a round
+// reads its PW_C summands unchecked.  Ring index r of summand f is
+// f mod PW_RING, stored at r + r/8 so that the threads' strided reads of 8
+// consecutive summands hit distinct banks.  value(y, x) forms the summand of
+// interior point (x, y); the sum is returned to every thread.
+constexpr int PW_T = 256;
+constexpr int PW_W = PW_T / 64;
+constexpr int PW_R = 8;
+constexpr int PW_C = PW_T * PW_R;
+constexpr int PW_SLOTS = 4;
+constexpr int PW_RING = PW_SLOTS * PW_C;
+constexpr int PW_LDS = PW_RING + PW_RING / 8;
+constexpr int PW_K0 = 128;
+
+struct PwShared {
+    double ring[PW_LDS];
+    double tot[2][PW_W];       // each wave's total of increments for an incoming parity 0, 1
+    int64_t brk_f[PW_W];       // each wave's first rejected summand (INT64_MAX: none)
+    double brk_s[PW_W];        // the running sum after it
+    unsigned par[PW_W];        // each wave's parity map: bit 0 passes the incoming parity
+                               // through, bit 1 is xor-ed onto it (or replaces it)
+};
+
+__device__ __forceinline__ int pw_slot(int64_t f)
+{
+    const int r = (int)(f & (PW_RING - 1));
+    return r + (r >> 3);
+}
+
+// Parity of an integer-valued double below 2^53 in magnitude.
+__device__ __forceinline__ int pw_odd(double k)
+{
+    const double h = k * 0.5;
+    return h != floor(h) ? 1 : 0;
+}
+
+// DPP move of a double (both halves), 0 where the source lane is outside
+// the pattern or its row is masked off.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double pw_dpp(double x)
+{
+    const uint64_t b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Exclusive prefix sum over the 64 lanes of a wave: row_shr 1, 2, 4, 8
+// within rows of 16, row_bcast 15 (rows 1, 3) and 31 (rows 2, 3) for the
+// inclusive sum, then wave_shr 1.  incl receives the inclusive sum.
+__device__ __forceinline__ double pw_wave_scan(double x, double& incl)
+{
+    x += pw_dpp<0x111, 0xf>(x);
+    x += pw_dpp<0x112, 0xf>(x);
+    x += pw_dpp<0x114, 0xf>(x);
+    x += pw_dpp<0x118, 0xf>(x);
+    x += pw_dpp<0x142, 0xa>(x);
+    x += pw_dpp<0x143, 0xc>(x);
+    incl = x;
+    return pw_dpp<0x138, 0xf>(x);
+}
+
+// s += the summands [p, end) in order, from the ring (every thread alike).
+__device__ __forceinline__ double pw_chain(const double* ring, double s, int64_t p, int64_t end)
+{
+    for (; p + 16 <= end; p += 16) {
+        double r[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) r[i] = ring[pw_slot(p + i)];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s += r[i];
+    }
+    for (; p < end; ++p) s += ring[pw_slot(p)];
+    return s;
+}
+
+template <typename V>
+__device__ __forceinline__ double walk_plane_sum(const Geom& g, V value, PwShared& sh)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nPx = g.nPx;
+    const int64_t N = (int64_t)nPx * g.nPy;
+    const int64_t nch = (N + PW_C - 1) / PW_C;
+    // (y, x) of this thread's summand in the next chunk to load: steps of
+    // PW_T summands within a chunk, PW_C from one chunk to the next
+    const int qy = PW_T / nPx, qx = PW_T - qy * nPx;
+    const int cy = PW_C / nPx, cx = PW_C - cy * nPx;
+    int ly = t / nPx, lx = t - (t / nPx) * nPx;
+    int64_t lf = t;
+    double pre[PW_R];
+    auto load_next = [&]() {
+        int y = ly, x = lx;
+        int64_t f = lf;
+#pragma unroll
+        for (int j = 0; j < PW_R; ++j) {
+            pre[j] = f < N ? value(y, x) : 0.0;
+            f += PW_T;
+            x += qx;
+            y += qy;
+            if (x >= nPx) {
+                x -= nPx;
+                ++y;
+            }
+        }
+        lf += PW_C;
+        lx += cx;
+        ly += cy;
+        if (lx >= nPx) {
+            lx -= nPx;
+            ++ly;
+        }
+    };
+    // chunk c from registers, or zeros past the last
+    auto store_chunk = [&](int64_t c) {
+        const bool z = c >= nch;
+#pragma unroll
+        for (int j = 0; j < PW_R; ++j) sh.ring[pw_slot(c * PW_C + t + j * PW_T)] = z ? 0.0 : pre[j];
+    };
+    for (int c = 0; c < 3; ++c) {
+        if (c < nch) load_next();
+        store_chunk(c);
+    }
+    int64_t hi = 2;   // highest chunk stored in the ring
+    __syncthreads();
+    int64_t p = N < PW_K0 ? N : PW_K0;
+    double s = pw_chain(sh.ring, 0.0, 0, p);
+    int breaks = 0;
+    const int64_t budget = 64 + N / 4096;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    while (p < N) {
+        const int64_t a = p / PW_C;
+        const bool fetch = hi == a + 2 && a + 3 <= nch;
+        if (fetch && a + 3 < nch) load_next();
+        const double as = fabs(s);
+        if (!(as >= 0x1p-1021 && as < __builtin_inf()) || breaks >= budget) {
+            const int64_t end = (a + 1) * PW_C < N ? (a + 1) * PW_C : N;
+            s = pw_chain(sh.ring, s, p, end);
+            p = end;
+            __syncthreads();
+        } else {
+            int e;
+            (void)frexp(as, &e);
+            e -= 1;   // |s| in [2^e, 2^(e+1))
+            const double lim = ldexp(1.0, e);
+            const double u = ldexp(1.0, e - 52);
+            const double M = ldexp(s, 52 - e);   // s / u, |M| in [2^52, 2^53)
+            // accepted prefixes P: M + P in (2^52, 2^53), or its negative
+            const double plo = s > 0.0 ? 0x1p52 - M : -0x1p53 - M;
+            const double phi = s > 0.0 ? 0x1p53 - M : -0x1p52 - M;
+            const int64_t f0 = p + (int64_t)t * PW_R;
+            double v[PW_R], q[PW_R];
+            unsigned flag = 0;       // summands rejected
+            unsigned halfm = 0;      // halfway cases
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < PW_R; ++i) {
+                v[i] = sh.ring[pw_slot(f0 + i)];
+                const double sc = ldexp(v[i], 52 - e);
+                const double r = rint(sc);
+                const bool half = fabs(sc - r) == 0.5;
+                if (!(fabs(v[i]) < lim)) flag |= 1u << i;
+                if (half) halfm |= 1u << i;
+                acc += half ? sc - 0.5 : r;   // the floor for a halfway case
+                q[i] = acc;
+            }
+            // The parity of the running integer M flips by the parity of a sum
+            // of increments, so a thread without halfway cases passes on its
+            // incoming parity flipped by that of its total.  With them (rare
+            // but for planes of equal or dyadic values): the first one's
+            // rounding bit is the incoming parity ^ g2, added as bf once the
+            // parity is known; it leaves M even, so each later one's bit is
+            // the parity of the increments since the tie before it, and the
+            // parity passed on is that of the increments after the last tie.
+            const bool seen = halfm != 0;
+            unsigned after = 0;      // summands from the first halfway case on
+            int g2 = 0, gam;
+            if (!seen) {
+                gam = pw_odd(acc);
+            } else {
+                const int fh = __builtin_ctz(halfm);
+                after = ~0u << fh;
+                double q0L = 0.0, add = 0.0;   // the prefix at the last tie; later rounding bits
+#pragma unroll
+                for (int i = 0; i < PW_R; ++i) {
+                    const double q0 = q[i];
+                    if (i == fh) {
+                        g2 = pw_odd(q0);
+                        q0L = q0;
+                    } else if (i > fh && ((halfm >> i) & 1u)) {
+                        add += (double)pw_odd(q0 - q0L);
+                        q0L = q0;
+                    }
+                    q[i] = q0 + add;
+                }
+                gam = pw_odd(acc - q0L);
+                acc = q[PW_R - 1];
+            }
+            // incoming parity of this lane: base ^ (dep & the wave's incoming parity)
+            const uint64_t Rm = __ballot(seen);
+            const uint64_t Xb = __ballot(gam);
+            const uint64_t Rb = Rm & below;
+            int base, dep;
+            if (Rb) {
+                const int r = 63 - __clzll((long long)Rb);
+                base = __popcll(Xb & below & ~((uint64_t(1) << r) - 1)) & 1;
+                dep = 0;
+            } else {
+                base = __popcll(Xb & below) & 1;
+                dep = 1;
+            }
+            const int b0 = base ^ g2;   // the first halfway case's bit for an incoming parity 0
+            double incl;
+            double E = pw_wave_scan(acc + (seen ? (double)b0 : 0.0), incl);
+            // for an incoming parity 1 the lanes with seen && dep add 1 - 2*b0 instead of b0
+            // (counts as signed ints: __popcll is unsigned)
+            const uint64_t up1 = __ballot(seen && dep && !b0), dn1 = __ballot(seen && dep && b0);
+            if (lane == 63) {
+                sh.tot[0][w] = incl;
+                sh.tot[1][w] = incl + (double)((int)__popcll(up1) - (int)__popcll(dn1));
+            }
+            if (lane == 0) {
+                int xw;
+                if (Rm) {
+                    const int r = 63 - __clzll((long long)Rm);
+                    xw = __popcll(Xb & ~((uint64_t(1) << r) - 1)) & 1;
+                } else {
+                    xw = __popcll(Xb) & 1;
+                }
+                sh.par[w] = (Rm ? 0u : 1u) | ((unsigned)xw << 1);
+            }
+            __syncthreads();
+            int pin = pw_odd(M), mine = 0;
+            double Ew = 0.0, T = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < PW_W; ++ww) {
+                if (ww == w) mine = pin;
+                const double tw = pin ? sh.tot[1][ww] : sh.tot[0][ww];
+                if (ww < w) Ew += tw;
+                T += tw;
+                const unsigned pm = sh.par[ww];
+                pin = (pm & 1u) ? (pin ^ (int)(pm >> 1)) : (int)(pm >> 1);
+            }
+            if (mine) E += (double)((int)__popcll(up1 & below) - (int)__popcll(dn1 & below));
+            E += Ew;
+            const double bf = seen ? (double)(base ^ (dep & mine) ^ g2) : 0.0;
+            const double lo = plo - E, up = phi - E;
+#pragma unroll
+            for (int i = 0; i < PW_R; ++i) {
+                q[i] += ((after >> i) & 1u) ? bf : 0.0;
+                if (!(q[i] > lo && q[i] < up)) flag |= 1u << i;
+            }
+            const uint64_t bal = __ballot(flag != 0);
+            if (bal) {
+                if (lane == __builtin_ctzll(bal)) {
+                    const int j = __builtin_ctz(flag);
+                    double before = E, vj = 0.0;
+#pragma unroll
+                    for (int i = 0; i < PW_R; ++i) {
+                        if (i < j) before = E + q[i];
+                        if (i == j) vj = v[i];
+                    }
+                    sh.brk_s[w] = (M + before) * u + vj;
+                    sh.brk_f[w] = f0 + j;
+                }
+            } else if (lane == 0) {
+                sh.brk_f[w] = INT64_MAX;
+            }
+            __syncthreads();
+            int64_t bfi = INT64_MAX;
+            double bs = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < PW_W; ++ww) {
+                const int64_t fw = sh.brk_f[ww];
+                if (fw < bfi) {
+                    bfi = fw;
+                    bs = sh.brk_s[ww];
+                }
+            }
+            if (bfi != INT64_MAX) {
+                s = bs;
+                p = bfi + 1;
+                ++breaks;
+            } else {
+                s = (M + T) * u;
+                p = p + PW_C < N ? p + PW_C : N;
+            }
+        }
+        // chunk a+3 into the slot of chunk a-1, read from the round at a+2 on
+        if (fetch) {
+            store_chunk(a + 3);
+            hi = a + 3;
+        }
+    }
+    return s;
+}
+
 // Horizontal means of RO (blockIdx.y = 0) and TT (1) on every plane of
 // register U, into out[k * nz + z], divided by float(nPx*nPy) as there.  For
 // RO the z ghost planes take the planes two in, which fluxes() copies there
@@ -971,15 +1298,15 @@ __device__ __forceinline__ void plane_mean_body(const Geom& g, const TI* __restr
     if (threadIdx.x == 0) out[k * g.nz + z] = s / (double)(float)(g.nPx * g.nPy);
 }
 
-// One wave per plane, the reference's summation order.
+// PW_T threads per plane, the reference's order (walk_plane_sum).
 template <typename TI>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-plane_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64, int want_ro,
-                  int want_tt, double* __restrict__ out)
+__global__ void __launch_bounds__(PW_T)
+plane_mean_walk_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64, int want_ro,
+                       int want_tt, double* __restrict__ out)
 {
-    __shared__ __align__(16) double buf[2][PM_G * 64];
+    __shared__ PwShared sh;
     plane_mean_body(g, U, bg64, want_ro, want_tt, out,
-                    [&](const Geom& gg, auto value) { return ordered_plane_sum(gg, value, buf); });
+                    [&](const Geom& gg, auto value) { return walk_plane_sum(gg, value, sh); });
 }
 
 // PT_T threads per plane, the tree of the mixed form.

@@ -999,9 +999,11 @@ static bool needs_prepass(const mhd3d_ctx* ctx)
 // it stays zero), then for RLAX the mean convective flux FCONM and the
 // relaxation's scalar pass (TFAC, CORRECT, the exit test).  stage = 1..3
 // inside a step (a stopped domain skips it), 0 for mhd3d_fluxes.
-// The plane sums follow the reference's order (one dependent chain per
-// plane) in the FP64 form and in mhd3d_fluxes; inside the mixed form's
-// steps a fixed-order tree of partial sums (tree_plane_sum).
+// The plane sums follow the reference's order in the FP64 form and in
+// mhd3d_fluxes: the RO and TT means by the exact block-parallel walk
+// (walk_plane_sum), the convective-flux means, sums of both signs near zero,
+// by one wave per plane (ordered_plane_sum); inside the mixed form's steps a
+// fixed-order tree of partial sums (tree_plane_sum).
 template <typename TI>
 static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t stream)
 {
@@ -1015,8 +1017,8 @@ static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t str
             plane_mean_tree_kernel<TI><<<grd, PT_T, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro, want_tt,
                                                                  ctx->ext_buf);
         else
-            plane_mean_kernel<TI><<<grd, 64, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro, want_tt,
-                                                          ctx->ext_buf);
+            plane_mean_walk_kernel<TI><<<grd, PW_T, 0, stream>>>(ctx->g, U, ctx->bg64, want_ro, want_tt,
+                                                                 ctx->ext_buf);
         HIPCHK(ctx, hipGetLastError());
     }
     if (f & EXT_RLAX) {
