@@ -832,77 +832,85 @@ MHD_TU_LOCAL __global__ void set_stop_kernel(DevScalars* s, double nit_stop)
 // horizontal_mean (3dmhdsub.h:698-718): the sum over a plane's interior in
 // the reference's order -- y outer, x inner, one running double -- so it
 // rounds exactly as there.  A sum in that order is one chain of dependent
-// additions, walked by one wave per plane (the planes in parallel).  The
-// wave streams the plane's nPx*nPy summands in that order as chunks of 64
-// (lane l forms summand 64c + l; a chunk may span two rows), PM_G chunks per
-// group with the next group's loads in flight, stages each group in LDS and
-// adds it from uniform-address LDS reads, which reach every lane at once:
-// the additions are the only serial work.  value(y, x) forms the summand of
-// interior point (x, y); the sum is returned to every lane.
-constexpr int PM_G = 8;
+// additions; here it is walked by wave 0 of the block, while PM_P producer
+// waves form the summands: the plane is a flat sequence of groups of
+// PM_G * 64 summands, producer wave w forms chunks w-1, w-1+PM_P, ... of 64
+// (lane l: summand 64c + l) of the next group into one half of an LDS double
+// buffer while wave 0 adds the current group from the other half, from
+// uniform-address LDS reads that reach every lane at once.  A summand that
+// costs loads and a division (the convective flux of RLAX) is then off the
+// chain's path: the additions are the only serial work.  value(y, x) forms
+// the summand of interior point (x, y); the sum is returned to wave 0.
+constexpr int PM_G = 16;
+constexpr int PM_P = 2;
+constexpr int PM_T = 64 * (1 + PM_P);
+static_assert(PM_G % PM_P == 0, "a producer's summands form one arithmetic progression");
 
 template <typename V>
 __device__ __forceinline__ double ordered_plane_sum(const Geom& g, V value, double (*buf)[PM_G * 64])
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nPx = g.nPx;
     const int64_t N = (int64_t)nPx * g.nPy;
-    // (y, x) of this lane's next summand f, advanced by 64 summands per chunk
-    int64_t f = lane;
-    int y = lane / nPx, x = lane - (lane / nPx) * nPx;
-    const int qy = 64 / nPx, qx = 64 - qy * nPx;
-    auto next = [&]() -> double {
-        const double v = f < N ? value(y, x) : 0.0;
-        f += 64;
-        x += qx;
-        y += qy;
-        if (x >= nPx) {
-            x -= nPx;
-            ++y;
-        }
-        return v;
-    };
-    double pre[PM_G];
-#pragma unroll
-    for (int d = 0; d < PM_G; ++d) pre[d] = next();
-    double s = 0.0;
     const int64_t ngroups = (N + PM_G * 64 - 1) / (PM_G * 64);
-    for (int64_t gi = 0; gi < ngroups; ++gi) {
+    // a producer lane's summands: f = 64 (w-1) + lane + 64 PM_P j, j = 0, 1, ...
+    // (with PM_G a multiple of PM_P the stride holds across groups)
+    const int f0 = 64 * (w > 0 ? w - 1 : 0) + lane;
+    int64_t f = f0;
+    int y = f0 / nPx, x = f0 - (f0 / nPx) * nPx;
+    const int qy = 64 * PM_P / nPx, qx = 64 * PM_P - qy * nPx;
+    auto produce = [&](int64_t gi) {
         double* b = buf[gi & 1];
 #pragma unroll
-        for (int d = 0; d < PM_G; ++d) b[d * 64 + lane] = pre[d];
-#pragma unroll
-        for (int d = 0; d < PM_G; ++d) pre[d] = next();
-        // group gi+2 writes this buffer after the next barrier, which every
-        // read below precedes
-        __syncthreads();
-        const int64_t base = gi * (PM_G * 64);
-        if (base + PM_G * 64 <= N) {
-            // batches of 16 values: the reads of batch j+1 are issued before
-            // the additions of batch j, so LDS latency stays hidden
-            const double2* b2 = reinterpret_cast<const double2*>(b);
-            constexpr int NB = PM_G * 64 / 16;
-            double2 r[2][8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) r[0][i] = b2[i];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                if (j + 1 < NB) {
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) r[(j + 1) & 1][i] = b2[(j + 1) * 8 + i];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    s += r[j & 1][i].x;
-                    s += r[j & 1][i].y;
-                }
-                __builtin_amdgcn_sched_barrier(0);
+        for (int d = 0; d < PM_G / PM_P; ++d) {
+            b[(d * PM_P + w - 1) * 64 + lane] = f < N ? value(y, x) : 0.0;
+            f += 64 * PM_P;
+            x += qx;
+            y += qy;
+            if (x >= nPx) {
+                x -= nPx;
+                ++y;
             }
-        } else {
-            const int n = (int)(N - base);
-            for (int j = 0; j < n; ++j) s += b[j];
         }
+    };
+    if (w > 0) produce(0);
+    __syncthreads();
+    double s = 0.0;
+    for (int64_t gi = 0; gi < ngroups; ++gi) {
+        if (w > 0) {
+            if (gi + 1 < ngroups) produce(gi + 1);
+        } else {
+            const double* b = buf[gi & 1];
+            const int64_t base = gi * (PM_G * 64);
+            if (base + PM_G * 64 <= N) {
+                // batches of 16 values: the reads of batch j+1 are issued
+                // before the additions of batch j, so LDS latency stays hidden
+                const double2* b2 = reinterpret_cast<const double2*>(b);
+                constexpr int NB = PM_G * 64 / 16;
+                double2 r[2][8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) r[0][i] = b2[i];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    if (j + 1 < NB) {
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) r[(j + 1) & 1][i] = b2[(j + 1) * 8 + i];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        s += r[j & 1][i].x;
+                        s += r[j & 1][i].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+                const int n = (int)(N - base);
+                for (int j = 0; j < n; ++j) s += b[j];
+            }
+        }
+        // the half just added is the one the producers fill next
+        __syncthreads();
     }
     return s;
 }
@@ -1371,8 +1379,9 @@ __device__ __forceinline__ void conv_flux_mean_body(const Geom& g, const TI* __r
     if (threadIdx.x == 0) fconm[z] = s / (double)(float)(g.nPx * g.nPy);
 }
 
+// PM_T threads per plane: wave 0 walks the chain, PM_P waves form the summands.
 template <typename TI>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(PM_T)
 conv_flux_mean_kernel(Geom g, const TI* __restrict__ U, const double* __restrict__ bg64,
                       const double* __restrict__ ttm, int shr, const DevScalars* __restrict__ ds,
                       int stopped, double* __restrict__ fconm)

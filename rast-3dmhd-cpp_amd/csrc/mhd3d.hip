@@ -1002,8 +1002,9 @@ static bool needs_prepass(const mhd3d_ctx* ctx)
 // The plane sums follow the reference's order in the FP64 form and in
 // mhd3d_fluxes: the RO and TT means by the exact block-parallel walk
 // (walk_plane_sum), the convective-flux means, sums of both signs near zero,
-// by one wave per plane (ordered_plane_sum); inside the mixed form's steps a
-// fixed-order tree of partial sums (tree_plane_sum).
+// by one chain per plane with its summands formed beside it
+// (ordered_plane_sum); inside the mixed form's steps a fixed-order tree of
+// partial sums (tree_plane_sum).
 template <typename TI>
 static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t stream)
 {
@@ -1029,7 +1030,7 @@ static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t str
             conv_flux_mean_tree_kernel<TI><<<ctx->g.nPz, PT_T, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
                                                                             shr, ctx->ds, 1, fconm);
         else
-            conv_flux_mean_kernel<TI><<<ctx->g.nPz, 64, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
+            conv_flux_mean_kernel<TI><<<ctx->g.nPz, PM_T, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
                                                                      shr, ctx->ds, stage ? 1 : 0, fconm);
         HIPCHK(ctx, hipGetLastError());
         relax_kernel<<<1, 1, 0, stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
