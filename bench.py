@@ -311,17 +311,46 @@ def measure_served(n=40, rate=50.0, quantum=10):
     return out
 
 
-def profile_run(steps, size=(512, 512, 256), mixed=False):
+def branch_config(name, size, precision=0):
+    """The branch workloads of the plane-sum kernels (stratified polytrope,
+    Ra 3e4): "iD1" hydro with density diffusion iD 1 (RO means); "lRem1"
+    hydro with the Rempel conduction, iTCon 1, izCon 1 (TT means); "rlax"
+    MHD+rotation with the RLAX relaxation, iTCon 0, izCon 1, two z ranks
+    (TT means, the convective-flux means, the relaxation pass)."""
+    import dataclasses
+    import numpy as np
+    from mhd3d.initial import stratified_config
+
+    mag = name == "rlax"
+    cfg = stratified_config(*size, lMag=mag, lRot=mag, Ra=3e4, precision=precision)
+    par = cfg.par.copy()
+    zs = np.linspace(0.0, 1.0, size[2] + 2)
+    if name == "iD1":
+        par[55], par[56] = 0.5, 0.4
+        return dataclasses.replace(cfg, par=par, iD=1)
+    if name == "lRem1":
+        return dataclasses.replace(cfg, par=par, lRem=True, iTCon=1, izCon=1, TU=1.0, DZTU=-0.02,
+                                   DZTB=0.03, kappa=(1.0 + 0.5 * zs ** 2, 1.0 * zs))
+    if name == "rlax":
+        par[57], par[58], par[59] = 1.0, 0.5, 0.2
+        return dataclasses.replace(cfg, par=par, iTCon=0, izCon=1, TIMI=3.7, relax_nPEz=2)
+    raise ValueError(f"unknown branch workload {name!r}")
+
+
+def profile_run(steps, size=(512, 512, 256), mixed=False, branch=None):
     """For rocprofv3: `steps` steps of the MHD+rotation workload at `size`
     (the headline workload at 512x512x256, Ra 1e6; Ra 1e5 at other sizes;
-    FP64 or the mixed form), then a plain device copy of 1 GiB whose
-    counters calibrate FETCH/WRITE_SIZE."""
+    FP64 or the mixed form) or of a branch workload (branch_config), then a
+    plain device copy of 1 GiB whose counters calibrate FETCH/WRITE_SIZE."""
     from mhd3d.desc import Descriptor
     from mhd3d.domain import Domain
     from mhd3d.initial import stratified_config
 
     ra = 1e6 if tuple(size) == (512, 512, 256) else 1e5
-    cfg = stratified_config(*size, Ra=ra, lMag=True, lRot=True, precision=1 if mixed else 0)
+    if branch:
+        cfg = branch_config(branch, tuple(size), precision=1 if mixed else 0)
+    else:
+        cfg = stratified_config(*size, Ra=ra, lMag=True, lRot=True, precision=1 if mixed else 0)
     dom = Domain(Descriptor(cfg))
     dom.init_stratified(seed=12345)
     dom.step(steps)
@@ -333,7 +362,8 @@ def profile_run(steps, size=(512, 512, 256), mixed=False):
     rc = L.lib().mhd3d_calibration_copy(1 << 30, 3, C.byref(ms))
     if rc != L.OK:
         raise L.MhdError(rc, "calibration copy failed")
-    log(f"profile run: {steps} steps of {'x'.join(map(str, size))} ({'mixed' if mixed else 'FP64'}), "
+    log(f"profile run: {steps} steps of {'x'.join(map(str, size))} ({'mixed' if mixed else 'FP64'}"
+        f"{', ' + branch if branch else ''}), "
         f"1 + 3 copies of {1 << 30} bytes, "
         f"{ms.value:.4f} ms per copy ({2 * (1 << 30) / ms.value / 1e6:.0f} GB/s)")
 
@@ -352,6 +382,8 @@ def main():
                     help="run STEPS steps and a calibration copy for rocprofv3, print nothing "
                          "(MHD+rotation at --size; --mixed for the mixed form)")
     ap.add_argument("--mixed", action="store_true", help="with --profile: the mixed form")
+    ap.add_argument("--branch", default=None, choices=["iD1", "lRem1", "rlax"],
+                    help="with --profile: a branch workload (plane means, the relaxation)")
     ap.add_argument("--ceiling", action="store_true", help="print the streaming ceiling only")
     args = ap.parse_args()
 
@@ -361,7 +393,7 @@ def main():
         sys.exit(2)
 
     if args.profile:
-        profile_run(args.profile, tuple(int(v) for v in args.size.split("x")), args.mixed)
+        profile_run(args.profile, tuple(int(v) for v in args.size.split("x")), args.mixed, args.branch)
         return
     if args.ceiling:
         print(json.dumps(measure_ceiling()), flush=True)

@@ -17,11 +17,21 @@ namespace {
 
 constexpr int kMask[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR, EXT_RLAX};
 
+// Schedule of each kernel: one barrier per plane (double-buffered planes)
+// or two (single-buffered).  MHD's double-buffered planes take 106-114 KB of
+// LDS, one block per CU; single-buffered, 76-82 KB, so the variants whose
+// registers allow two waves per SIMD run two blocks per CU: iD at every
+// stage, lRem, iTCon 2 and RLAX at stages 1-2.  The others stay register-
+// bound at one block per CU, where one barrier per plane is the faster.
+template <bool MAG, int STAGE, int M>
+constexpr bool kOneBarrier = !MAG || M == kMask[EXTV_SHR] || M == kMask[EXTV_ALL] ||
+                             (STAGE == 3 && M != kMask[EXTV_ID]);
+
 template <bool MAG, bool ROT, int STAGE, int M>
 void launch_m(const StageArgs& a, hipStream_t s, int target_blocks)
 {
     const StreamPlan p = stream_plan(a.g, target_blocks);
-    stage_stream_kernel<double, MAG, ROT, STAGE, 1, true, true, true, M>
+    stage_stream_kernel<double, MAG, ROT, STAGE, 1, kOneBarrier<MAG, STAGE, M>, true, true, M>
         <<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
 }
 
@@ -50,8 +60,8 @@ void stage_mask(bool mag, bool rot, int stage, const StageArgs& a, hipStream_t s
 template <int M>
 const void* fn_mask(bool mag)
 {
-    return mag ? (const void*)stage_stream_kernel<double, true, true, 2, 1, true, true, true, M>
-               : (const void*)stage_stream_kernel<double, false, true, 2, 1, true, true, true, M>;
+    return mag ? (const void*)stage_stream_kernel<double, true, true, 2, 1, kOneBarrier<true, 2, M>, true, true, M>
+               : (const void*)stage_stream_kernel<double, false, true, 2, 1, kOneBarrier<false, 2, M>, true, true, M>;
 }
 
 }  // namespace

@@ -383,10 +383,11 @@ struct CflPart {
     double bad;
 };
 
-// Block reduction of CflAcc into part[b] (wave shuffles, then LDS).
-__device__ __forceinline__ void cfl_block_reduce(CflAcc& a, CflPart* part, int b)
+// Block reduction of CflAcc into part[b] (wave shuffles, then LDS: sh,
+// 6 x 16 doubles, which a kernel may lend from its own buffer once every
+// thread is past its last use of it).
+__device__ __forceinline__ void cfl_block_reduce(CflAcc& a, CflPart* part, int b, double (*sh)[16])
 {
-    __shared__ double sh[6][16];
     for (int k = 0; k < 4; ++k) a.w[k] = wave_min(a.w[k]);
     a.um = wave_max(a.um);
     a.bad = wave_max(a.bad);
@@ -409,6 +410,12 @@ __device__ __forceinline__ void cfl_block_reduce(CflAcc& a, CflPart* part, int b
         part[b].um = a.um;
         part[b].bad = a.bad;
     }
+}
+
+__device__ __forceinline__ void cfl_block_reduce(CflAcc& a, CflPart* part, int b)
+{
+    __shared__ double sh[6][16];
+    cfl_block_reduce(a, part, b, sh);
 }
 
 // ---------------------------------------------------------------------------

@@ -626,7 +626,12 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
     const int ze = chunk == nch - 1 ? a.g.nPz : zb + zchunk;
     CflAcc cfl;   // stage 3: TSTEP of the next step on the new state
     stage_item<T, MAG, ROT, STAGE, ONE_BARRIER, DEFER, EXT, EXTM, S32>(a, tile_x, tile_y, zb, ze, lds, cfl);
-    if (STAGE == 3 && a.cfl_parts) cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x);
+    if (STAGE == 3 && a.cfl_parts) {
+        // the reduction borrows the plane buffers (a separate 768 B would
+        // take an iD variant's two blocks per CU over the CU's LDS)
+        __syncthreads();
+        cfl_block_reduce(cfl, a.cfl_parts, a.cfl_base + blockIdx.x, reinterpret_cast<double (*)[16]>(lds));
+    }
 }
 
 struct StreamPlan {
