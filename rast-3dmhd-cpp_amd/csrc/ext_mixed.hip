@@ -26,17 +26,19 @@ constexpr int kMaskS32[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT
 // Schedule of each kernel, as for the FP64 form's: two barriers per plane
 // (single-buffered planes, 76-82 KB of LDS instead of 106-114 KB) where the
 // registers then allow two MHD blocks per CU -- every stage of the wall-only,
-// iD, iTCon 2 and RLAX kernels, stages 1-2 of lRem -- and one barrier per
-// plane where registers hold them to one block anyway.
+// iD, iTCon 2, RLAX and lRem kernels -- and one barrier per plane where
+// registers hold them to one block anyway.
 template <bool MAG, int STAGE, int M>
-constexpr bool kOneBarrierS32 = !MAG || M == kMaskS32[EXTV_SHR] || M == kMaskS32[EXTV_ALL] ||
-                                (STAGE == 3 && M == kMaskS32[EXTV_REM]);
+constexpr bool kOneBarrierS32 = !MAG || M == kMaskS32[EXTV_SHR] || M == kMaskS32[EXTV_ALL];
+// lRem's stage 3 under a two-wave register bound (it needs 263 registers)
+template <bool MAG, int STAGE, int M>
+constexpr int kWavesS32 = (MAG && STAGE == 3 && M == kMaskS32[EXTV_REM]) ? 2 : 1;
 
 template <bool MAG, bool ROT, int STAGE, int M>
 void launch_s32(const StageArgs& a, hipStream_t s, int target_blocks)
 {
     const StreamPlan p = stream_plan(a.g, target_blocks);
-    stage_stream_kernel<double, MAG, ROT, STAGE, 1, kOneBarrierS32<MAG, STAGE, M>, true, true, M, true>
+    stage_stream_kernel<double, MAG, ROT, STAGE, kWavesS32<MAG, STAGE, M>, kOneBarrierS32<MAG, STAGE, M>, true, true, M, true>
         <<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
 }
 

@@ -17,21 +17,26 @@ namespace {
 
 constexpr int kMask[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR, EXT_RLAX};
 
-// Schedule of each kernel: one barrier per plane (double-buffered planes)
-// or two (single-buffered).  MHD's double-buffered planes take 106-114 KB of
-// LDS, one block per CU; single-buffered, 76-82 KB, so the variants whose
-// registers allow two waves per SIMD run two blocks per CU: iD at every
-// stage, lRem, iTCon 2 and RLAX at stages 1-2.  The others stay register-
-// bound at one block per CU, where one barrier per plane is the faster.
+// Schedule and register budget of each kernel: one barrier per plane
+// (double-buffered planes) or two (single-buffered).  MHD's double-buffered
+// planes take 106-114 KB of LDS, one block per CU; single-buffered, 76-82 KB,
+// so MHD kernels take two barriers where their registers then allow two
+// waves per SIMD: iD at every stage, lRem, iTCon 2 and RLAX at stages 1-2,
+// and the stage-3 kernels of those three under a two-wave register bound
+// (256 VGPRs, 36-76 B per lane spilled; measured 5-11% faster per step than
+// one block per CU without spills).  lShr and the all-branch combinations
+// stay register-bound at one block per CU, where one barrier per plane is
+// the faster.
 template <bool MAG, int STAGE, int M>
-constexpr bool kOneBarrier = !MAG || M == kMask[EXTV_SHR] || M == kMask[EXTV_ALL] ||
-                             (STAGE == 3 && M != kMask[EXTV_ID]);
+constexpr bool kOneBarrier = !MAG || M == kMask[EXTV_SHR] || M == kMask[EXTV_ALL];
+template <bool MAG, int STAGE, int M>
+constexpr int kWaves = (MAG && STAGE == 3 && M != kMask[EXTV_ID] && !kOneBarrier<MAG, STAGE, M>) ? 2 : 1;
 
 template <bool MAG, bool ROT, int STAGE, int M>
 void launch_m(const StageArgs& a, hipStream_t s, int target_blocks)
 {
     const StreamPlan p = stream_plan(a.g, target_blocks);
-    stage_stream_kernel<double, MAG, ROT, STAGE, 1, kOneBarrier<MAG, STAGE, M>, true, true, M>
+    stage_stream_kernel<double, MAG, ROT, STAGE, kWaves<MAG, STAGE, M>, kOneBarrier<MAG, STAGE, M>, true, true, M>
         <<<p.nblocks, dim3(STX, STY, 1), 0, s>>>(a, p.zchunk, p.nchunks, p.ntx, p.nty, p.xcd_remap);
 }
 
