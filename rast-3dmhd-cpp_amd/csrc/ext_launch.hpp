@@ -20,6 +20,7 @@ enum ExtVariant : int {
     EXTV_HEAT = 4,   // iTCon 2 heat loss
     EXTV_ALL = 5,    // any combination
     EXTV_RLAX = 6,   // the RLAX correction of FT
+    EXTV_IDREM = 7,  // density diffusion with lRem (both plane means)
 };
 
 // The narrowest variant whose branch set holds the interior flags.
@@ -32,6 +33,7 @@ inline int ext_variant(int flags)
     if (f == EXT_SHR) return EXTV_SHR;
     if (f == EXT_HEAT) return EXTV_HEAT;
     if (f == EXT_RLAX) return EXTV_RLAX;
+    if ((f & EXT_REM) && (f & ~(EXT_DH | EXT_DV | EXT_REM)) == 0) return EXTV_IDREM;
     return EXTV_ALL;
 }
 

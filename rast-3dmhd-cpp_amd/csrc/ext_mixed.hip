@@ -21,7 +21,8 @@ namespace {
 
 // interior branch sets by ExtVariant (EXTV_NONE: a run whose branches act on
 // the walls only still evaluates its fluxes from the FP32 copies in FP64)
-constexpr int kMaskS32[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR, EXT_RLAX};
+constexpr int kMaskS32[8] = {0,        EXT_DH | EXT_DV, EXT_REM,  EXT_SHR,
+                             EXT_HEAT, EXT_INTERIOR,    EXT_RLAX, EXT_DH | EXT_DV | EXT_REM};
 
 // Schedule of each kernel, as for the FP64 form's: two barriers per plane
 // (single-buffered planes, 76-82 KB of LDS instead of 106-114 KB) where the
@@ -82,6 +83,7 @@ const void* ext_stream_fn_s32(bool mag, int variant)
     case EXTV_SHR: return fn_s32<kMaskS32[EXTV_SHR]>(mag);
     case EXTV_HEAT: return fn_s32<kMaskS32[EXTV_HEAT]>(mag);
     case EXTV_RLAX: return fn_s32<kMaskS32[EXTV_RLAX]>(mag);
+    case EXTV_IDREM: return fn_s32<kMaskS32[EXTV_IDREM]>(mag);
     default: return fn_s32<kMaskS32[EXTV_ALL]>(mag);
     }
 }
@@ -96,6 +98,7 @@ void ext_stage_s32(bool mag, bool rot, int stage, int variant, const StageArgs& 
     case EXTV_SHR: stage_s32<kMaskS32[EXTV_SHR]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_HEAT: stage_s32<kMaskS32[EXTV_HEAT]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_RLAX: stage_s32<kMaskS32[EXTV_RLAX]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_IDREM: stage_s32<kMaskS32[EXTV_IDREM]>(mag, rot, stage, a, s, target_blocks); break;
     default: stage_s32<kMaskS32[EXTV_ALL]>(mag, rot, stage, a, s, target_blocks); break;
     }
 }

@@ -15,7 +15,8 @@ namespace mhd {
 
 namespace {
 
-constexpr int kMask[7] = {0, EXT_DH | EXT_DV, EXT_REM, EXT_SHR, EXT_HEAT, EXT_INTERIOR, EXT_RLAX};
+constexpr int kMask[8] = {0,        EXT_DH | EXT_DV, EXT_REM,      EXT_SHR,
+                          EXT_HEAT, EXT_INTERIOR,    EXT_RLAX,     EXT_DH | EXT_DV | EXT_REM};
 
 // Schedule and register budget of each kernel: one barrier per plane
 // (double-buffered planes) or two (single-buffered).  MHD's double-buffered
@@ -79,6 +80,7 @@ const void* ext_stream_fn(bool mag, int variant)
     case EXTV_SHR: return fn_mask<kMask[EXTV_SHR]>(mag);
     case EXTV_HEAT: return fn_mask<kMask[EXTV_HEAT]>(mag);
     case EXTV_RLAX: return fn_mask<kMask[EXTV_RLAX]>(mag);
+    case EXTV_IDREM: return fn_mask<kMask[EXTV_IDREM]>(mag);
     default: return fn_mask<kMask[EXTV_ALL]>(mag);
     }
 }
@@ -92,6 +94,7 @@ void ext_stage(bool mag, bool rot, int stage, int variant, const StageArgs& a, h
     case EXTV_SHR: stage_mask<kMask[EXTV_SHR]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_HEAT: stage_mask<kMask[EXTV_HEAT]>(mag, rot, stage, a, s, target_blocks); break;
     case EXTV_RLAX: stage_mask<kMask[EXTV_RLAX]>(mag, rot, stage, a, s, target_blocks); break;
+    case EXTV_IDREM: stage_mask<kMask[EXTV_IDREM]>(mag, rot, stage, a, s, target_blocks); break;
     default: stage_mask<kMask[EXTV_ALL]>(mag, rot, stage, a, s, target_blocks); break;
     }
 }
