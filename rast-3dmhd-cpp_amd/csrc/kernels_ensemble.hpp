@@ -75,7 +75,8 @@ __global__ void __launch_bounds__(256, WAVES)
     ensemble_stage_kernel(const StageArgs* __restrict__ args, const WorkItem* __restrict__ items,
                           int nitems)
 {
-    __shared__ __align__(16) char lds[StreamLds<T, true, ONE_BARRIER>::bytes];
+    // sized for MHD items; a hydro item's planes and ring fit inside
+    __shared__ __align__(16) char lds[StageLds<T, true, STAGE, ONE_BARRIER, DEFER>::bytes];
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const WorkItem w = items[it];
         const StageArgs& a = args[w.member];

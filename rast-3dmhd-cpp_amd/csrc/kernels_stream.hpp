@@ -228,12 +228,27 @@ struct StreamLds {
     static constexpr int NRING = ONE_BARRIER ? 4 : 3;
     static constexpr int NSTAR = ONE_BARRIER ? 3 : 2;
     static constexpr int bytes = (int)sizeof(T) * (NRING * NR + NSTAR * NS) * SHP;
+    // the mixed form's stage-1 ring of update operands (stage_item, ZLDS)
+    // follows the planes, 16-byte aligned
+    static constexpr int ring_at = (bytes + 15) & ~15;
+};
+
+// LDS of a kernel whose work items run stage_item<T, MAG', ..., STAGE,
+// ONE_BARRIER, DEFER> for MAG' up to MAG: the planes, and in the mixed form's
+// stage 1 with the deferred update and two barriers, two planes of FP64
+// update operands per thread after them.
+template <typename T, bool MAG, int STAGE, bool ONE_BARRIER, bool DEFER, bool RO_N = false>
+struct StageLds {
+    static constexpr bool ring = !std::is_same<T, double>::value && STAGE == 1 && DEFER && !ONE_BARRIER;
+    static constexpr int bytes = ring ? StreamLds<T, MAG, ONE_BARRIER, RO_N>::ring_at +
+                                            2 * (MAG ? NF_MAG : NF_HYDRO) * STX * STY * (int)sizeof(double)
+                                      : StreamLds<T, MAG, ONE_BARRIER, RO_N>::bytes;
 };
 
 // One work item of a stage: the 32 x 8 tile (tile_x, tile_y) over the
 // interior planes zb .. ze-1 (the wall planes are a separate launch: their
 // one-sided stencils from global memory would raise this kernel's register
-// budget).  lds: StreamLds::bytes of shared memory.  Stage 3 accumulates the
+// budget).  lds: StageLds::bytes of shared memory.  Stage 3 accumulates the
 // CFL terms of every point it updates into cfl.
 // S32 (with T = double): the mixed form with the off-default branches --
 // fluxes evaluated in FP64 (with the branches) from the FP32 copies of the
@@ -312,9 +327,8 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
     // instead of being read from HBM again (most of them have left L2 by
     // then: that read doubled the stage's fetched bytes).  Only with the
     // deferred update and two barriers, whose LDS leaves room for it.
-    constexpr bool ZLDS = !F64 && STAGE == 1 && DEFER && !ONE_BARRIER;
-    constexpr int ZRN = ZLDS ? 2 * NF * STX * STY : 1;
-    __shared__ double zring[ZRN];
+    constexpr bool ZLDS = StageLds<T, MAG, STAGE, ONE_BARRIER, DEFER, RO_N>::ring;
+    double* zring = reinterpret_cast<double*>(lds + StreamLds<T, MAG, ONE_BARRIER, RO_N>::ring_at);
     SAcc<T, MAG, RO_N> acc;
     T row[3] = {T(0), T(0), T(0)};
     T rinv[3] = {T(0), T(0), T(0)};
@@ -613,7 +627,7 @@ __global__ void __launch_bounds__(256, WAVES) stage_stream_kernel(StageArgs a, i
                                                                   int ntx, int nty, int xcd_remap)
 {
     __shared__ __align__(16)
-        char lds[StreamLds<T, MAG, ONE_BARRIER, EXT && (EXTM & (EXT_DH | EXT_DV)) != 0>::bytes];
+        char lds[StageLds<T, MAG, STAGE, ONE_BARRIER, DEFER, EXT && (EXTM & (EXT_DH | EXT_DV)) != 0>::bytes];
     if (domain_halted(a)) return;
     const int nb = gridDim.x;
     const int bid = blockIdx.x;
