@@ -1894,6 +1894,25 @@ static int ens_build(mhd3d_ensemble* e)
     return MHD3D_OK;
 }
 
+// The ensemble kernel variant an ensemble or a server starts with, the
+// fastest measured on MI355X (the autotuner may pick another): FP64 two
+// waves per SIMD with the deferred update; mixed one wave per SIMD, one
+// barrier per plane and the deferred update (the 40-run served trace at
+// 6488 against 6076 Mcell-steps/s with the FP64 choice).  The override
+// MHD3D_TUNE_ENS_<FP64|MIXED>=waves,sched[,chunk] replaces it.
+static void ens_default_variant(mhd3d_ensemble* e)
+{
+    e->waves = e->mixed ? 1 : 2;
+    e->sched = e->mixed ? 3 : 2;
+    const char* tv = std::getenv(e->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
+    int w = 0, sc = 0, ch = 0;
+    if (tv && std::sscanf(tv, "%d,%d,%d", &w, &sc, &ch) >= 2 && (w == 1 || w == 2) && sc >= 0 && sc <= 3) {
+        e->waves = w;
+        e->sched = sc;
+        e->chunk = ch > 0 ? ch : 0;
+    }
+}
+
 extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3d_ensemble** out)
 {
     if (!out || !descs || n < 1) return MHD3D_EINVAL;
@@ -1921,18 +1940,7 @@ extern "C" int mhd3d_ensemble_create(const mhd3d_desc* const* descs, int n, mhd3
     }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(MHD3D_EHIP, "hipStreamCreate failed");
-    // default variant: two waves per SIMD with the deferred update, the
-    // fastest measured for the FP64 and the mixed ensemble kernels on MI355X
-    // (the autotuner may pick another)
-    e->waves = 2;
-    e->sched = 2;
-    const char* tv = std::getenv(e->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
-    int w = 0, sc = 0, ch = 0;
-    if (tv && std::sscanf(tv, "%d,%d,%d", &w, &sc, &ch) >= 2 && (w == 1 || w == 2) && sc >= 0 && sc <= 3) {
-        e->waves = w;
-        e->sched = sc;
-        e->chunk = ch > 0 ? ch : 0;
-    }
+    ens_default_variant(e);
     int rc = ens_build(e);
     if (rc) return bail(rc, e->err);
     *out = e;
@@ -2424,18 +2432,7 @@ extern "C" int mhd3d_server_create(int precision, int64_t hbm_budget_bytes, int 
     s->ens = new mhd3d_ensemble();
     s->ens->owns = false;
     s->ens->mixed = precision == MHD3D_MIXED;
-    s->ens->waves = 2;
-    s->ens->sched = 2;
-    // the ensemble variant override (MHD3D_TUNE_ENS_<FP64|MIXED>=waves,sched[,chunk])
-    // applies to the server's ensemble as well
-    const char* tv = std::getenv(s->ens->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
-    int tw = 0, tsc = 0, tch = 0;
-    if (tv && std::sscanf(tv, "%d,%d,%d", &tw, &tsc, &tch) >= 2 && (tw == 1 || tw == 2) && tsc >= 0 &&
-        tsc <= 3) {
-        s->ens->waves = tw;
-        s->ens->sched = tsc;
-        s->ens->chunk = tch > 0 ? tch : 0;
-    }
+    ens_default_variant(s->ens);
     if (hipStreamCreateWithFlags(&s->ens->stream, hipStreamNonBlocking) != hipSuccess) {
         g_srv_create_error = "hipStreamCreate failed";
         mhd3d_server_destroy(s);
