@@ -1895,14 +1895,14 @@ static int ens_build(mhd3d_ensemble* e)
 }
 
 // The ensemble kernel variant an ensemble or a server starts with, the
-// fastest measured on MI355X (the autotuner may pick another): FP64 two
-// waves per SIMD with the deferred update; mixed one wave per SIMD, one
-// barrier per plane and the deferred update (the 40-run served trace at
-// 6488 against 6076 Mcell-steps/s with the FP64 choice).  The override
-// MHD3D_TUNE_ENS_<FP64|MIXED>=waves,sched[,chunk] replaces it.
+// fastest measured on MI355X for the 40-run served trace (the autotuner may
+// pick another): one wave per SIMD with the deferred update, two barriers
+// per plane in FP64 (6296 Mcell-steps/s; two waves 6186-6225, one barrier
+// 5014) and one in the mixed form (6488; two waves and two barriers 6076).
+// The override MHD3D_TUNE_ENS_<FP64|MIXED>=waves,sched[,chunk] replaces it.
 static void ens_default_variant(mhd3d_ensemble* e)
 {
-    e->waves = e->mixed ? 1 : 2;
+    e->waves = 1;
     e->sched = e->mixed ? 3 : 2;
     const char* tv = std::getenv(e->mixed ? "MHD3D_TUNE_ENS_MIXED" : "MHD3D_TUNE_ENS_FP64");
     int w = 0, sc = 0, ch = 0;
