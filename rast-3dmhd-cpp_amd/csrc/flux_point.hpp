@@ -149,7 +149,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
                                            T& FV, T& FW, T& FR, T& FT, const ExtPt* e = nullptr)
 {
     // lShr drops every FT term inside the reference's `#if !lShr` blocks
-    const bool heat_terms = !EXT || !((EXTM & EXT_SHR) && (e->flags & EXT_SHR));
+    const bool heat_terms = !EXT || !((EXTM & EXT_SHR) && (EXTM == EXT_SHR || (e->flags & EXT_SHR)));
     const Ops<T> op = make_ops<T>(c, m);
     const T ORE = T(c.ORE), OCV = T(c.OCV);
     const T ORE_C43 = T(c.ORE_C43);
