@@ -149,7 +149,7 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
                                            T& FV, T& FW, T& FR, T& FT, const ExtPt* e = nullptr)
 {
     // lShr drops every FT term inside the reference's `#if !lShr` blocks
-    const bool heat_terms = !EXT || !((EXTM & EXT_SHR) && (EXTM == EXT_SHR || (e->flags & EXT_SHR)));
+    const bool heat_terms = !EXT || !ext_on<EXTM, EXT_SHR>(e->flags);
     const Ops<T> op = make_ops<T>(c, m);
     const T ORE = T(c.ORE), OCV = T(c.OCV);
     const T ORE_C43 = T(c.ORE_C43);
@@ -238,8 +238,8 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
 
     // ---- conduction :315-319 (func_d2_mul_array_z: ((d2*g)*a[z])*s)
     tmp = T(c.OCV_REPR);
-    if (EXT && (EXTM & EXT_REM) && (e->flags & EXT_REM)) {
-        if ((EXTM & EXT_SHR) && (e->flags & EXT_SHR)) {
+    if (EXT && ext_on<EXTM, EXT_REM>(e->flags)) {
+        if (ext_on<EXTM, EXT_SHR>(e->flags)) {
             // lRem with lShr: the TTM update and WW1 = TT - TTM sit inside
             // `#if !lShr` (:297-302), so WW1 still holds P = RO*TT (:267)
             // and the static TTM stays zero, which makes HRAD zero
@@ -268,9 +268,9 @@ __device__ __forceinline__ void flux_hydro(const A& a, const Phys& c, const Poin
         FT += ((dzT * RI) * T(m.dkap)) * tmp;
     }
     // ---- RLAX relaxation :426 (func_base_extend: FT += CORRECT[z] * (RLAX*TFAC))
-    if (EXT && (EXTM & EXT_RLAX) && (e->flags & EXT_RLAX)) FT += e->corr * e->rscale;
+    if (EXT && ext_on<EXTM, EXT_RLAX>(e->flags)) FT += e->corr * e->rscale;
     // ---- embedded heat loss, iTCon 2 :443-461: FT -= (((amp*RO)*gx)*gy)*gz
-    if (EXT && (EXTM & EXT_HEAT) && (e->flags & EXT_HEAT))
+    if (EXT && ext_on<EXTM, EXT_HEAT>(e->flags))
         FT -= (((e->heat_amp * RI) * e->hx) * e->hy) * e->hz;
 
     // ---- viscous :462-472

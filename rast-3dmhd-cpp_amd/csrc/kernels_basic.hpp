@@ -249,7 +249,7 @@ __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int 
         e.cdh = X.cdh;
         e.cdv = X.cdv;
     }
-    if ((EXTM & EXT_REM) && (X.flags & EXT_REM)) {
+    if (ext_on<EXTM, EXT_REM>(X.flags)) {
         const double* t = X.means + a.g.nz;
         const double tm = t[z - 1], t0 = t[z], tp = t[z + 1];
         e.ttm[0] = tm;
@@ -261,12 +261,12 @@ __device__ __forceinline__ void ext_point(const StageArgs& a, int x, int y, int 
         h += (tp - tm) * a.c.hz * dzz * a.m.DKAPA[z];
         e.hrad = h;
     }
-    if ((EXTM & EXT_SHR) && (X.flags & EXT_SHR)) e.ft_old = X.ft[i];
-    if ((EXTM & EXT_RLAX) && (X.flags & EXT_RLAX)) {
+    if (ext_on<EXTM, EXT_SHR>(X.flags)) e.ft_old = X.ft[i];
+    if (ext_on<EXTM, EXT_RLAX>(X.flags)) {
         e.corr = X.relax[z];
         e.rscale = X.relax[a.g.nz];
     }
-    if ((EXTM & EXT_HEAT) && (X.flags & EXT_HEAT)) {
+    if (ext_on<EXTM, EXT_HEAT>(X.flags)) {
         e.heat_amp = X.heat_amp;
         e.hx = X.heat_x[x + 1];
         e.hy = X.heat_y[y + 1];

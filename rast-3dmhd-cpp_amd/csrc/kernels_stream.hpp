@@ -569,7 +569,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& a, int tile_x, int t
             if constexpr (EXT) ext_point<EXTM>(a, gx, gy, z, (int64_t)z * g.plane + go, e);
             flux_point<MAG, ROT, 0, T, !F64, EXT, EXTM>(acc, a.c, pm, o, &e);
             if constexpr (EXT && (EXTM & EXT_SHR)) {
-                if (valid && (a.x.flags & EXT_SHR)) a.x.ft[(int64_t)z * g.plane + go] = o.FT;
+                if (valid && ext_on<EXTM, EXT_SHR>(a.x.flags)) a.x.ft[(int64_t)z * g.plane + go] = o.FT;
             }
             // FP64 stage 1: Z = U, the centre values of plane z
             double uc[NF];

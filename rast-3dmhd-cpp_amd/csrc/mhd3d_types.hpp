@@ -106,6 +106,18 @@ enum : int {
 // so a run with none of these keeps the default interior kernel.
 constexpr int EXT_INTERIOR = EXT_DH | EXT_DV | EXT_REM | EXT_SHR | EXT_HEAT | EXT_RLAX;
 
+// Whether branch X is on in a kernel compiled for branch set EXTM: never
+// outside the set; always in a kernel compiled for X alone, or for lRem in
+// the iD-with-lRem set, which are launched only for runs whose interior
+// branches are exactly that set (so both paths need not be held in
+// registers); otherwise as the run's flags say.
+template <int EXTM, int X>
+__host__ __device__ constexpr bool ext_on(int flags)
+{
+    return (EXTM & X) != 0 &&
+           (EXTM == X || (X == EXT_REM && EXTM == (EXT_DH | EXT_DV | EXT_REM)) || (flags & X) != 0);
+}
+
 // Operands of the off-default branches (device pointers, host-formed scalars).
 struct ExtArgs {
     int flags = 0;
