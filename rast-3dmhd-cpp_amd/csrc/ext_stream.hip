@@ -22,12 +22,12 @@ constexpr int kMask[8] = {0,        EXT_DH | EXT_DV, EXT_REM,      EXT_SHR,
 // (double-buffered planes) or two (single-buffered).  MHD's double-buffered
 // planes take 106-114 KB of LDS, one block per CU; single-buffered, 76-82 KB,
 // so MHD kernels take two barriers where their registers then allow two
-// waves per SIMD: iD at every stage, lRem, iTCon 2 and RLAX at stages 1-2,
-// and the stage-3 kernels of those three under a two-wave register bound
-// (256 VGPRs, 36-76 B per lane spilled; measured 5-11% faster per step than
-// one block per CU without spills).  lShr and the all-branch combinations
-// stay register-bound at one block per CU, where one barrier per plane is
-// the faster.
+// waves per SIMD: iD, iD with lRem, lRem, iTCon 2 and RLAX, the stage-3
+// kernels of the last four under a two-wave register bound (239-245 VGPRs,
+// no spills, with their branch tests constant; it held them at 256 with a
+// few spills before, still faster than one block per CU).  lShr (256 VGPRs
+// and 77-101 AGPRs) and the all-branch combinations stay register-bound at
+// one block per CU, where one barrier per plane is the faster.
 template <bool MAG, int STAGE, int M>
 constexpr bool kOneBarrier = !MAG || M == kMask[EXTV_SHR] || M == kMask[EXTV_ALL];
 template <bool MAG, int STAGE, int M>
