@@ -1444,88 +1444,118 @@ __device__ __forceinline__ double relax_fradm(const RelaxArgs& r, const double* 
 
 // The relaxation's host-side logic of fluxes() (3dmhdsub.h:322-441), for a
 // reference run on P z ranks of nRz planes (rank q's local plane l is global
-// plane q*nRz + l), by one thread: the TFAC update from the wall-plane mean,
-// ALPHA from FCONM and FRADM, the per-rank trapezoid scans of CORRECT over
-// planes 1..nRz+1 (the rank's upper ghost plane with FCONM = 0 -- WW2 is
-// never written on z ghost planes -- and FRADM of the next rank's first
-// plane, 0 on the last rank, which receives none), the end values summed
-// across ranks as :405-421 do, and the exit test.  Writes out[z] = CORRECT
-// (planes 1..nPz) and out[nz] = RLAX*TFAC; endval holds P values.  `stage`
-// (1-3) marks a call inside a step: the domain then stops at once at an
-// exit (the stage's kernels do nothing); with stage 0 (mhd3d_fluxes) the
-// state records the exit and the host stops the domain after the fluxes.
-MHD_TU_LOCAL __global__ void relax_kernel(RelaxArgs r, Geom g, const double* __restrict__ TTM,
-                                          const double* __restrict__ FCONM, DevScalars* ds,
-                                          RelaxState* rs, double* __restrict__ out,
-                                          double* __restrict__ endval, int stage)
+// plane q*nRz + l): the TFAC update from the wall-plane mean, ALPHA from
+// FCONM and FRADM, the per-rank trapezoid scans of CORRECT over planes
+// 1..nRz+1 (the rank's upper ghost plane with FCONM = 0 -- WW2 is never
+// written on z ghost planes -- and FRADM of the next rank's first plane, 0
+// on the last rank, which receives none), the end values summed across
+// ranks as :405-421 do, and the exit test.  The block forms every plane's
+// ALPHA and the ZEE differences at once (they depend on no scan); thread 0
+// then runs the scans in the reference's order from them, and the block
+// writes out[z] = CORRECT (planes 1..nPz); out[nz] = RLAX*TFAC; endval holds
+// P values.  Work arrays: RELAX_WORK(nz, P) doubles, in dynamic LDS
+// (use_lds) or at wk.  `stage` (1-3) marks a call inside a step: the domain
+// then stops at once at an exit (the stage's kernels do nothing); with stage
+// 0 (mhd3d_fluxes) the state records the exit and the host stops the domain
+// after the fluxes.
+#define RELAX_WORK(nz, P) (3 * (size_t)(nz) + 4 + 2 * (size_t)(P))
+MHD_TU_LOCAL __global__ void __launch_bounds__(256)
+relax_kernel(RelaxArgs r, Geom g, const double* __restrict__ TTM, const double* __restrict__ FCONM,
+             DevScalars* ds, RelaxState* rs, double* __restrict__ out, double* __restrict__ endval, int stage,
+             double* __restrict__ wk, int use_lds)
 {
-    if (threadIdx.x != 0) return;
+    extern __shared__ double relax_lds[];
     if (stage && ds->halted) return;
-    const int nPz = g.nPz, P = r.P, nRz = r.nRz;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int nPz = g.nPz, P = r.P, nRz = r.nRz, nz = g.nz;
+    double* A = use_lds ? relax_lds : wk;   // ALPHA of plane z, 1..nPz
+    double* C = A + (nz + 1);               // CORRECT of plane z before the ranks' sums
+    double* DZ = C + (nz + 1);              // ZEE[z] - ZEE[z-1], z = 2..nPz+1
+    double* G = DZ + (nz + 2);              // ALPHA of rank q's upper ghost plane
+    double* S = G + P;                      // the sum of end values rank q adds
     const double DT = ds->dt;
     const double TIMT = r.TIMI + ds->timc;
-    if ((TIMT > r.TSTART) && (rs->TFAC > (double)1e-4f)) {
-        const double step = DT / (3.0 * r.TOFF);
-        if (r.combo == 1) {
-            if (TTM[nPz] <= rs->TTM_MAX) {
-                rs->TFAC -= step;
-            } else {
-                rs->TTM_MAX = TTM[nPz];
-                rs->TFAC += step;
-                if (2.0 < rs->TFAC) rs->TFAC = 2.0;
-            }
-        } else if (r.combo == 2) {
-            if (TTM[1] >= rs->TTM_MIN) {
-                rs->TFAC -= step;
-            } else {
-                rs->TTM_MIN = TTM[1];
-                rs->TFAC += step;
-                if (2.0 < rs->TFAC) rs->TFAC = 2.0;
-            }
-        } else {
+    const bool active = (TIMT > r.TSTART) && (rs->TFAC > (double)1e-4f);
+    if (active && r.combo != 1 && r.combo != 2) {
+        if (t == 0) {
             rs->exited = 2;
             rs->exit_stage = stage;
             if (stage) ds->halted |= HALT_EXIT;
-            return;
         }
-        // per-rank scans (the rank's sum of the ranks' end values follows)
-        for (int q = 0; q < P; ++q) {
-            const int g0 = q * nRz;
-            double aprev = 0.0, c = 0.0;
-            for (int l = 1; l <= nRz + 1; ++l) {
-                const int z = g0 + l;
-                const double fc = l <= nRz ? FCONM[z] : 0.0;
-                const double fr = (l <= nRz || q < P - 1) ? relax_fradm(r, TTM, z, nPz) : 0.0;
-                double a = 1.0 - (fc + fr) / r.FTOT;
-                a = a / (1.0 + 4.0 * fabs(a));
-                if (l > 1) c = c + 0.5 * (a + aprev) * (r.ZEE[z] - r.ZEE[z - 1]);
-                if (l <= nRz) out[z] = c;
-                aprev = a;
-            }
-            endval[q] = c;
+        return;
+    }
+    if (active) {
+        for (int z = 1 + t; z <= nPz; z += nt) {
+            double a = 1.0 - (FCONM[z] + relax_fradm(r, TTM, z, nPz)) / r.FTOT;
+            A[z] = a / (1.0 + 4.0 * fabs(a));
         }
-        for (int q = 0; q < P; ++q) {
-            double SUM = 0.0;
+        for (int z = 2 + t; z <= nPz + 1; z += nt) DZ[z] = r.ZEE[z] - r.ZEE[z - 1];
+        for (int q = t; q < P; q += nt) {
+            const int z = (q + 1) * nRz + 1;
+            const double fr = q < P - 1 ? relax_fradm(r, TTM, z, nPz) : 0.0;
+            double a = 1.0 - (0.0 + fr) / r.FTOT;
+            G[q] = a / (1.0 + 4.0 * fabs(a));
+        }
+    }
+    // every thread has read TFAC and formed its values before thread 0 goes on
+    __syncthreads();
+    if (t == 0) {
+        if (active) {
+            const double step = DT / (3.0 * r.TOFF);
             if (r.combo == 1) {
-                for (int k = 0; k < q; ++k) SUM += endval[k];
+                if (TTM[nPz] <= rs->TTM_MAX) {
+                    rs->TFAC -= step;
+                } else {
+                    rs->TTM_MAX = TTM[nPz];
+                    rs->TFAC += step;
+                    if (2.0 < rs->TFAC) rs->TFAC = 2.0;
+                }
             } else {
-                for (int k = P - 1; k >= q; --k) {
-                    SUM -= endval[k];
-                    SUM -= endval[k];
+                if (TTM[1] >= rs->TTM_MIN) {
+                    rs->TFAC -= step;
+                } else {
+                    rs->TTM_MIN = TTM[1];
+                    rs->TFAC += step;
+                    if (2.0 < rs->TFAC) rs->TFAC = 2.0;
                 }
             }
-            for (int l = 1; l <= nRz; ++l) out[q * nRz + l] += SUM;
+            // per-rank scans (the rank's sum of the ranks' end values follows)
+            for (int q = 0; q < P; ++q) {
+                const int g0 = q * nRz;
+                double aprev = 0.0, c = 0.0;
+                for (int l = 1; l <= nRz + 1; ++l) {
+                    const int z = g0 + l;
+                    const double a = l <= nRz ? A[z] : G[q];
+                    if (l > 1) c = c + 0.5 * (a + aprev) * DZ[z];
+                    if (l <= nRz) C[z] = c;
+                    aprev = a;
+                }
+                endval[q] = c;
+            }
+            for (int q = 0; q < P; ++q) {
+                double SUM = 0.0;
+                if (r.combo == 1) {
+                    for (int k = 0; k < q; ++k) SUM += endval[k];
+                } else {
+                    for (int k = P - 1; k >= q; --k) {
+                        SUM -= endval[k];
+                        SUM -= endval[k];
+                    }
+                }
+                S[q] = SUM;
+            }
         }
-    } else {
-        for (int z = 1; z <= nPz; ++z) out[z] = 0.0;
+        out[nz] = r.RLAX * rs->TFAC;
+        if (r.combo != 0 && fmod(TIMT, 25.0) <= DT && fmod(rs->icall, 3.0) == 0.0) {
+            rs->exited = 1;
+            rs->exit_stage = stage;
+            if (stage) ds->halted |= HALT_EXIT;
+        }
+        rs->icall += 1.0;
     }
-    out[g.nz] = r.RLAX * rs->TFAC;
-    if (r.combo != 0 && fmod(TIMT, 25.0) <= DT && fmod(rs->icall, 3.0) == 0.0) {
-        rs->exited = 1;
-        rs->exit_stage = stage;
-        if (stage) ds->halted |= HALT_EXIT;
-    }
-    rs->icall += 1.0;
+    __syncthreads();
+    // CORRECT, then the ranks' sum added to it (:405-421)
+    for (int z = 1 + t; z <= nPz; z += nt) out[z] = active ? C[z] + S[(z - 1) / nRz] : 0.0;
 }
 
 // Stops the domain at a recorded exit (after an mhd3d_fluxes call).

@@ -295,7 +295,8 @@ static int setup_ext(mhd3d_ctx* ctx, const mhd3d_desc* d)
     }
     if (X.flags & EXT_RLAX) {
         // FCONM [nz] | CORRECT [nz] and RLAX*TFAC | rank end values [nz] | ZEE [nz]
-        const size_t nb = 4 * (size_t)nz + 1;
+        // | relax_kernel's work arrays, when they exceed its LDS
+        const size_t nb = 4 * (size_t)nz + 1 + RELAX_WORK(nz, d->relax_nPEz);
         std::vector<double> hb(nb, 0.0);
         std::memcpy(&hb[3 * (size_t)nz + 1], d->ZEE, nz * sizeof(double));
         HIPCHK(ctx, hipMalloc(&ctx->relax_buf, nb * sizeof(double)));
@@ -1033,9 +1034,11 @@ static int run_prepass_t(mhd3d_ctx* ctx, const TI* U, int stage, hipStream_t str
             conv_flux_mean_kernel<TI><<<ctx->g.nPz, PM_T, 0, stream>>>(ctx->g, U, ctx->bg64, ctx->ext_buf + nz,
                                                                      shr, ctx->ds, stage ? 1 : 0, fconm);
         HIPCHK(ctx, hipGetLastError());
-        relax_kernel<<<1, 1, 0, stream>>>(ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs,
-                                         ctx->relax_buf + nz, ctx->relax_buf + 2 * (size_t)nz + 1,
-                                         stage);
+        const size_t wbytes = RELAX_WORK(nz, ctx->ra.P) * sizeof(double);
+        const int use_lds = wbytes <= 65536 ? 1 : 0;
+        relax_kernel<<<1, 256, use_lds ? wbytes : 0, stream>>>(
+            ctx->ra, ctx->g, ctx->ext_buf + nz, fconm, ctx->ds, ctx->rs, ctx->relax_buf + nz,
+            ctx->relax_buf + 2 * (size_t)nz + 1, stage, ctx->relax_buf + 4 * (size_t)nz + 1, use_lds);
         HIPCHK(ctx, hipGetLastError());
     }
     return MHD3D_OK;
