@@ -12,8 +12,10 @@ MHD3D_EEXIT in the sixth step's first fluxes() call (the state is that
 after five steps), and -- with the call count shifted by mhd3d_fluxes --
 in the second stage (the state is then the stage-1 output); other walls
 stop as soon as the relaxation starts; set_state clears an exit and the
-domain runs on with the statics it has.  FP64 form: bit identity, 1e-12
-stated as the tolerance.
+domain runs on with the statics it has.  Deep grids (96 planes on 8
+ranks, 2800 planes on 4) take the relaxation pass's work arrays from LDS
+and from global memory.  FP64 form: bit identity, 1e-12 stated as the
+tolerance.
 """
 import dataclasses
 
@@ -119,6 +121,19 @@ def test_wide_planes(n):
     assert dom.step(5) == dts[-1]
     _same(dom, orc, n)
     _same_statics(dom, orc, n)
+
+
+@pytest.mark.parametrize("n,P", [((4, 4, 96), 8), ((4, 4, 2800), 4)])
+def test_deep_grids(n, P):
+    """Many planes and ranks: the relaxation pass's work arrays in LDS (96
+    planes on 8 ranks) and, past 64 KB of them, in global memory (2800
+    planes on 4 ranks)."""
+    print(f"relax deep grid {n} P={P}", flush=True)
+    desc, dom, orc = _pair(_cfg(n, "it0", P=P), seed=13)
+    dts = [orc.step() for _ in range(3)]
+    assert dom.step(3) == dts[-1]
+    _same(dom, orc, (n, P))
+    _same_statics(dom, orc, (n, P))
 
 
 @pytest.mark.parametrize("case", ["it0_mhd", "it1_rem"])
